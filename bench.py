@@ -1,0 +1,149 @@
+"""Throughput of the plain bisection sampler on one MI355X.
+
+Workload (one GPU): 3 He atoms in 3D with the LM2M2 pair potential, mass
+4.0026 u, N = 1024 slices, beta = 10 1/K, 16384 chains, bisection level 6,
+FP32 coordinates, xorshift128 streams, operators virial energy and mean pair
+distance.  One step = one launch of 32 sweeps over every chain followed by
+one measurement.  value = slice-updates/s (one slice-update = one interior
+slice of a segment, all 9 coordinates, proposed by one bisection move).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, S, W, BETA, DOF = 1024, 6, 16384, 10.0, 9
+SWEEPS_PER_STEP = 32
+SEG = 1 << S
+UPDATES_PER_SWEEP = W * (N // SEG) * (SEG - 1)
+# algorithmic HBM bytes per slice-update: a segment reads 2^S + 1 slices and
+# writes 2^S - 1 slices of DOF FP32 coordinates, i.e. 8 DOF 2^S / (2^S - 1)
+BYTES_PER_UPDATE = 8.0 * DOF * SEG / (SEG - 1)
+# VALU operations per slice-update as written (transcendentals and divides
+# counted once): 26 per coordinate + 2 potential evaluations of 3 LM2M2 pairs
+VALU_PER_UPDATE = 26 * DOF + 2 * (39 * 3 + 1)
+HBM_PEAK_GBS = 8000.0
+VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s, one FP32 op per lane per clock
+
+
+def make_kernel(steps_per_launch, seed):
+    from feynsimul_amd.kernel_args import KernelArgs
+    from feynsimul_amd.kernel import PIMCKernel
+    from feynsimul_amd.physical_systems import Lm2m2Cluster
+    system = Lm2m2Cluster(3)
+    ka = KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=BETA, S=S, enableOperator=True,
+                    enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                    enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                    enableGlobalOldPath=False, enableBins=False, enableDouble=False,
+                    metroStepsPerOperatorRun=steps_per_launch, operatorRuns=1, nbrOfWalkersPerWorkGroup=1,
+                    operators=(system.energyOp, system.meanPairDistanceOp), seed=seed, sampler="wave")
+    return system, PIMCKernel(ka)
+
+
+def initial_paths(system, rng, walkers=W, sigma=3.0):
+    import numpy as np
+    pos = system.initialPositions(walkers, rng, side=3.5, sigma=sigma)
+    return np.repeat(pos[:, :, None], N, axis=2).reshape(walkers, DOF * N)
+
+
+def cpu_baseline(target_s=12.0):
+    """Reference-order CPU restatement (oracle) on this host's cores, bounded sample."""
+    import numpy as np
+    from oracle import oracle as O
+    from feynsimul_amd.physical_systems import Lm2m2Cluster
+    system = Lm2m2Cluster(3)
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    lib = O.build(system, N, BETA, operators=(system.energyOp, system.meanPairDistanceOp), native=True)
+    rng = np.random.default_rng(1)
+    walkers = max(cores * 4, 16)
+    sweeps = 2
+    while True:
+        paths = np.ascontiguousarray(initial_paths(system, rng, walkers).astype(np.float32))
+        cfg = O.make_cfg(N, S, DOF, walkers, BETA, 1, sweeps, nOps=2)
+        seeds = O.xorshift_rows(3, 0, cfg.nThreads + 1)
+        t0 = time.perf_counter()
+        O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=cores)
+        dt = time.perf_counter() - t0
+        if dt > target_s / 4 or walkers * sweeps > 1 << 16:
+            break
+        sweeps *= 2
+    upd = walkers * (N // SEG) * (SEG - 1) * sweeps
+    return {"value": upd / dt, "unit": "slice-updates/s", "cores": cores, "kind": "port",
+            "sample": "%d chains x %d sweeps of the same workload, reference work-item order, FP32, "
+                      "gcc -O3 -march=native -fopenmp, %.1f s" % (walkers, sweeps, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: chains are independent and this engine runs on a single device; "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    torch.cuda.set_device(0)
+    system, k = make_kernel(SWEEPS_PER_STEP, seed=12345)
+    rng = np.random.default_rng(0)
+    k.setPaths(initial_paths(system, rng))
+    for _ in range(args.warmup):
+        k.run()
+    stream = k._stream
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        k.run(wait=False)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    launch_ms = [a.elapsed_time(b) for a, b in ev]
+    kern_s = float(np.mean(launch_ms)) * 1e-3
+    updates = UPDATES_PER_SWEEP * SWEEPS_PER_STEP
+    value = updates * args.steps / wall
+    achieved = BYTES_PER_UPDATE * updates / kern_s / 1e9
+    E = float(k.getOperators()[0].mean())
+    line = {
+        "metric": "path-slice updates/s (plain bisection sampler)",
+        "value": value,
+        "unit": "slice-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded Gaussian blobs (sigma 3 A) around a random triangle of side 3.5 A",
+        "config": {"workload": "LM2M2 He3, 3D Cartesian, N=1024, beta=10/K, 16384 chains, S=6, xorshift128, "
+                               "32 sweeps + 1 measurement per step", "sampler": "wave",
+                   "chains": W, "slices": N, "bisection_level": S},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "fs_wave_sweep", "avg_launch_ms": kern_s * 1e3,
+                     "bytes_per_update": BYTES_PER_UPDATE, "updates_per_launch": updates},
+        "valu": {"ops_per_update": VALU_PER_UPDATE, "achieved": VALU_PER_UPDATE * updates / kern_s,
+                 "peak": VALU_PEAK, "frac": VALU_PER_UPDATE * updates / kern_s / VALU_PEAK},
+        "acceptance": k.getAcceptanceRate(),
+        "energy_K": E,
+    }
+    if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,134 @@
+"""ctypes binding of libfeynsimul.so (declarations in include/feynsimul.h).
+
+The library is the only compute path: importing this module fails loudly if
+it has not been built (``python -c "import __graft_entry__ as g; g.build()"``
+or ``make -C feynsimul_amd/csrc``).
+"""
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfeynsimul.so")
+
+FS_ABI_VERSION = 1
+FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3, 4
+FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
+
+EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_build_log",
+           "fs_source", "fs_run", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
+           "fs_ranlux_draw")
+
+
+class FsConfig(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_int32), ("sampler", ctypes.c_int32), ("N", ctypes.c_int32),
+        ("S", ctypes.c_int32), ("DOF", ctypes.c_int32), ("walkers_per_group", ctypes.c_int32),
+        ("n_walkers", ctypes.c_int64), ("parallelize_path", ctypes.c_int32), ("local_size", ctypes.c_int32),
+        ("double_precision", ctypes.c_int32), ("ranlux", ctypes.c_int32), ("ranlux_lux", ctypes.c_int32),
+        ("fast_math", ctypes.c_int32), ("ranlux_int_range", ctypes.c_uint64), ("n_ops", ctypes.c_int32),
+        ("n_corr", ctypes.c_int32), ("cor_lags", ctypes.c_int32), ("bins", ctypes.c_int32),
+        ("bin_res", ctypes.c_int32), ("waves_per_block", ctypes.c_int32), ("beta", ctypes.c_double),
+        ("x_min", ctypes.c_double), ("x_max", ctypes.c_double), ("op_norm", ctypes.c_double)]
+
+
+class FsBuffers(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "paths", "accepts", "seeds", "seeds_snap", "rl", "grl", "old_paths", "op_means", "op_totals", "series",
+        "cor_means", "bins", "pot_cache")]
+
+
+class FsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libfeynsimul.so is not built (%s); run __graft_entry__.build() or "
+                          "make -C feynsimul_amd/csrc" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, u64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64
+    L.fs_abi_version.restype = ctypes.c_int
+    L.fs_last_error.restype = ctypes.c_char_p
+    L.fs_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
+    L.fs_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp)]
+    L.fs_destroy.argtypes = [vp]
+    L.fs_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.fs_source.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.fs_run.argtypes = [vp, ctypes.POINTER(FsBuffers), i32, i32, i32, vp]
+    L.fs_eval_system.argtypes = [vp, vp, vp, vp, i64, vp]
+    L.fs_seed_xorshift.argtypes = [vp, i64, u64, i64, vp]
+    L.fs_ranlux_init.argtypes = [vp, vp, i64, vp]
+    L.fs_xorshift_draw.argtypes = [vp, vp, vp, vp, i64, i32, vp]
+    L.fs_ranlux_draw.argtypes = [vp, vp, vp, i64, i32, i32, vp]
+    for name in EXPORTS:
+        if name not in ("fs_abi_version", "fs_last_error"):
+            getattr(L, name).restype = ctypes.c_int
+    if L.fs_abi_version() != FS_ABI_VERSION:
+        raise ImportError("libfeynsimul.so ABI %d, expected %d" % (L.fs_abi_version(), FS_ABI_VERSION))
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != FS_OK:
+        raise FsError(rc, lib().fs_last_error().decode(errors="replace"))
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    check(lib().fs_device_count(ctypes.byref(n)))
+    return n.value
+
+
+def _text(fn, handle):
+    need = ctypes.c_size_t(0)
+    check(fn(handle, None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value)
+    check(fn(handle, buf, need.value, ctypes.byref(need)))
+    return buf.value.decode(errors="replace")
+
+
+class Engine:
+    """One compiled sampler module (owner of the fs_engine handle)."""
+
+    def __init__(self, cfg: FsConfig, system_src: str, compile_only=False):
+        self._h = ctypes.c_void_p()
+        L = lib()
+        rc = L.fs_create(ctypes.byref(cfg), system_src.encode(), int(compile_only), ctypes.byref(self._h))
+        if rc != FS_OK:
+            msg = L.fs_last_error().decode(errors="replace")
+            self.build_log = _text(L.fs_build_log, self._h) if self._h.value else ""
+            if self._h.value:
+                L.fs_destroy(self._h)
+                self._h = ctypes.c_void_p()
+            raise FsError(rc, msg)
+        self.cfg = cfg
+        self.build_log = _text(L.fs_build_log, self._h)
+
+    @property
+    def source(self):
+        return _text(lib().fs_source, self._h)
+
+    def run(self, bufs: FsBuffers, operator_runs, metro_steps, series_offset, stream):
+        check(lib().fs_run(self._h, ctypes.byref(bufs), operator_runs, metro_steps, series_offset, stream))
+
+    def eval_system(self, x_ptr, pot_ptr, ops_ptr, n, stream):
+        check(lib().fs_eval_system(self._h, x_ptr, pot_ptr, ops_ptr, n, stream))
+
+    def close(self):
+        if self._h.value:
+            lib().fs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

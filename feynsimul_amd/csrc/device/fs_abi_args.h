@@ -1,0 +1,27 @@
+// fs_abi_args.h -- the kernel-argument block shared by the host engine
+// (compiled by hipcc) and the per-system sampler module (compiled by hiprtc).
+// Plain C layout: pointers and 32/64-bit scalars only.
+#pragma once
+
+struct fs_ranlux;
+
+struct fs_kargs {
+    void *paths;          // FLOAT_TYPE [walkers][DOF][N]  (DOF-major per walker)
+    unsigned *accepts;    // exact: [threads]; wave: [walkers]
+    unsigned *seeds;      // xorshift rows (uint32 x4). exact: [threads+1]; wave: lanes [walkers*64] then groups [walkers]
+    unsigned *seedsSnap;  // exact: copy of `seeds` taken before the launch (group rows are read from it)
+    fs_ranlux *rl;        // RANLUX states. exact: [threads]; wave: [walkers*64]
+    fs_ranlux *grl;       // RANLUX group states. exact: [threads]; wave: [walkers]
+    void *oldPaths;       // exact: FLOAT_TYPE [threads][(2^S-1)*DOF]
+    void *opMeans;        // exact: FLOAT_TYPE [threads*nOps]; wave: double [walkers][nOps]
+    double *opTotals;     // wave, optional: running FP64 (sum, compensation) [walkers][nOps][2]
+    float *series;        // wave, optional: per-measurement slice means [measurement][walkers][nOps]
+    void *corMeans;       // exact: FLOAT_TYPE [walkers][nCorr][N/2]; wave: double [walkers][nCorr][lags][2]
+    unsigned *bins;       // histogram counts
+    void *potCache;       // wave, optional: FLOAT_TYPE [walkers][N] potential at every slice
+    long long nWalkers;
+    int operatorRuns;
+    int metroSteps;
+    int seriesOffset;     // index of the first measurement of this launch in `series`
+    int flags;            // bit 0: potCache valid on entry
+};

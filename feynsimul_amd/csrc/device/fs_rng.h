@@ -1,0 +1,286 @@
+// fs_rng.h -- device random number generators of the PIMC engine (gfx950).
+//
+// Shared by the ahead-of-time utility kernels (fs_rng_kernels.hip) and by the
+// sampler template that hiprtc compiles per system (fs_sampler.h).
+//
+//  * xorshift128 (Marsaglia 2003) with the reference's conversions,
+//    FeynSimul/GPUsrc/kernel.c:214-239.
+//  * RANLUX as implemented by RANLUXCL v1.3.1 (24-bit subtract-with-borrow,
+//    lags 10 and 24, luxury skipping in the "planar" and "planar shift"
+//    schemes, ranluxcl_init seeding), examples/pyopencl-ranluxcl.cl:265-725,
+//    893-920, and the per-launch float4 buffering of kernel.c:180-207.
+//  * splitmix64 (Steele, Lea and Flood 2014) to derive per-stream seeds.
+//
+// The RANLUX lag table is a 24-entry register array; every index below is a
+// compile-time constant so the state never leaves VGPRs.
+#pragma once
+
+typedef unsigned int fs_u32;
+typedef unsigned long long fs_u64;
+
+#ifndef FS_DEV
+#define FS_DEV __device__ __forceinline__
+#endif
+
+// ---------------------------------------------------------------------------
+// splitmix64: stream row r is seeded from the generator state seed + r.
+// ---------------------------------------------------------------------------
+FS_DEV fs_u64 fs_splitmix64_next(fs_u64 &state)
+{
+    state += 0x9E3779B97F4A7C15ull;
+    fs_u64 z = state;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------------------
+// xorshift128
+// ---------------------------------------------------------------------------
+struct fs_xs128 {
+    fs_u32 x, y, z, w;
+};
+
+FS_DEV void fs_xs_step(fs_xs128 &s)
+{
+    fs_u32 t = s.x ^ (s.x << 11);
+    s.x = s.y;
+    s.y = s.z;
+    s.z = s.w;
+    s.w = s.w ^ (s.w >> 19) ^ (t ^ (t >> 8));
+}
+
+FS_DEV fs_u32 fs_xs_int(fs_xs128 &s)
+{
+    fs_xs_step(s);
+    return s.w;
+}
+
+// FP32: w rounded to float times the literal 2.328306437080797e-10 read as a
+// float, which is exactly 2^-32, so the value lies in [0, 1].
+FS_DEV float fs_xs_float32(fs_xs128 &s)
+{
+    fs_xs_step(s);
+    return (float)s.w * 0x1p-32f;
+}
+
+// FP64: w times the double literal 2.328306437080797e-10 (about 1/(2^32-1)).
+FS_DEV double fs_xs_float64(fs_xs128 &s)
+{
+    fs_xs_step(s);
+    return (double)s.w * 2.328306437080797e-10;
+}
+
+template <typename F> FS_DEV F fs_xs_float(fs_xs128 &s);
+template <> FS_DEV float fs_xs_float<float>(fs_xs128 &s) { return fs_xs_float32(s); }
+template <> FS_DEV double fs_xs_float<double>(fs_xs128 &s) { return fs_xs_float64(s); }
+
+// ---------------------------------------------------------------------------
+// RANLUX
+// ---------------------------------------------------------------------------
+// In-memory layout (112 bytes): s01..s24, carry, dummy, in24, stepnr -- the
+// field order of ranluxcl_state_t, so a state table is interchangeable.
+struct fs_ranlux {
+    float s[24];
+    float carry;
+    float dummy;
+    int in24;
+    int stepnr;
+};
+
+#define FS_TWOM24 0x1p-24f
+#define FS_TWOM12 0x1p-12f
+
+// Luxury level -> values skipped after each 24 produced.
+__host__ __device__ constexpr int fs_ranlux_nskip(int lux)
+{
+    return lux == 0 ? 0 : lux == 1 ? 24 : lux == 2 ? 76 : lux == 3 ? 200 : lux == 4 ? 380 : lux - 24;
+}
+
+// One SWB step at cycle position T: x = s[lag 10] - s[lag 24] - carry, taken
+// mod 1 with a borrow; the result replaces the lag-24 entry.  Values below
+// 2^-12 get 24 more low bits from the next lag-10 entry (never 0 returned).
+template <int T> FS_DEV float fs_rl_step(fs_ranlux &r)
+{
+    constexpr int tgt = 23 - T;
+    constexpr int lag10 = (9 - T + 24) % 24;
+    constexpr int lag9 = (8 - T + 24) % 24;
+    float uni = r.s[lag10] - r.s[tgt] - r.carry;
+    const bool borrow = uni < 0.0f;
+    uni = borrow ? uni + 1.0f : uni;
+    r.carry = borrow ? FS_TWOM24 : 0.0f;
+    r.s[tgt] = uni;
+    float out = uni;
+    if (uni < FS_TWOM12) {
+        out += FS_TWOM24 * r.s[lag9];
+        if (out == 0.0f) out = FS_TWOM24 * FS_TWOM24;
+    }
+    return out;
+}
+
+// Steps [FIRST, LAST) of one cycle, outputs discarded.
+template <int FIRST, int LAST> FS_DEV void fs_rl_run(fs_ranlux &r)
+{
+    if constexpr (FIRST < LAST) {
+        (void)fs_rl_step<FIRST>(r);
+        fs_rl_run<FIRST + 1, LAST>(r);
+    }
+}
+
+template <int T0> FS_DEV void fs_rl_quad(fs_ranlux &r, float &a, float &b, float &c, float &d)
+{
+    a = fs_rl_step<T0 + 0>(r);
+    b = fs_rl_step<T0 + 1>(r);
+    c = fs_rl_step<T0 + 2>(r);
+    d = fs_rl_step<T0 + 3>(r);
+}
+
+// Luxury skip of NSKIP values starting at the current cycle position.
+template <int NSKIP> FS_DEV void fs_rl_skip(fs_ranlux &r)
+{
+    // initial skips finish the current cycle
+    switch (r.stepnr) {
+    case 4: fs_rl_run<4, 24>(r); break;
+    case 8: fs_rl_run<8, 24>(r); break;
+    case 12: fs_rl_run<12, 24>(r); break;
+    case 16: fs_rl_run<16, 24>(r); break;
+    case 20: fs_rl_run<20, 24>(r); break;
+    default: break;
+    }
+    const int initial = r.stepnr ? 24 - r.stepnr : 0;
+    const int cycles = (NSKIP - initial) / 24;
+    const int rest = NSKIP - initial - cycles * 24;
+    for (int c = 0; c < cycles; c++) fs_rl_run<0, 24>(r);
+    if (rest > 0) fs_rl_run<0, 4>(r);
+    if (rest > 4) fs_rl_run<4, 8>(r);
+    if (rest > 8) fs_rl_run<8, 12>(r);
+    if (rest > 12) fs_rl_run<12, 16>(r);
+    if (rest > 16) fs_rl_run<16, 20>(r);
+    r.stepnr = rest;
+}
+
+// ranluxcl32: four values in (0, 1).
+template <int NSKIP> FS_DEV void fs_rl_next4(fs_ranlux &r, float out[4])
+{
+    const int t0 = r.stepnr;
+    switch (t0) {
+    case 0: fs_rl_quad<0>(r, out[0], out[1], out[2], out[3]); break;
+    case 4: fs_rl_quad<4>(r, out[0], out[1], out[2], out[3]); break;
+    case 8: fs_rl_quad<8>(r, out[0], out[1], out[2], out[3]); break;
+    case 12: fs_rl_quad<12>(r, out[0], out[1], out[2], out[3]); break;
+    case 16: fs_rl_quad<16>(r, out[0], out[1], out[2], out[3]); break;
+    default: fs_rl_quad<20>(r, out[0], out[1], out[2], out[3]); break;
+    }
+    r.stepnr = (t0 == 20) ? 0 : t0 + 4;
+    if constexpr (NSKIP % 24 == 0) {
+        if constexpr (NSKIP > 0) {
+            if (t0 == 20) fs_rl_skip<NSKIP>(r);
+        }
+    } else {
+        r.in24 += 4;
+        if (r.in24 == 24) {
+            r.in24 = 0;
+            fs_rl_skip<NSKIP>(r);
+        }
+    }
+}
+
+// ranluxcl64: two float4 draws make four 48-bit doubles.
+template <int NSKIP> FS_DEV void fs_rl_next4_double(fs_ranlux &r, double out[4])
+{
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        float v[4];
+        fs_rl_next4<NSKIP>(r, v);
+        if (v[0] == FS_TWOM24 * FS_TWOM24) v[0] = 0.0f;
+        if (v[2] == FS_TWOM24 * FS_TWOM24) v[2] = 0.0f;
+        out[2 * h + 0] = (double)v[0] + (double)v[1] / 16777216;
+        out[2 * h + 1] = (double)v[2] + (double)v[3] / 16777216;
+    }
+}
+
+// ranluxcl_init: xorshift64 (13, 7, 17) expands the seed into 24 fractions of
+// 24 bits each, then 32 cycles of warm-up.
+FS_DEV void fs_rl_init(fs_u64 x, fs_ranlux &r)
+{
+    const float inv = 0x1p-24f;
+#pragma unroll
+    for (int blk = 0; blk < 3; blk++) {
+        fs_u64 a, b, c;
+        x ^= (x << 13); x ^= (x >> 7); x ^= (x << 17); a = x;
+        x ^= (x << 13); x ^= (x >> 7); x ^= (x << 17); b = x;
+        x ^= (x << 13); x ^= (x >> 7); x ^= (x << 17); c = x;
+        r.s[8 * blk + 0] = (float)(a >> 40) * inv;
+        r.s[8 * blk + 1] = (float)((a & 0x000000FFFFFFFFFFull) >> 16) * inv;
+        r.s[8 * blk + 2] = (float)(((a & 0x000000000000FFFFull) << 8) + (b >> 56)) * inv;
+        r.s[8 * blk + 3] = (float)((b & 0x00FFFFFFFFFFFFFFull) >> 32) * inv;
+        r.s[8 * blk + 4] = (float)((b & 0x00000000FFFFFFFFull) >> 8) * inv;
+        r.s[8 * blk + 5] = (float)(((b & 0x00000000000000FFull) << 16) + (c >> 48)) * inv;
+        r.s[8 * blk + 6] = (float)((c & 0x0000FFFFFFFFFFFFull) >> 24) * inv;
+        r.s[8 * blk + 7] = (float)(c & 0x0000000000FFFFFFull) * inv;
+    }
+    r.dummy = 0.0f;
+    r.in24 = 0;
+    r.stepnr = 0;
+    r.carry = (r.s[23] == 0.0f) ? FS_TWOM24 : 0.0f;
+    for (int i = 0; i < 32; i++) fs_rl_run<0, 24>(r);
+}
+
+FS_DEV void fs_rl_load(const fs_ranlux *__restrict__ tab, long idx, fs_ranlux &r)
+{
+    const float4 *p = reinterpret_cast<const float4 *>(tab + idx);
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        float4 v = p[q];
+        r.s[4 * q + 0] = v.x;
+        r.s[4 * q + 1] = v.y;
+        r.s[4 * q + 2] = v.z;
+        r.s[4 * q + 3] = v.w;
+    }
+    float4 t = p[6];
+    r.carry = t.x;
+    r.dummy = t.y;
+    r.in24 = __float_as_int(t.z);
+    r.stepnr = __float_as_int(t.w);
+}
+
+FS_DEV void fs_rl_store(fs_ranlux *__restrict__ tab, long idx, const fs_ranlux &r)
+{
+    float4 *p = reinterpret_cast<float4 *>(tab + idx);
+#pragma unroll
+    for (int q = 0; q < 6; q++) p[q] = make_float4(r.s[4 * q], r.s[4 * q + 1], r.s[4 * q + 2], r.s[4 * q + 3]);
+    p[6] = make_float4(r.carry, r.dummy, __int_as_float(r.in24), __int_as_float(r.stepnr));
+}
+
+// Per-launch buffered draw (kernel.c:180-207): refill four values whenever the
+// launch-local counter is a multiple of four.
+template <typename F, int NSKIP> struct fs_rl_wrap {
+    fs_ranlux st;
+    F buf[4];
+    fs_u32 cnt;
+
+    FS_DEV F next_float()
+    {
+        if ((cnt & 3u) == 0u) {
+            if constexpr (sizeof(F) == 8) {
+                double v[4];
+                fs_rl_next4_double<NSKIP>(st, v);
+                buf[0] = v[0]; buf[1] = v[1]; buf[2] = v[2]; buf[3] = v[3];
+            } else {
+                float v[4];
+                fs_rl_next4<NSKIP>(st, v);
+                buf[0] = v[0]; buf[1] = v[1]; buf[2] = v[2]; buf[3] = v[3];
+            }
+        }
+        const fs_u32 k = (cnt++) & 3u;
+        // select rather than index so the buffer stays in registers
+        F v = buf[0];
+        v = (k == 1u) ? buf[1] : v;
+        v = (k == 2u) ? buf[2] : v;
+        v = (k == 3u) ? buf[3] : v;
+        return v;
+    }
+
+    // ranluxIntWrapper: (uint)(u * float(ranluxIntMax + 1)).
+    FS_DEV fs_u32 next_int(F scale) { return (fs_u32)(next_float() * scale); }
+};

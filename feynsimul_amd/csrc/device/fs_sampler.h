@@ -1,0 +1,482 @@
+// fs_sampler.h -- Metropolis bisection samplers of the PIMC engine (gfx950).
+//
+// Compiled by hiprtc once per (system, N, S, DOF, operators, flags), after a
+// generated prelude that defines the configuration macros below and the
+// system section (the user's potential / operator / correlator strings
+// wrapped as fs_potential_v, fs_operator_v, fs_correlator_v).  This mirrors
+// the reference, which pastes the same strings into its OpenCL template and
+// rebuilds per configuration (FeynSimul/kernel.py:253-382).
+//
+// Configuration macros: FS_N, FS_S, FS_DOF, FS_NOPS, FS_NCORR, FS_CORLAGS,
+// FLOAT_TYPE, FS_DOUBLE, FS_RANLUX, FS_NSKIP, FS_ISCALE, FS_BINS, FS_BINRES,
+// FS_XMIN, FS_XMAX, FS_INVBIN, FS_OPNORM, FS_PARALLEL, FS_W, FS_LOCAL,
+// FS_WPB (waves per block of the wave sampler), FS_SAMPLER (0 exact, 1 wave).
+//
+// Two samplers share the move:
+//
+//  fs_exact_sweep -- one work-item per reference work-item, with the
+//    reference's work-item numbering, stream rows, draw order and
+//    accumulation order (kernel.c:514-837; geometry kernel.py:192-211).  It
+//    is the parity anchor against the CPU restatement of the reference.
+//
+//  fs_wave_sweep -- the throughput sampler.  One wavefront owns one walker
+//    for the whole launch; a block of 64 consecutive slices is spread one
+//    slice per lane, so the 2^S-1 midpoints of a segment, the potential
+//    evaluations and the action sum all run in parallel across lanes and
+//    the segment never leaves registers.  Each lane has its own stream;
+//    the tiling offset of every step comes from a per-walker group stream
+//    (the ParallelizePath scheme of kernel.c:713-716).  The proposal and
+//    the acceptance rule are the reference's; only the assignment of
+//    random numbers differs.
+#pragma once
+
+#include "fs_abi_args.h"
+#include "fs_rng.h"
+
+typedef FLOAT_TYPE fs_f;
+
+#define FS_SEG (1 << FS_S)
+#define FS_B (FS_N < 64 ? FS_N : 64)
+#define FS_NB (FS_N / FS_B)
+#define FS_KPAIR ((FS_DOF + 1) / 2)
+#define FS_KDRAW (2 * FS_KPAIR)
+#define FS_NOPS1 (FS_NOPS > 0 ? FS_NOPS : 1)
+#define FS_NCORR1 (FS_NCORR > 0 ? FS_NCORR : 1)
+
+#if FS_SEG > FS_N
+#error "2^S must not exceed N"
+#endif
+
+// ---------------------------------------------------------------------------
+// Random streams as the samplers see them
+// ---------------------------------------------------------------------------
+struct fs_stream {
+#if FS_RANLUX
+    fs_rl_wrap<fs_f, FS_NSKIP> g;
+    FS_DEV void load(const fs_ranlux *tab, long idx)
+    {
+        fs_rl_load(tab, idx, g.st);
+        g.cnt = 0u;
+    }
+    FS_DEV void store(fs_ranlux *tab, long idx) const { fs_rl_store(tab, idx, g.st); }
+    FS_DEV fs_f uniform() { return g.next_float(); }
+    FS_DEV fs_u32 integer() { return g.next_int((fs_f)FS_ISCALE); }
+#else
+    fs_xs128 g;
+    FS_DEV void load(const fs_u32 *rows, long idx)
+    {
+        const uint4 v = reinterpret_cast<const uint4 *>(rows)[idx];
+        g.x = v.x; g.y = v.y; g.z = v.z; g.w = v.w;
+    }
+    FS_DEV void store(fs_u32 *rows, long idx) const
+    {
+        reinterpret_cast<uint4 *>(rows)[idx] = make_uint4(g.x, g.y, g.z, g.w);
+    }
+    FS_DEV fs_f uniform() { return fs_xs_float<fs_f>(g); }
+    FS_DEV fs_u32 integer() { return fs_xs_int(g); }
+#endif
+};
+
+// sigmaN[n] = sqrt(2^S eps / (2 2^n)), the Levy midpoint width (kernel.c:605-619).
+FS_DEV fs_f fs_sigma(int n)
+{
+    return sqrt(((fs_f)FS_SEG * eps / ((fs_f)2.0 * (fs_f)(1 << n))));
+}
+
+FS_DEV void fs_gather(const fs_f *__restrict__ path, int s, fs_f x[FS_DOF])
+{
+#pragma unroll
+    for (int d = 0; d < FS_DOF; d++) x[d] = path[d * FS_N + s];
+}
+
+// DOF-dimensional grid histogram over raw coordinates (kernel.c:478-506).
+FS_DEV void fs_histogram(const fs_f x[FS_DOF], fs_u32 *__restrict__ bins)
+{
+#if FS_BINS
+    int idx = 0, inc = 1;
+#pragma unroll
+    for (int p = 0; p < FS_DOF; p++) {
+        if (x[p] < (fs_f)FS_XMIN || x[p] >= (fs_f)FS_XMAX) return;
+        idx += (int)((fs_u32)((x[p] - (fs_f)FS_XMIN) * (fs_f)FS_INVBIN)) * inc;
+        inc *= FS_BINRES;
+    }
+    atomicAdd(bins + idx, 1u);
+#endif
+}
+
+// ===========================================================================
+// fs_exact_sweep
+// ===========================================================================
+#if FS_SAMPLER == 0
+FS_DEV void fs_exact_move(fs_f *__restrict__ path, fs_f *__restrict__ oldPath, fs_u32 startPoint,
+                          fs_stream &st, fs_u32 &accepts)
+{
+    fs_f x[FS_DOF];
+    fs_f actionOld = (fs_f)0.0;
+    for (int i = 1; i < FS_SEG; i++) {
+        int iCurr = i + (int)startPoint;
+        if (iCurr >= FS_N) iCurr -= FS_N;
+        fs_gather(path, iCurr, x);
+#pragma unroll
+        for (int d = 0; d < FS_DOF; d++) oldPath[(i - 1) * FS_DOF + d] = x[d];
+        actionOld += eps * fs_potential_v(x);
+    }
+    for (int n = 1; n <= FS_S; n++) {
+        const fs_f sig = fs_sigma(n);
+        for (int i = 0; i < (1 << (n - 1)); i++) {
+            fs_u32 pm = (fs_u32)((1 + 2 * i) << (FS_S - n)) + startPoint;
+            fs_u32 pl = (fs_u32)(i << (FS_S - n + 1)) + startPoint;
+            fs_u32 pr = (fs_u32)((i + 1) << (FS_S - n + 1)) + startPoint;
+            if (pm >= (fs_u32)FS_N) pm -= FS_N;
+            if (pl >= (fs_u32)FS_N) pl -= FS_N;
+            if (pr >= (fs_u32)FS_N) pr -= FS_N;
+#pragma unroll
+            for (int d = 0; d < FS_DOF; d++) {
+                const fs_f u = st.uniform();
+                const fs_f v = st.uniform();
+                fs_f rz = (fs_f)0.0;
+                if (v != (fs_f)0.0) rz = sig * sqrt(-2.0 * log(v)) * sin(2.0 * M_PI * u);
+                path[pm + d * FS_N] = 0.5 * (path[pl + d * FS_N] + path[pr + d * FS_N]) + rz;
+            }
+        }
+    }
+    fs_f actionNew = (fs_f)0.0;
+    for (int i = 1; i < FS_SEG; i++) {
+        int iCurr = i + (int)startPoint;
+        if (iCurr >= FS_N) iCurr -= FS_N;
+        fs_gather(path, iCurr, x);
+        actionNew += eps * fs_potential_v(x);
+    }
+    if (exp(actionOld - actionNew) < st.uniform()) {
+        for (int i = 1; i < FS_SEG; i++) {
+            int iCurr = i + (int)startPoint;
+            if (iCurr >= FS_N) iCurr -= FS_N;
+#pragma unroll
+            for (int d = 0; d < FS_DOF; d++) path[d * FS_N + iCurr] = oldPath[(i - 1) * FS_DOF + d];
+        }
+    } else {
+        accepts++;
+    }
+}
+
+extern "C" __global__ void fs_exact_sweep(fs_kargs a)
+{
+#if FS_PARALLEL
+    // local (T, W), global ((walkers/W) T, W); lid0 varies fastest.
+    constexpr int T = FS_N >> FS_S;
+    const int lid0 = threadIdx.x % T;
+    const int lid1 = threadIdx.x / T;
+    const long nG = a.nWalkers / FS_W;
+    const long g = blockIdx.x;
+    const long walker = g * FS_W + lid1;
+    const long threadId = (g * T + lid0) + (long)lid1 * nG * T;
+    const long groupRow = g + (long)lid1 * nG;
+#else
+    const long walker = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (walker >= a.nWalkers) return;
+    const int lid0 = 0;
+    const long threadId = walker;
+    const long groupRow = walker / FS_LOCAL;
+#endif
+    fs_f *path = reinterpret_cast<fs_f *>(a.paths) + walker * (long)(FS_N * FS_DOF);
+    fs_f *oldPath = reinterpret_cast<fs_f *>(a.oldPaths) + threadId * (long)((FS_SEG - 1) * FS_DOF);
+
+    fs_stream own, grp;
+#if FS_RANLUX
+    own.load(a.rl, threadId);
+    grp.load(a.grl, threadId);
+#else
+    own.load(a.seeds, threadId);
+    grp.load(a.seedsSnap, groupRow);
+#endif
+    fs_u32 accepts = 0;
+    fs_f opAccum[FS_NOPS1];
+#pragma unroll
+    for (int k = 0; k < FS_NOPS1; k++) opAccum[k] = (fs_f)0.0;
+
+    for (int i = 0; i < a.operatorRuns; i++) {
+#if FS_PARALLEL
+        __syncthreads();
+#endif
+        for (int j = 0; j < a.metroSteps; j++) {
+#if FS_PARALLEL
+            const fs_u32 start = ((grp.integer() & (fs_u32)(FS_SEG - 1)) + (fs_u32)(FS_SEG * lid0)) % (fs_u32)FS_N;
+#else
+            const fs_u32 start = (grp.integer() & (fs_u32)(FS_N - 1)) % (fs_u32)FS_N;
+#endif
+            fs_exact_move(path, oldPath, start, own, accepts);
+#if FS_PARALLEL
+            __syncthreads();
+#endif
+        }
+#if FS_NOPS > 0 || FS_BINS || FS_NCORR > 0
+#if FS_PARALLEL
+        const int t0 = lid0 * FS_SEG, t1 = (lid0 + 1) * FS_SEG;
+#else
+        const int t0 = 0, t1 = FS_N;
+#endif
+        for (int ts = t0; ts < t1; ts++) {
+            fs_f x[FS_DOF];
+            fs_gather(path, ts, x);
+#if FS_NOPS > 0
+            fs_operator_v(x, opAccum);
+#endif
+#if FS_BINS
+            fs_histogram(x, a.bins);
+#endif
+#if FS_NCORR > 0
+            fs_f *cor = reinterpret_cast<fs_f *>(a.corMeans) + walker * (long)(FS_NCORR * (FS_N / 2));
+            for (int corTI = 0; corTI < FS_N / 2; corTI++) {
+                int corT = corTI + lid0;
+                if (corT >= FS_N / 2) corT -= FS_N / 2;
+                const int ts2 = (ts + corT < FS_N) ? ts + corT : ts + corT - FS_N;
+                fs_f prod[FS_NCORR1], x2[FS_DOF];
+#pragma unroll
+                for (int c = 0; c < FS_NCORR1; c++) prod[c] = (fs_f)1.0;
+                fs_correlator_v(x, prod);
+                fs_gather(path, ts2, x2);
+                fs_correlator_v(x2, prod);
+#pragma unroll
+                for (int c = 0; c < FS_NCORR; c++) cor[c * (FS_N / 2) + corT] += prod[c];
+#if FS_PARALLEL
+                __syncthreads();
+#endif
+            }
+#endif
+        }
+#endif
+    }
+#if FS_NOPS > 0
+    fs_f *opm = reinterpret_cast<fs_f *>(a.opMeans);
+#pragma unroll
+    for (int k = 0; k < FS_NOPS; k++) opm[k + threadId * FS_NOPS] = opAccum[k] * (fs_f)FS_OPNORM;
+#endif
+    a.accepts[threadId] = accepts;
+#if FS_RANLUX
+    own.store(a.rl, threadId);
+    grp.store(a.grl, threadId);
+#else
+    own.store(a.seeds, threadId);
+#endif
+}
+#endif  // FS_SAMPLER == 0
+
+// ===========================================================================
+// fs_wave_sweep
+// ===========================================================================
+#if FS_SAMPLER == 1 && FS_SEG <= 64
+
+template <typename T> FS_DEV T fs_lane_read(T v, int src) { return __shfl(v, src, 64); }
+
+// Sum over aligned groups of G lanes (xor butterfly); every lane of a group
+// ends with the same bits.
+template <int G, typename T> FS_DEV T fs_group_sum(T v)
+{
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs a)
+{
+    const int lane = threadIdx.x & 63;
+    const long w = (long)blockIdx.x * FS_WPB + (threadIdx.x >> 6);
+    if (w >= a.nWalkers) return;  // whole wavefronts only
+
+    fs_f *path = reinterpret_cast<fs_f *>(a.paths) + w * (long)(FS_N * FS_DOF);
+    const bool active = lane < FS_B;
+    const int ip = lane & (FS_SEG - 1);        // position inside the segment
+    const int segBase = lane & ~(FS_SEG - 1);  // lane holding the segment's left end
+    // level at which this position is generated: S - ctz(ip)
+    const int lvl = ip ? FS_S - __builtin_ctz((unsigned)ip) : 0;
+    const fs_f sig = fs_sigma(lvl > 0 ? lvl : 1);
+
+    fs_stream rng, grp;
+#if FS_RANLUX
+    rng.load(a.rl, w * 64 + lane);
+    grp.load(a.grl, w);
+#else
+    rng.load(a.seeds, w * 64 + lane);
+    grp.load(a.seeds, a.nWalkers * 64 + w);
+#endif
+
+    fs_u32 nacc = 0;
+    double opS[FS_NOPS1], opC[FS_NOPS1];
+#pragma unroll
+    for (int k = 0; k < FS_NOPS1; k++) opS[k] = opC[k] = 0.0;
+
+#if FS_NCORR > 0
+    extern __shared__ char fs_smem[];
+    fs_f *cbuf = reinterpret_cast<fs_f *>(fs_smem) + (threadIdx.x >> 6) * (FS_N * FS_NCORR);
+#endif
+
+    for (int i = 0; i < a.operatorRuns; i++) {
+        for (int j = 0; j < a.metroSteps; j++) {
+            const fs_u32 r = grp.integer() & (fs_u32)(FS_SEG - 1);
+            for (int g = 0; g < FS_NB; g++) {
+                const int base = (int)r + g * FS_B;
+                const int s = (base + (active ? lane : 0)) & (FS_N - 1);
+                const int sR = (base + FS_B) & (FS_N - 1);
+                fs_f xo[FS_DOF], xn[FS_DOF], xr[FS_DOF], z[FS_KDRAW];
+                fs_gather(path, s, xo);
+                fs_gather(path, sR, xr);
+
+                // every lane draws the same count so buffered streams stay in step
+                fs_f u[FS_KDRAW];
+#pragma unroll
+                for (int k = 0; k < FS_KDRAW; k++) u[k] = rng.uniform();
+#pragma unroll
+                for (int k = 0; k < FS_KPAIR; k++) {
+                    fs_f sn = (fs_f)0.0, cs = (fs_f)0.0;
+                    const fs_f v = u[2 * k + 1];
+                    if (v != (fs_f)0.0) {
+                        const fs_f rho = sqrt(-2.0 * log(v));
+                        const fs_f ph = 2.0 * M_PI * u[2 * k];
+                        sn = rho * sin(ph);
+                        cs = rho * cos(ph);
+                    }
+                    z[2 * k] = sig * sn;
+                    z[2 * k + 1] = sig * cs;
+                }
+                const fs_f vOld = fs_potential_v(xo);
+#pragma unroll
+                for (int d = 0; d < FS_DOF; d++) xn[d] = xo[d];
+#pragma unroll
+                for (int n = 1; n <= FS_S; n++) {
+                    const int h = FS_SEG >> n;
+                    const bool gen = (ip & (2 * h - 1)) == h;
+                    const bool rightEnd = lane + h == FS_B;
+#pragma unroll
+                    for (int d = 0; d < FS_DOF; d++) {
+                        const fs_f L = fs_lane_read(xn[d], lane - h);
+                        fs_f R = fs_lane_read(xn[d], (lane + h) & 63);
+                        R = rightEnd ? xr[d] : R;
+                        if (gen) xn[d] = 0.5 * (L + R) + z[d];
+                    }
+                }
+                const fs_f vNew = fs_potential_v(xn);
+                fs_f dv = (ip != 0 && active) ? vNew - vOld : (fs_f)0.0;
+                dv = fs_group_sum<FS_SEG>(dv);
+                const fs_f dS = eps * dv;
+                const fs_f ua = fs_lane_read(u[0], segBase);
+                const bool acc = !(exp(-dS) < ua);
+                if (acc && ip == 0 && active) nacc++;
+                if (acc && ip != 0 && active) {
+#pragma unroll
+                    for (int d = 0; d < FS_DOF; d++) path[d * FS_N + s] = xn[d];
+                }
+            }
+        }
+
+        // ---- measurement over all N slices ----
+#if FS_NOPS > 0 || FS_BINS || FS_NCORR > 0
+        fs_f part[FS_NOPS1];
+#pragma unroll
+        for (int k = 0; k < FS_NOPS1; k++) part[k] = (fs_f)0.0;
+        if (active) {
+            for (int t = lane; t < FS_N; t += FS_B) {
+                fs_f x[FS_DOF];
+                fs_gather(path, t, x);
+#if FS_NOPS > 0
+                fs_operator_v(x, part);
+#endif
+#if FS_BINS
+                fs_histogram(x, a.bins);
+#endif
+#if FS_NCORR > 0
+                fs_f prod[FS_NCORR1];
+#pragma unroll
+                for (int c = 0; c < FS_NCORR1; c++) prod[c] = (fs_f)1.0;
+                fs_correlator_v(x, prod);
+#pragma unroll
+                for (int c = 0; c < FS_NCORR; c++) cbuf[c * FS_N + t] = prod[c];
+#endif
+            }
+        }
+#if FS_NOPS > 0
+#pragma unroll
+        for (int k = 0; k < FS_NOPS; k++) {
+            const fs_f m = fs_group_sum<FS_B>(part[k]);
+            const double y = (double)m - opC[k];
+            const double t = opS[k] + y;
+            opC[k] = (t - opS[k]) - y;
+            opS[k] = t;
+            if (a.series != nullptr && lane == 0)
+                a.series[((long)(a.seriesOffset + i) * a.nWalkers + w) * FS_NOPS + k] = (float)(m / (fs_f)FS_N);
+        }
+#endif
+#if FS_NCORR > 0
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        {
+            double *cb = reinterpret_cast<double *>(a.corMeans) + w * (long)(FS_NCORR * FS_CORLAGS * 2);
+#pragma unroll
+            for (int c = 0; c < FS_NCORR; c++) {
+                const fs_f *cv = cbuf + c * FS_N;
+                for (int tau = lane; tau < FS_CORLAGS; tau += 64) {
+                    fs_f acc = (fs_f)0.0;
+                    for (int t = 0; t < FS_N; t++) acc += cv[t] * cv[(t + tau) & (FS_N - 1)];
+                    double *bin = cb + (c * FS_CORLAGS + tau) * 2;
+                    const double sum = bin[0], comp = bin[1];
+                    const double y = (double)acc - comp;
+                    const double t2 = sum + y;
+                    bin[1] = (t2 - sum) - y;
+                    bin[0] = t2;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+#endif
+#endif
+    }
+
+    // ---- launch outputs ----
+    const fs_u32 tot = fs_group_sum<64>(nacc);
+    if (lane == 0) a.accepts[w] = tot;
+#if FS_NOPS > 0
+    if (lane == 0) {
+        double *opm = reinterpret_cast<double *>(a.opMeans);
+#pragma unroll
+        for (int k = 0; k < FS_NOPS; k++) {
+            opm[w * FS_NOPS + k] = opS[k] / ((double)a.operatorRuns * (double)FS_N);
+            if (a.opTotals != nullptr) {
+                double *tk = a.opTotals + (w * FS_NOPS + k) * 2;
+                const double y = (opS[k] - opC[k]) - tk[1];
+                const double t = tk[0] + y;
+                tk[1] = (t - tk[0]) - y;
+                tk[0] = t;
+            }
+        }
+    }
+#endif
+#if FS_RANLUX
+    rng.store(a.rl, w * 64 + lane);
+    if (lane == 0) grp.store(a.grl, w);
+#else
+    rng.store(a.seeds, w * 64 + lane);
+    if (lane == 0) grp.store(a.seeds, a.nWalkers * 64 + w);
+#endif
+}
+#endif  // FS_SAMPLER == 1
+
+// ===========================================================================
+// Helpers for parity tests of the compiled system section
+// ===========================================================================
+extern "C" __global__ void fs_eval_system(const fs_f *__restrict__ x, double *__restrict__ pot,
+                                          double *__restrict__ ops, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fs_f v[FS_DOF];
+#pragma unroll
+    for (int d = 0; d < FS_DOF; d++) v[d] = x[i * FS_DOF + d];
+    pot[i] = (double)fs_potential_v(v);
+    fs_f acc[FS_NOPS1];
+#pragma unroll
+    for (int k = 0; k < FS_NOPS1; k++) acc[k] = (fs_f)0.0;
+#if FS_NOPS > 0
+    fs_operator_v(v, acc);
+#endif
+#pragma unroll
+    for (int k = 0; k < FS_NOPS; k++) ops[i * FS_NOPS + k] = (double)acc[k];
+}

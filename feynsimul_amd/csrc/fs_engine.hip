@@ -1,0 +1,463 @@
+// fs_engine.hip -- host side of libfeynsimul.so: the C ABI of include/feynsimul.h.
+//
+// fs_create turns a configuration plus the system section into one hiprtc
+// module (the sampler template fs_sampler.h is baked into this library), the
+// role cl.Program(...).build plays in the reference (FeynSimul/kernel.py:346-382).
+// Launch geometry follows the reference for the exact sampler
+// (kernel.py:192-211) and is the wave sampler's own otherwise.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/feynsimul.h"
+#include "device/fs_abi_args.h"
+#include "fs_rng_kernels.hip"
+
+namespace {
+
+const char kRngHeader[] = {
+#embed "device/fs_rng.h"
+    , 0};
+const char kArgsHeader[] = {
+#embed "device/fs_abi_args.h"
+    , 0};
+const char kSamplerHeader[] = {
+#embed "device/fs_sampler.h"
+    , 0};
+
+thread_local std::string g_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define FS_HIP(call)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) return fail(FS_ERR_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+std::string lit(double v)
+{
+    char b[64];
+    snprintf(b, sizeof(b), "%1.17e", v);
+    return b;
+}
+
+int copy_out(const std::string &s, char *buf, size_t len, size_t *needed)
+{
+    if (needed) *needed = s.size() + 1;
+    if (buf && len) {
+        size_t n = s.size() < len - 1 ? s.size() : len - 1;
+        memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return FS_OK;
+}
+
+}  // namespace
+
+struct fs_engine {
+    fs_config cfg{};
+    std::string source;
+    std::string log;
+    std::vector<char> code;
+    hipModule_t module = nullptr;
+    hipFunction_t fn_exact = nullptr;
+    hipFunction_t fn_wave = nullptr;
+    hipFunction_t fn_eval = nullptr;
+    int wpb = 4;
+    size_t wave_smem = 0;
+    int device = -1;
+};
+
+namespace {
+
+int validate(const fs_config *c)
+{
+    if (!c) return fail(FS_ERR_INVALID, "config is NULL");
+    if (c->abi_version != FS_ABI_VERSION)
+        return fail(FS_ERR_INVALID, "ABI version %d, library has %d", c->abi_version, FS_ABI_VERSION);
+    if (c->N <= 0 || (c->N & (c->N - 1)))
+        return fail(FS_ERR_INVALID, "Path node number, N, must be a power of 2.");
+    if (c->S < 1 || (1 << c->S) > c->N) return fail(FS_ERR_INVALID, "2^S must be less than or equal to N.");
+    if (c->DOF < 1 || c->DOF > 64) return fail(FS_ERR_INVALID, "DOF must be in [1, 64]");
+    if (c->n_walkers < 1) return fail(FS_ERR_INVALID, "nbrOfWalkers must be positive");
+    if (c->walkers_per_group < 1 || c->n_walkers % c->walkers_per_group)
+        return fail(FS_ERR_INVALID, "The number of walkers must be a multiple of the number of walkers per work group.");
+    if (c->n_ops < 0 || c->n_corr < 0) return fail(FS_ERR_INVALID, "negative operator count");
+    if (c->ranlux) {
+        const int l = c->ranlux_lux;
+        if (l < 0 || (l > 4 && l < 24)) return fail(FS_ERR_INVALID, "RANLUX luxury value %d gives a negative skip", l);
+        if (l >= 24 && ((l - 24) % 4 != 0 || (l - 24 > 0 && l - 24 < 24)))
+            return fail(FS_ERR_INVALID, "RANLUX skip %d must be 0 or a multiple of 4 that is >= 24", l - 24);
+    }
+    if (c->sampler == FS_SAMPLER_EXACT) {
+        if (c->parallelize_path) {
+            const long long T = c->N >> c->S;
+            if (T * c->walkers_per_group > 1024)
+                return fail(FS_ERR_INVALID, "work-group of %lld items exceeds 1024", T * c->walkers_per_group);
+        } else if (c->local_size < 1) {
+            return fail(FS_ERR_INVALID, "local_size must be positive");
+        }
+    } else if (c->sampler == FS_SAMPLER_WAVE) {
+        if ((1 << c->S) > 64) return fail(FS_ERR_INVALID, "wave sampler needs 2^S <= 64");
+        if (c->n_corr > 0 && (c->cor_lags < 1 || c->cor_lags > c->N / 2))
+            return fail(FS_ERR_INVALID, "cor_lags must be in [1, N/2]");
+    } else {
+        return fail(FS_ERR_INVALID, "unknown sampler %d", c->sampler);
+    }
+    if (c->bins && (c->bin_res < 1 || !(c->x_max > c->x_min)))
+        return fail(FS_ERR_INVALID, "histogram needs bin_res >= 1 and x_max > x_min");
+    return FS_OK;
+}
+
+std::string build_source(const fs_config *c, const char *system_src, int wpb)
+{
+    std::string s;
+    auto def = [&](const char *k, const std::string &v) { s += "#define " + std::string(k) + " " + v + "\n"; };
+    auto defi = [&](const char *k, long long v) { def(k, std::to_string(v)); };
+    s += "// generated by fs_create\n";
+    def("FLOAT_TYPE", c->double_precision ? "double" : "float");
+    defi("FS_DOUBLE", c->double_precision ? 1 : 0);
+    defi("FS_N", c->N);
+    defi("FS_S", c->S);
+    defi("FS_DOF", c->DOF);
+    defi("FS_NOPS", c->n_ops);
+    defi("FS_NCORR", c->n_corr);
+    defi("FS_CORLAGS", c->cor_lags > 0 ? c->cor_lags : c->N / 2);
+    defi("FS_RANLUX", c->ranlux ? 1 : 0);
+    {
+        const int l = c->ranlux_lux;
+        const int nskip = l == 0 ? 0 : l == 1 ? 24 : l == 2 ? 76 : l == 3 ? 200 : l == 4 ? 380 : l - 24;
+        defi("FS_NSKIP", c->ranlux ? nskip : 0);
+        char b[64];
+        snprintf(b, sizeof(b), "%.9Ef", (double)c->ranlux_int_range);  // as kernel.py:334 formats it
+        def("FS_ISCALE", b);
+    }
+    defi("FS_BINS", c->bins ? 1 : 0);
+    defi("FS_BINRES", c->bin_res > 0 ? c->bin_res : 1);
+    def("FS_XMIN", lit(c->x_min));
+    def("FS_XMAX", lit(c->x_max));
+    def("FS_INVBIN", lit(c->bins ? (double)c->bin_res / (c->x_max - c->x_min) : 1.0));
+    def("FS_OPNORM", lit(c->op_norm));
+    defi("FS_PARALLEL", c->parallelize_path ? 1 : 0);
+    defi("FS_W", c->walkers_per_group);
+    defi("FS_LOCAL", c->local_size > 0 ? c->local_size : 1);
+    defi("FS_WPB", wpb);
+    defi("FS_SAMPLER", c->sampler);
+    def("eps", lit(c->beta / (double)c->N));
+    s += R"(
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+#ifndef M_PI_2
+#define M_PI_2 1.57079632679489661923
+#endif
+#define native_exp exp
+#define native_log log
+#define native_sqrt sqrt
+#define native_sin sin
+#define native_cos cos
+#define native_powr pow
+#define native_recip(a) (1.0 / (a))
+#define native_divide(a, b) ((a) / (b))
+#define half_exp exp
+#define half_log log
+#define half_sqrt sqrt
+#define mad(a, b, c) ((a) * (b) + (c))
+#define pown(a, n) pow((a), (FLOAT_TYPE)(n))
+#pragma clang force_cuda_host_device begin
+)";
+    s += system_src ? system_src : "";
+    s += "\n#pragma clang force_cuda_host_device end\n#include \"fs_sampler.h\"\n";
+    return s;
+}
+
+std::string device_arch()
+{
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) {
+        std::string a = p.gcnArchName;
+        const size_t colon = a.find(':');
+        if (colon != std::string::npos) a = a.substr(0, colon);
+        if (!a.empty()) return a;
+    }
+    return "gfx950";
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_abi_version(void) { return FS_ABI_VERSION; }
+
+const char *fs_last_error(void) { return g_error.c_str(); }
+
+int fs_device_count(int *count)
+{
+    if (!count) return fail(FS_ERR_INVALID, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return FS_OK;
+}
+
+int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs_engine **out)
+{
+    if (!out) return fail(FS_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate(cfg);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) ndev = 0;
+    if (!compile_only && ndev == 0) return fail(FS_ERR_NO_DEVICE, "no GPU visible");
+
+    fs_engine *e = new fs_engine();
+    e->cfg = *cfg;
+    const size_t fsize = cfg->double_precision ? 8 : 4;
+    int wpb = cfg->waves_per_block > 0 ? cfg->waves_per_block : 4;
+    if (cfg->sampler == FS_SAMPLER_WAVE && cfg->n_corr > 0) {
+        const size_t per = (size_t)cfg->N * cfg->n_corr * fsize;
+        while (wpb > 1 && per * wpb > 65536) wpb--;
+        if (per > 65536) {
+            delete e;
+            return fail(FS_ERR_INVALID, "correlator staging needs %zu bytes of LDS per wavefront", per);
+        }
+        e->wave_smem = per * wpb;
+    }
+    e->wpb = wpb;
+    e->source = build_source(cfg, system_src, wpb);
+
+    hiprtcProgram prog;
+    const char *hdrs[] = {kRngHeader, kArgsHeader, kSamplerHeader};
+    const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_sampler.h"};
+    if (hiprtcCreateProgram(&prog, e->source.c_str(), "fs_module.hip", 3, hdrs, names) != HIPRTC_SUCCESS) {
+        delete e;
+        return fail(FS_ERR_COMPILE, "hiprtcCreateProgram failed");
+    }
+    const std::string arch = "--offload-arch=" + (ndev ? device_arch() : std::string("gfx950"));
+    std::vector<std::string> opts = {arch, "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                                     "-packed-fp32-ops", "-fno-math-errno", "-ffp-contract=fast"};
+    if (!cfg->double_precision) {
+        opts.push_back("-Xclang");
+        opts.push_back("-cl-single-precision-constant");
+    }
+    if (cfg->fast_math) {
+        opts.push_back("-freciprocal-math");
+        opts.push_back("-fno-signed-zeros");
+        opts.push_back("-fno-trapping-math");
+        opts.push_back("-fapprox-func");
+    }
+    std::vector<const char *> copts;
+    for (auto &o : opts) copts.push_back(o.c_str());
+    const hiprtcResult cr = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    if (ls > 1) {
+        std::vector<char> log(ls + 1, 0);
+        hiprtcGetProgramLog(prog, log.data());
+        e->log = log.data();
+    }
+    if (cr != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        g_error = "hiprtc compilation failed:\n" + e->log;
+        *out = e;  // keep the engine so the caller can read the build log
+        return FS_ERR_COMPILE;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    e->code.resize(cs);
+    hiprtcGetCode(prog, e->code.data());
+    hiprtcDestroyProgram(&prog);
+
+    if (!compile_only) {
+        (void)hipGetDevice(&e->device);
+        hipError_t le = hipModuleLoadData(&e->module, e->code.data());
+        if (le != hipSuccess) {
+            delete e;
+            return fail(FS_ERR_HIP, "hipModuleLoadData failed: %s", hipGetErrorString(le));
+        }
+        const char *kname = cfg->sampler == FS_SAMPLER_WAVE ? "fs_wave_sweep" : "fs_exact_sweep";
+        hipFunction_t fn = nullptr;
+        hipError_t fe = hipModuleGetFunction(&fn, e->module, kname);
+        if (fe != hipSuccess) {
+            (void)hipModuleUnload(e->module);
+            delete e;
+            return fail(FS_ERR_HIP, "module has no %s: %s", kname, hipGetErrorString(fe));
+        }
+        (cfg->sampler == FS_SAMPLER_WAVE ? e->fn_wave : e->fn_exact) = fn;
+        (void)hipModuleGetFunction(&e->fn_eval, e->module, "fs_eval_system");
+        (void)hipGetLastError();
+    }
+    *out = e;
+    return FS_OK;
+}
+
+int fs_destroy(fs_engine *e)
+{
+    if (!e) return FS_OK;
+    if (e->module) (void)hipModuleUnload(e->module);
+    delete e;
+    return FS_OK;
+}
+
+int fs_build_log(const fs_engine *e, char *buf, size_t len, size_t *needed)
+{
+    if (!e) return fail(FS_ERR_INVALID, "engine is NULL");
+    return copy_out(e->log, buf, len, needed);
+}
+
+int fs_source(const fs_engine *e, char *buf, size_t len, size_t *needed)
+{
+    if (!e) return fail(FS_ERR_INVALID, "engine is NULL");
+    return copy_out(e->source, buf, len, needed);
+}
+
+int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t metro_steps, int32_t series_offset,
+           void *stream)
+{
+    if (!e || !b) return fail(FS_ERR_INVALID, "engine or buffers is NULL");
+    if (!e->module) return fail(FS_ERR_INVALID, "engine was created compile-only");
+    if (operator_runs < 0 || metro_steps < 0) return fail(FS_ERR_INVALID, "negative run length");
+    const fs_config &c = e->cfg;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!b->paths || !b->accepts) return fail(FS_ERR_INVALID, "paths and accepts are required");
+    if (c.ranlux ? (!b->rl || !b->grl) : !b->seeds) return fail(FS_ERR_INVALID, "random state buffers missing");
+    if (c.n_ops > 0 && !b->op_means) return fail(FS_ERR_INVALID, "op_means required");
+    if (c.n_corr > 0 && !b->cor_means) return fail(FS_ERR_INVALID, "cor_means required");
+    if (c.bins && !b->bins) return fail(FS_ERR_INVALID, "bins required");
+
+    fs_kargs a{};
+    a.paths = b->paths;
+    a.accepts = static_cast<unsigned *>(b->accepts);
+    a.seeds = static_cast<unsigned *>(b->seeds);
+    a.seedsSnap = static_cast<unsigned *>(b->seeds_snap);
+    a.rl = static_cast<fs_ranlux *>(b->rl);
+    a.grl = static_cast<fs_ranlux *>(b->grl);
+    a.oldPaths = b->old_paths;
+    a.opMeans = b->op_means;
+    a.opTotals = static_cast<double *>(b->op_totals);
+    a.series = static_cast<float *>(b->series);
+    a.corMeans = b->cor_means;
+    a.bins = static_cast<unsigned *>(b->bins);
+    a.potCache = b->pot_cache;
+    a.nWalkers = c.n_walkers;
+    a.operatorRuns = operator_runs;
+    a.metroSteps = metro_steps;
+    a.seriesOffset = series_offset;
+    void *args[] = {&a};
+
+    if (c.sampler == FS_SAMPLER_EXACT) {
+        if (!b->old_paths) return fail(FS_ERR_INVALID, "old_paths required by the exact sampler");
+        if (!c.ranlux) {
+            if (!b->seeds_snap) return fail(FS_ERR_INVALID, "seeds_snap required by the exact sampler");
+            const long long threads = c.parallelize_path ? c.n_walkers * (long long)(c.N >> c.S) : c.n_walkers;
+            FS_HIP(hipMemcpyAsync(b->seeds_snap, b->seeds, (size_t)(threads + 1) * 16, hipMemcpyDeviceToDevice, st));
+        }
+        unsigned grid, block;
+        if (c.parallelize_path) {
+            block = (unsigned)((c.N >> c.S) * c.walkers_per_group);
+            grid = (unsigned)(c.n_walkers / c.walkers_per_group);
+        } else {
+            block = 64;
+            grid = (unsigned)((c.n_walkers + 63) / 64);
+        }
+        FS_HIP(hipModuleLaunchKernel(e->fn_exact, grid, 1, 1, block, 1, 1, 0, st, args, nullptr));
+    } else {
+        if (!e->fn_wave) return fail(FS_ERR_INVALID, "module has no wave sampler");
+        const unsigned grid = (unsigned)((c.n_walkers + e->wpb - 1) / e->wpb);
+        FS_HIP(hipModuleLaunchKernel(e->fn_wave, grid, 1, 1, 64 * e->wpb, 1, 1, (unsigned)e->wave_smem, st, args,
+                                     nullptr));
+    }
+    return FS_OK;
+}
+
+int fs_eval_system(fs_engine *e, const void *x, void *pot, void *ops, int64_t n, void *stream)
+{
+    if (!e || !x || !pot) return fail(FS_ERR_INVALID, "engine, x and pot are required");
+    if (!e->module) return fail(FS_ERR_INVALID, "engine was created compile-only");
+    if (e->cfg.n_ops > 0 && !ops) return fail(FS_ERR_INVALID, "ops required");
+    if (n <= 0) return FS_OK;
+    long long nn = n;
+    void *args[] = {&x, &pot, &ops, &nn};
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    FS_HIP(hipModuleLaunchKernel(e->fn_eval, grid, 1, 1, 256, 1, 1, 0, reinterpret_cast<hipStream_t>(stream), args,
+                                 nullptr));
+    return FS_OK;
+}
+
+// ---- ahead-of-time random-stream kernels (fs_rng_kernels.hip) ----
+static int need_device()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(FS_ERR_NO_DEVICE, "no GPU visible");
+    return FS_OK;
+}
+
+int fs_seed_xorshift(void *rows, int64_t n, uint64_t seed, int64_t first, void *stream)
+{
+    if (!rows || n < 0) return fail(FS_ERR_INVALID, "bad rows");
+    if (int rc = need_device()) return rc;
+    if (n == 0) return FS_OK;
+    hipLaunchKernelGGL(fs_k_seed_xorshift, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), static_cast<uint4 *>(rows), (long long)n,
+                       (unsigned long long)seed, (long long)first);
+    FS_HIP(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_ranlux_init(void *tab, const void *seeds, int64_t n, void *stream)
+{
+    if (!tab || !seeds || n < 0) return fail(FS_ERR_INVALID, "bad ranlux table");
+    if (int rc = need_device()) return rc;
+    if (n == 0) return FS_OK;
+    hipLaunchKernelGGL(fs_k_ranlux_init, dim3((unsigned)((n + 127) / 128)), dim3(128), 0,
+                       reinterpret_cast<hipStream_t>(stream), static_cast<fs_ranlux *>(tab),
+                       static_cast<const unsigned long long *>(seeds), (long long)n);
+    FS_HIP(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_xorshift_draw(const void *rows, void *out_u32, void *out_f32, void *out_f64, int64_t n, int32_t ndraw,
+                     void *stream)
+{
+    if (!rows || n < 0 || ndraw < 0) return fail(FS_ERR_INVALID, "bad draw request");
+    if (int rc = need_device()) return rc;
+    if (n == 0 || ndraw == 0) return FS_OK;
+    hipLaunchKernelGGL(fs_k_xs_draw, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), static_cast<const uint4 *>(rows),
+                       static_cast<unsigned *>(out_u32), static_cast<float *>(out_f32),
+                       static_cast<double *>(out_f64), (long long)n, (int)ndraw);
+    FS_HIP(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_ranlux_draw(void *tab, void *out_f32, void *out_f64, int64_t n, int32_t nquad, int32_t lux, void *stream)
+{
+    if (!tab || n < 0 || nquad < 0 || lux < 0 || lux > 4 || (!out_f32 && !out_f64))
+        return fail(FS_ERR_INVALID, "bad ranlux draw request");
+    if (int rc = need_device()) return rc;
+    if (n == 0 || nquad == 0) return FS_OK;
+    hipLaunchKernelGGL(fs_k_rl_draw, dim3((unsigned)((n + 127) / 128)), dim3(128), 0,
+                       reinterpret_cast<hipStream_t>(stream), static_cast<fs_ranlux *>(tab),
+                       static_cast<float *>(out_f32), static_cast<double *>(out_f64), (long long)n, (int)nquad,
+                       (int)lux);
+    FS_HIP(hipGetLastError());
+    return FS_OK;
+}
+
+}  // extern "C"

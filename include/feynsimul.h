@@ -1,0 +1,142 @@
+/*
+ * feynsimul.h -- C ABI of the MI355X PIMC engine (libfeynsimul.so).
+ *
+ * Plain pointers and sizes only.  Device buffers are allocated by the caller
+ * (the engine borrows them); every call that touches the GPU is ordered on
+ * the hipStream_t passed as `stream` (NULL = the null stream).  Every entry
+ * point returns FS_OK (0) or an error code; fs_last_error() gives the
+ * message of the last failure on the calling thread.
+ *
+ * The reference drives one OpenCL program per configuration from Python
+ * (FeynSimul/kernel.py); the entry points below are what its driver would
+ * bind in place of pyopencl:
+ *   fs_create / fs_destroy        <- PIMCKernel.__init__ program build,
+ *                                    kernel.py:346-382 (cl.Program(...).build)
+ *   fs_build_log                  <- PIMCKernel.getBuildLog, kernel.py:778-785
+ *   fs_run                        <- PIMCKernel.run, kernel.py:459-506
+ *                                    (the `metropolis` launch, kernel.c:514-837)
+ *   fs_seed_xorshift              <- seed table fill, kernel.py:421-426
+ *   fs_ranlux_init                <- initRanlux / groupInitRanlux,
+ *                                    kernel.py:508-554, kernel.c:152-166
+ *   fs_xorshift_draw, fs_ranlux_draw, fs_eval_system
+ *                                 <- stream and expression dumps of
+ *                                    examples/ranlux_test_kernel.c:50-212
+ *                                    (test support)
+ */
+#ifndef FEYNSIMUL_H
+#define FEYNSIMUL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_ABI_VERSION 1
+
+enum {
+    FS_OK = 0,
+    FS_ERR_INVALID = 1,   /* bad argument or configuration */
+    FS_ERR_COMPILE = 2,   /* hiprtc rejected the system source; see fs_build_log */
+    FS_ERR_HIP = 3,       /* a HIP runtime call failed */
+    FS_ERR_NO_DEVICE = 4  /* no GPU visible */
+};
+
+enum { FS_SAMPLER_EXACT = 0, FS_SAMPLER_WAVE = 1 };
+
+/* One configuration = one compiled module (the reference rebuilds its
+ * program for every change of these, kernel.py:213-382). */
+typedef struct fs_config {
+    int32_t abi_version;        /* FS_ABI_VERSION */
+    int32_t sampler;            /* FS_SAMPLER_* */
+    int32_t N;                  /* slices, power of two */
+    int32_t S;                  /* bisection level, 2^S <= N */
+    int32_t DOF;                /* coordinates per slice */
+    int32_t walkers_per_group;  /* nbrOfWalkersPerWorkGroup */
+    int64_t n_walkers;          /* nbrOfWalkers */
+    int32_t parallelize_path;   /* enableParallelizePath (exact sampler) */
+    int32_t local_size;         /* work-group size that fixes group rows when !parallelize_path */
+    int32_t double_precision;   /* enableDouble */
+    int32_t ranlux;             /* enableRanlux */
+    int32_t ranlux_lux;         /* RANLUXCL luxury value: 0..4, or p >= 24 */
+    int32_t fast_math;          /* relaxed math as with -cl-fast-relaxed-math */
+    uint64_t ranlux_int_range;  /* ranluxIntMax + 1 (scale of integer draws) */
+    int32_t n_ops;              /* operators */
+    int32_t n_corr;             /* correlators */
+    int32_t cor_lags;           /* correlator lags (wave sampler; exact uses N/2) */
+    int32_t bins;               /* enableBins */
+    int32_t bin_res;            /* binResolutionPerDOF */
+    int32_t waves_per_block;    /* wave sampler launch shape (0 = automatic) */
+    double beta;
+    double x_min, x_max;
+    double op_norm;             /* exact sampler: 1/(operatorRuns N walkers/threads) */
+} fs_config;
+
+/* Device buffers of one launch; unused entries may be NULL.  Layouts:
+ *   paths     FLOAT [walkers][DOF][N]
+ *   exact:  accepts u32[threads]; seeds u32[threads+1][4]; seeds_snap same;
+ *           rl, grl 112-byte RANLUX states [threads]; old_paths FLOAT
+ *           [threads][(2^S-1) DOF]; op_means FLOAT[threads][n_ops];
+ *           cor_means FLOAT[walkers][n_corr][N/2]
+ *   wave:   accepts u32[walkers]; seeds u32[walkers*64 + walkers][4] (lane rows,
+ *           then group rows); rl [walkers*64]; grl [walkers];
+ *           op_means f64[walkers][n_ops]; op_totals f64[walkers][n_ops][2];
+ *           series f32[measurements][walkers][n_ops];
+ *           cor_means f64[walkers][n_corr][cor_lags][2]
+ *   bins u32[bin_res^DOF] */
+typedef struct fs_buffers {
+    void *paths;
+    void *accepts;
+    void *seeds;
+    void *seeds_snap;
+    void *rl;
+    void *grl;
+    void *old_paths;
+    void *op_means;
+    void *op_totals;
+    void *series;
+    void *cor_means;
+    void *bins;
+    void *pot_cache;
+} fs_buffers;
+
+typedef struct fs_engine fs_engine;
+
+int fs_abi_version(void);
+const char *fs_last_error(void);
+int fs_device_count(int *count);
+
+/* Compile the per-configuration module.  `system_src` is the system section:
+ * the user's helper code and the functions potential / fs_operator /
+ * fs_correlator with their *_v array wrappers.  Needs a GPU to load the
+ * module unless `compile_only` is set (then only the build log is kept). */
+int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs_engine **out);
+int fs_destroy(fs_engine *e);
+/* Copies the build log (NUL-terminated, truncated to len) and returns its full length in *needed. */
+int fs_build_log(const fs_engine *e, char *buf, size_t len, size_t *needed);
+/* Copies the generated module source (for verbose dumps). */
+int fs_source(const fs_engine *e, char *buf, size_t len, size_t *needed);
+
+/* One launch of operator_runs x metro_steps sweeps (the reference `run`). */
+int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t metro_steps,
+           int32_t series_offset, void *stream);
+
+/* Evaluate the compiled potential and operators at n points x[n][DOF]
+ * (FLOAT); writes f64 pot[n] and ops[n][n_ops]. */
+int fs_eval_system(fs_engine *e, const void *x, void *pot, void *ops, int64_t n, void *stream);
+
+/* rows[i] = xorshift128 state from splitmix64 state seed + first + i. */
+int fs_seed_xorshift(void *rows, int64_t n, uint64_t seed, int64_t first, void *stream);
+/* tab[i] = ranluxcl_init(seeds[i]); seeds is a device u64 array. */
+int fs_ranlux_init(void *tab, const void *seeds, int64_t n, void *stream);
+/* ndraw draws per row without advancing rows: u32, f32 and f64 (any may be NULL). */
+int fs_xorshift_draw(const void *rows, void *out_u32, void *out_f32, void *out_f64, int64_t n, int32_t ndraw,
+                     void *stream);
+/* nquad four-value RANLUX draws per state (advancing the states); lux 0..4. */
+int fs_ranlux_draw(void *tab, void *out_f32, void *out_f64, int64_t n, int32_t nquad, int32_t lux, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEYNSIMUL_H */

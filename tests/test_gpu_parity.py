@@ -1,0 +1,198 @@
+"""One launch of each GPU sampler vs the CPU oracle on the same inputs.
+
+exact sampler  <-> oracle.run_reference: the reference's work-item geometry,
+                   stream rows and draw order (kernel.c:514-837).
+wave sampler   <-> oracle.run_wave: the engine's slice-per-lane ordering.
+
+Random-number consumption is fixed per move, so the final stream states must
+be bit-identical.  Paths, acceptance counts and operator means must agree to
+FP32 tolerance; a Metropolis decision whose margin is below the rounding
+difference of the two compilers may flip, so a small fraction of walkers is
+allowed to differ and is excluded from the path comparison.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from feynsimul_amd.kernel_args import KernelArgs
+from feynsimul_amd.kernel import PIMCKernel, _GROUP_OFFSET
+from feynsimul_amd.physical_systems import HarmOsc, Lm2m2Cluster, AnharmOsc
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print("[case] " + msg, flush=True)
+
+
+def _system(name):
+    if name == "ho":
+        return HarmOsc()
+    if name == "aho":
+        return AnharmOsc(0.1)
+    return Lm2m2Cluster(3)
+
+
+def _initial_paths(system, W, N, rng):
+    if isinstance(system, Lm2m2Cluster):
+        pos = system.initialPositions(W, rng, sigma=0.3)
+        p = np.repeat(pos[:, :, None], N, axis=2) + rng.normal(0, 0.02, (W, system.DOF, N))
+        return p.reshape(W, system.DOF * N)
+    return rng.uniform(-1.0, 1.0, (W, system.DOF * N))
+
+
+def _ka(system, W, N, S, beta, opRuns, steps, parallel, ranlux, double, sampler, Wg=1, seed=11):
+    return KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=S, enableOperator=True,
+                      enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                      enableSingleNodeMove=False, enableParallelizePath=parallel, enableGlobalPath=False,
+                      enableGlobalOldPath=False, enableBins=False, enableDouble=double,
+                      enableRanlux=ranlux, luxuaryFactor=1 if ranlux else None,
+                      ranluxIntMax=2 ** 31 - 1 if ranlux else None, metroStepsPerOperatorRun=steps,
+                      operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=Wg, operators=(system.energyOp,),
+                      seed=seed, sampler=sampler, fastMath=False)
+
+
+def _compare(gp, rp, acc, racc, walker_of_thread, tol, min_match=0.95):
+    same_t = acc == racc
+    assert same_t.mean() >= min_match, "acceptance counts differ on %d of %d" % ((~same_t).sum(), len(same_t))
+    bad_w = np.unique(walker_of_thread[~same_t])
+    ok = np.ones(gp.shape[0], bool)
+    ok[bad_w] = False
+    err = np.abs(gp[ok] - rp[ok]).max()
+    assert err < tol, "path error %g" % err
+    return ok
+
+
+CASES_EXACT = [
+    # name, W, N, S, beta, parallel, ranlux, double, Wg
+    ("ho", 32, 64, 3, 5.0, True, False, False, 1),
+    ("ho", 32, 64, 3, 5.0, True, False, False, 4),
+    ("ho", 64, 32, 2, 4.0, False, False, False, 8),
+    ("ho", 16, 64, 4, 5.0, True, True, False, 1),
+    ("ho", 16, 64, 3, 5.0, True, False, True, 1),
+    ("lm2m2", 8, 128, 4, 10.0, True, False, False, 1),
+]
+
+
+@pytest.mark.parametrize("case", CASES_EXACT, ids=lambda c: "%s-W%d-N%d-S%d-par%d-rl%d-f64%d-Wg%d" % (
+    c[0], c[1], c[2], c[3], c[5], c[6], c[7], c[8]))
+def test_exact_sampler_matches_reference_order(gpu, case):
+    name, W, N, S, beta, par, rl, dbl, Wg = case
+    _case("exact " + str(case))
+    system = _system(name)
+    rng = np.random.default_rng(3)
+    p0 = _initial_paths(system, W, N, rng)
+    opRuns, steps = 3, 2
+    k = PIMCKernel(_ka(system, W, N, S, beta, opRuns, steps, par, rl, dbl, "exact", Wg))
+    if rl:
+        k.initRanlux()
+        k.groupInitRanlux()
+    k.setPaths(p0)
+    k.run()
+    ft = np.float64 if dbl else np.float32
+    lib = O.build(system, N, beta, operators=(system.energyOp,), double=dbl)
+    cfg = O.make_cfg(N, S, system.DOF, W, beta, opRuns, steps, walkersPerGroup=Wg, parallelizePath=par,
+                     ranlux=rl, nskip=O.nskip_of_lux(2), nOps=1)
+    paths = np.ascontiguousarray(p0.astype(ft))
+    T = cfg.nThreads
+    tid, grow = k._thread_layout()
+    walker_of_thread = np.empty(T, int)
+    walker_of_thread[tid.reshape(-1)] = np.repeat(np.arange(W), tid.shape[1])
+    if rl:
+        st = O.ranlux_init(lib, O.ranlux_seeds(11, 0, T))
+        row_of_thread = np.empty(T, np.int64)
+        row_of_thread[tid.reshape(-1)] = np.repeat(grow, tid.shape[1])
+        gseeds = np.array([O.splitmix64_pair((11 + _GROUP_OFFSET + int(r)) & ((1 << 64) - 1))[0]
+                           for r in row_of_thread], dtype=np.uint64)
+        gst = O.ranlux_init(lib, gseeds)
+        ref = O.run_reference(lib, cfg, paths, rl=st, grl=gst)
+        np.testing.assert_array_equal(k._rl.cpu().numpy().view(np.uint8).reshape(T, 112),
+                                      st.view(np.uint8).reshape(T, 112))
+    else:
+        seeds = O.xorshift_rows(11, 0, T + 1)
+        ref = O.run_reference(lib, cfg, paths, seeds=seeds)
+        np.testing.assert_array_equal(k._seeds.cpu().numpy().view(np.uint32), seeds)
+    acc = k._accepts.cpu().numpy().view(np.uint32)
+    tol = 2e-4 if name == "ho" else 2e-3
+    ok = _compare(k.getPaths(), paths, acc, ref["accepts"], walker_of_thread, tol)
+    # operator means: the reference layout is per work-item
+    opm = k._operatorValues.cpu().numpy()
+    ok_t = ok[walker_of_thread]
+    rel = np.abs(opm[ok_t] - ref["opMeans"][ok_t]) / (np.abs(ref["opMeans"][ok_t]) + 1e-3)
+    assert rel.max() < 1e-3, "operator mean error %g" % rel.max()
+
+
+CASES_WAVE = [
+    # name, W, N, S, beta, ranlux, double
+    ("ho", 64, 256, 4, 10.0, False, False),
+    ("ho", 32, 32, 3, 4.0, False, False),
+    ("ho", 32, 128, 6, 10.0, True, False),
+    ("aho", 32, 256, 5, 10.0, False, True),
+    ("lm2m2", 16, 256, 6, 10.0, False, False),
+]
+
+
+@pytest.mark.parametrize("case", CASES_WAVE, ids=lambda c: "%s-W%d-N%d-S%d-rl%d-f64%d" % (
+    c[0], c[1], c[2], c[3], c[5], c[6]))
+def test_wave_sampler_matches_oracle(gpu, case):
+    name, W, N, S, beta, rl, dbl = case
+    _case("wave " + str(case))
+    system = _system(name)
+    rng = np.random.default_rng(5)
+    p0 = _initial_paths(system, W, N, rng)
+    opRuns, steps = 3, 2
+    k = PIMCKernel(_ka(system, W, N, S, beta, opRuns, steps, True, rl, dbl, "wave"))
+    if rl:
+        k.initRanlux()
+        k.groupInitRanlux()
+    k.setPaths(p0)
+    k.run()
+    ft = np.float64 if dbl else np.float32
+    lib = O.build(system, N, beta, operators=(system.energyOp,), double=dbl)
+    cfg = O.make_cfg(N, S, system.DOF, W, beta, opRuns, steps, ranlux=rl, nskip=O.nskip_of_lux(2), nOps=1)
+    paths = np.ascontiguousarray(p0.astype(ft))
+    if rl:
+        lst = O.ranlux_init(lib, O.ranlux_seeds(11, 0, W * 64))
+        gst = O.ranlux_init(lib, O.ranlux_seeds(11, _GROUP_OFFSET, W))
+        ref = O.run_wave(lib, cfg, paths, lane_rl=lst, group_rl=gst)
+        np.testing.assert_array_equal(k._rl.cpu().numpy().view(np.uint8).reshape(W * 64, 112),
+                                      lst.view(np.uint8).reshape(W * 64, 112))
+        np.testing.assert_array_equal(k._grl.cpu().numpy().view(np.uint8).reshape(W, 112),
+                                      gst.view(np.uint8).reshape(W, 112))
+    else:
+        lane = O.xorshift_rows(11, 0, W * 64)
+        grp = O.xorshift_rows(11, W * 64, W)
+        ref = O.run_wave(lib, cfg, paths, lane_seeds=lane, group_seeds=grp)
+        dev = k._seeds.cpu().numpy().view(np.uint32)
+        np.testing.assert_array_equal(dev[: W * 64], lane)
+        np.testing.assert_array_equal(dev[W * 64:], grp)
+    acc = k._accepts.cpu().numpy().view(np.uint32)
+    tol = 2e-4 if name != "lm2m2" else 2e-3
+    ok = _compare(k.getPaths(), paths, acc, ref["accepts"], np.arange(W), tol, min_match=0.9)
+    ops = k.getOperators()[0]
+    rel = np.abs(ops[ok] - ref["opMeans"][ok]) / (np.abs(ref["opMeans"][ok]) + 1e-3)
+    assert rel.max() < 1e-3, "operator mean error %g" % rel.max()
+
+
+def test_system_section_matches_oracle(gpu):
+    """The hiprtc-compiled potential and virial operator vs gcc-compiled strings and the FP64 formula."""
+    from feynsimul_amd.physical_systems.lm2m2 import lm2m2_kelvin
+    _case("eval_system lm2m2 3 atoms")
+    system = Lm2m2Cluster(3)
+    k = PIMCKernel(_ka(system, 8, 64, 3, 10.0, 1, 1, True, False, True, "wave"))
+    rng = np.random.default_rng(1)
+    r = rng.uniform(2.2, 12.0, 4000)
+    x = np.zeros((4000, 9))
+    x[:, 3] = r * system.massScale  # atom 2 on the x axis, atom 3 far away
+    x[:, 6:] = 1e3
+    pot, ops = k.evalSystem(x)
+    ref = np.array([lm2m2_kelvin(v) for v in r])
+    np.testing.assert_allclose(pot, ref, rtol=1e-9, atol=1e-9)
+    lib = O.build(system, 64, 10.0, operators=(system.energyOp,), double=True)
+    xo = np.ascontiguousarray(x)
+    for i in range(0, 4000, 97):
+        v = lib.fs_oracle_potential(xo[i].ctypes.data)
+        assert abs(v - pot[i]) <= 1e-9 * max(1.0, abs(v))
+        acc = np.zeros(2)
+        lib.fs_oracle_operators(xo[i].ctypes.data, acc.ctypes.data)
+        assert abs(acc[0] - ops[i, 0]) <= 1e-9 * max(1.0, abs(acc[0]))

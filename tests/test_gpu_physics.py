@@ -1,0 +1,118 @@
+"""Estimators of the GPU samplers against FP64 closed forms.
+
+Harmonic oscillator, primitive action: <x^2>_N = E_vir = (1/N) sum_k 1/a_k with
+a_k = (2 - 2 cos(2 pi k / N)) / eps + eps; the correlator and the density are
+fixed by the same modes (oracle.ho_discrete_*).  Walkers are independent
+chains, so the standard error is the spread of per-walker means / sqrt(W).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from feynsimul_amd.kernel_args import KernelArgs
+from feynsimul_amd.kernel import PIMCKernel
+from feynsimul_amd.physical_systems import HarmOsc, AnharmOsc
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print("[case] " + msg, flush=True)
+
+
+def _ka(system, W, N, S, beta, opRuns, steps, sampler, Wg=1, corr=False, bins=False, seed=2024, lags=None,
+        ranlux=False, ops=None):
+    return KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=S, enableOperator=True,
+                      enableCorrelator=corr, enableBisection=True, enablePathShift=False,
+                      enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                      enableGlobalOldPath=False, enableBins=bins, enableDouble=False,
+                      enableRanlux=ranlux, luxuaryFactor=1 if ranlux else None,
+                      ranluxIntMax=2 ** 31 - 1 if ranlux else None,
+                      metroStepsPerOperatorRun=steps, operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=Wg,
+                      operators=ops or (system.energyOp,), correlators=("x1",) if corr else None,
+                      xMin=-3.5, xMax=3.5, binResolutionPerDOF=70, seed=seed, sampler=sampler,
+                      correlatorLags=lags)
+
+
+def test_ho_wave_energy_c1(gpu):
+    """1D HO, N=256, beta=10, 4096 walkers, S=4: 2000 burn-in sweeps, measure every 32 sweeps."""
+    _case("HO wave C1 energy")
+    system = HarmOsc()
+    W, N, S, beta = 4096, 256, 4, 10.0
+    rng = np.random.default_rng(0)
+    burn = PIMCKernel(_ka(system, W, N, S, beta, 1, 2000, "wave"))
+    burn.setPaths(rng.uniform(-1, 1, (W, N)))
+    burn.run()
+    k = PIMCKernel(_ka(system, W, N, S, beta, 100, 32, "wave", seed=99))
+    k.setPaths(burn.getPaths())
+    k.run()
+    per = k.getOperators()[0]
+    E, se = per.mean(), per.std() / np.sqrt(W)
+    exact = O.ho_discrete_x2(N, beta)
+    assert abs(exact - 0.499950082044) < 1e-10
+    print("E = %.6f +- %.6f, exact %.9f, AR %.3f" % (E, se, exact, k.getAcceptanceRate()))
+    assert abs(E - exact) < 4 * se
+    assert se < 2e-3
+
+
+def test_ho_exact_tutorial_run(gpu):
+    """The tutorial run (doc/tutorial/index.rst:31-74): N=128, beta=11, S=6, 896 walkers,
+    4 per work-group, 150 x 40 steps; the reference printed 0.499803373124."""
+    _case("HO exact tutorial")
+    system = HarmOsc()
+    W, N, S, beta = 896, 128, 6, 11.0
+    k = PIMCKernel(_ka(system, W, N, S, beta, 150, 40, "exact", Wg=4))
+    k.run()  # paths start at zero, as in the reference
+    k2 = PIMCKernel(_ka(system, W, N, S, beta, 150, 40, "exact", Wg=4, seed=5))
+    k2.setPaths(k.getPaths())
+    k2.run()
+    per = k2.getOperators()[0]
+    E, se = per.mean(), per.std() / np.sqrt(W)
+    exact = O.ho_discrete_x2(N, beta)
+    print("E = %.6f +- %.6f, exact %.9f, AR %.3f" % (E, se, exact, k2.getAcceptanceRate()))
+    assert abs(E - exact) < 4 * se
+    # the reference's own printed value is a single draw from the same distribution
+    assert abs(0.499803373124 - exact) < 6 * se
+
+
+def test_ho_correlator_and_density(gpu):
+    _case("HO wave correlator + bins")
+    system = HarmOsc()
+    W, N, S, beta = 1024, 128, 4, 8.0
+    rng = np.random.default_rng(1)
+    burn = PIMCKernel(_ka(system, W, N, S, beta, 1, 1000, "wave"))
+    burn.setPaths(rng.uniform(-1, 1, (W, N)))
+    burn.run()
+    k = PIMCKernel(_ka(system, W, N, S, beta, 50, 16, "wave", corr=True, bins=True, seed=7, lags=32))
+    k.setPaths(burn.getPaths())
+    k.run()
+    mean, se = k.getCorrelator()
+    exact = O.ho_discrete_correlator(N, beta)[:32]
+    z = np.abs(mean[0] - exact) / se[0]
+    print("correlator max z %.2f" % z.max())
+    assert z.max() < 5
+    dens = k.getBinCounts()
+    x2 = O.ho_discrete_x2(N, beta)
+    edges = np.linspace(-3.5, 3.5, 71)
+    from math import erf, sqrt
+    cdf = np.array([0.5 * (1 + erf(e / sqrt(2 * x2))) for e in edges])
+    p_exact = np.diff(cdf) / (edges[1] - edges[0])
+    assert np.abs(dens - p_exact).max() < 0.02
+
+
+def test_aho_energy_vs_transfer_matrix(gpu):
+    _case("AHO wave energy vs transfer matrix")
+    system = AnharmOsc(0.1)
+    W, N, S, beta = 2048, 128, 4, 8.0
+    rng = np.random.default_rng(2)
+    burn = PIMCKernel(_ka(system, W, N, S, beta, 1, 1000, "wave"))
+    burn.setPaths(rng.uniform(-1, 1, (W, N)))
+    burn.run()
+    k = PIMCKernel(_ka(system, W, N, S, beta, 50, 16, "wave", seed=8, ops=(system.positionSquaredOp,)))
+    k.setPaths(burn.getPaths())
+    k.run()
+    per = k.getOperators()[0]
+    x2, se = per.mean(), per.std() / np.sqrt(W)
+    ref, _, _ = O.transfer_matrix_1d(system.V, N, beta)
+    print("<x^2> = %.6f +- %.6f, transfer matrix %.6f" % (x2, se, ref))
+    assert abs(x2 - ref) < 4 * se
