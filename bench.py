@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s, one FP32 op per lane per clock
 
 
-def make_kernel(steps_per_launch, seed):
+def make_kernel(steps_per_launch, seed, storage=None):
     from feynsimul_amd.kernel_args import KernelArgs
     from feynsimul_amd.kernel import PIMCKernel
     from feynsimul_amd.physical_systems import Lm2m2Cluster
@@ -42,7 +42,8 @@ def make_kernel(steps_per_launch, seed):
                     enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
                     enableGlobalOldPath=False, enableBins=False, enableDouble=False,
                     metroStepsPerOperatorRun=steps_per_launch, operatorRuns=1, nbrOfWalkersPerWorkGroup=1,
-                    operators=(system.energyOp, system.meanPairDistanceOp), seed=seed, sampler="wave")
+                    operators=(system.energyOp, system.meanPairDistanceOp), seed=seed, sampler="wave",
+                    pathStorage=storage)
     return system, PIMCKernel(ka)
 
 
@@ -71,9 +72,9 @@ def cpu_baseline(target_s=12.0):
         t0 = time.perf_counter()
         O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=cores)
         dt = time.perf_counter() - t0
-        if dt > target_s / 4 or walkers * sweeps > 1 << 16:
+        if dt > target_s / 2 or sweeps >= 1 << 14:
             break
-        sweeps *= 2
+        sweeps = max(sweeps * 2, int(sweeps * target_s / max(dt, 1e-3)))
     upd = walkers * (N // SEG) * (SEG - 1) * sweeps
     return {"value": upd / dt, "unit": "slice-updates/s", "cores": cores, "kind": "port",
             "sample": "%d chains x %d sweeps of the same workload, reference work-item order, FP32, "
@@ -86,6 +87,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--storage", choices=["registers", "memory"], default=None,
+                    help="where the wave sampler keeps each path during a launch (default: automatic)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: chains are independent and this engine runs on a single device; "
@@ -94,7 +97,7 @@ def main():
     import numpy as np
     import torch
     torch.cuda.set_device(0)
-    system, k = make_kernel(SWEEPS_PER_STEP, seed=12345)
+    system, k = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
     rng = np.random.default_rng(0)
     k.setPaths(initial_paths(system, rng))
     for _ in range(args.warmup):
@@ -130,6 +133,7 @@ def main():
         "data": "synthetic: seeded Gaussian blobs (sigma 3 A) around a random triangle of side 3.5 A",
         "config": {"workload": "LM2M2 He3, 3D Cartesian, N=1024, beta=10/K, 16384 chains, S=6, xorshift128, "
                                "32 sweeps + 1 measurement per step", "sampler": "wave",
+                   "path_storage": args.storage or "auto",
                    "chains": W, "slices": N, "bisection_level": S},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfeynsimu
 FS_ABI_VERSION = 1
 FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
+FS_STORAGE_AUTO, FS_STORAGE_MEMORY, FS_STORAGE_REGISTERS = 0, 1, 2
 
 EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_build_log",
            "fs_source", "fs_run", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
@@ -27,7 +28,8 @@ class FsConfig(ctypes.Structure):
         ("fast_math", ctypes.c_int32), ("ranlux_int_range", ctypes.c_uint64), ("n_ops", ctypes.c_int32),
         ("n_corr", ctypes.c_int32), ("cor_lags", ctypes.c_int32), ("bins", ctypes.c_int32),
         ("bin_res", ctypes.c_int32), ("waves_per_block", ctypes.c_int32), ("beta", ctypes.c_double),
-        ("x_min", ctypes.c_double), ("x_max", ctypes.c_double), ("op_norm", ctypes.c_double)]
+        ("x_min", ctypes.c_double), ("x_max", ctypes.c_double), ("op_norm", ctypes.c_double),
+        ("path_storage", ctypes.c_int32)]
 
 
 class FsBuffers(ctypes.Structure):

@@ -23,11 +23,12 @@
 //    for the whole launch; a block of 64 consecutive slices is spread one
 //    slice per lane, so the 2^S-1 midpoints of a segment, the potential
 //    evaluations and the action sum all run in parallel across lanes and
-//    the segment never leaves registers.  Each lane has its own stream;
-//    the tiling offset of every step comes from a per-walker group stream
-//    (the ParallelizePath scheme of kernel.c:713-716).  The proposal and
-//    the acceptance rule are the reference's; only the assignment of
-//    random numbers differs.
+//    the segment never leaves registers.  Lane j always holds the slices
+//    congruent to j mod 64 and draws from its own stream; with tiling
+//    offset r it plays block position (j - r) mod 64.  The offset of every
+//    step comes from a per-walker group stream (the ParallelizePath scheme
+//    of kernel.c:713-716).  The proposal and the acceptance rule are the
+//    reference's; only the assignment of random numbers differs.
 #pragma once
 
 #include "fs_abi_args.h"
@@ -268,6 +269,15 @@ extern "C" __global__ void fs_exact_sweep(fs_kargs a)
 
 template <typename T> FS_DEV T fs_lane_read(T v, int src) { return __shfl(v, src, 64); }
 
+FS_DEV float fs_readlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+FS_DEV double fs_readlane(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Sum over aligned groups of G lanes (xor butterfly); every lane of a group
 // ends with the same bits.
 template <int G, typename T> FS_DEV T fs_group_sum(T v)
@@ -277,6 +287,19 @@ template <int G, typename T> FS_DEV T fs_group_sum(T v)
     return v;
 }
 
+// Path storage of the wave sampler.  When the walker's whole path fits in
+// registers (N >= 64, at most 16 blocks of 64 slices), lane j keeps slices
+// j, j + 64, ... for the whole launch together with the potential at each of
+// them, so a launch touches HBM once on entry and once on exit; the block
+// index is uniform, so the slot of a block is a register-indexed move.
+// Otherwise every block is loaded from and stored to the path in memory.
+#ifndef FS_RESIDENT
+#define FS_RESIDENT (FS_N >= 64 && FS_NB <= 16 && FS_NB * (FS_DOF + 1) * (FS_DOUBLE ? 2 : 1) <= 176)
+#endif
+#if FS_RESIDENT
+typedef fs_f fs_slots __attribute__((ext_vector_type(FS_NB)));
+#endif
+
 extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs a)
 {
     const int lane = threadIdx.x & 63;
@@ -285,12 +308,12 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
 
     fs_f *path = reinterpret_cast<fs_f *>(a.paths) + w * (long)(FS_N * FS_DOF);
     const bool active = lane < FS_B;
-    const int ip = lane & (FS_SEG - 1);        // position inside the segment
-    const int segBase = lane & ~(FS_SEG - 1);  // lane holding the segment's left end
-    // level at which this position is generated: S - ctz(ip)
-    const int lvl = ip ? FS_S - __builtin_ctz((unsigned)ip) : 0;
-    const fs_f sig = fs_sigma(lvl > 0 ? lvl : 1);
+    fs_f sigtab[FS_S + 1];
+#pragma unroll
+    for (int n = 1; n <= FS_S; n++) sigtab[n] = fs_sigma(n);
+    sigtab[0] = sigtab[1];
 
+    // lane j draws from stream 64 w + j, i.e. the stream follows the slice residue
     fs_stream rng, grp;
 #if FS_RANLUX
     rng.load(a.rl, w * 64 + lane);
@@ -310,17 +333,56 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
     fs_f *cbuf = reinterpret_cast<fs_f *>(fs_smem) + (threadIdx.x >> 6) * (FS_N * FS_NCORR);
 #endif
 
+#if FS_RESIDENT
+    fs_slots xs[FS_DOF];
+    fs_slots vc;
+#pragma unroll 1
+    for (int k = 0; k < FS_NB; k++) {
+        fs_f x[FS_DOF];
+        fs_gather(path, k * 64 + lane, x);
+#pragma unroll
+        for (int d = 0; d < FS_DOF; d++) xs[d][k] = x[d];
+        vc[k] = fs_potential_v(x);
+    }
+#endif
+
     for (int i = 0; i < a.operatorRuns; i++) {
         for (int j = 0; j < a.metroSteps; j++) {
-            const fs_u32 r = grp.integer() & (fs_u32)(FS_SEG - 1);
-            for (int g = 0; g < FS_NB; g++) {
-                const int base = (int)r + g * FS_B;
-                const int s = (base + (active ? lane : 0)) & (FS_N - 1);
-                const int sR = (base + FS_B) & (FS_N - 1);
-                fs_f xo[FS_DOF], xn[FS_DOF], xr[FS_DOF], z[FS_KDRAW];
-                fs_gather(path, s, xo);
-                fs_gather(path, sR, xr);
+            const int r = (int)(grp.integer() & (fs_u32)(FS_SEG - 1));
+            // this lane plays block position p; its slice is r + gB + p
+            const int p = active ? ((lane - r) & (FS_B - 1)) : lane;
+            const int ip = p & (FS_SEG - 1);
+            const bool wrap = lane < r;                 // slice lies in the next block of slots
+            const int toPos = (lane + r) & (FS_B - 1);  // lane that plays position `lane`
+            const int lvl = ip ? FS_S - __builtin_ctz((unsigned)ip) : 0;
+            fs_f sig = sigtab[0];
+#pragma unroll
+            for (int n = 1; n <= FS_S; n++) sig = (lvl == n) ? sigtab[n] : sig;
 
+#if FS_RESIDENT
+#pragma unroll 1
+#endif
+            for (int g = 0; g < FS_NB; g++) {
+                fs_f xo[FS_DOF], xn[FS_DOF], xr[FS_DOF], z[FS_KDRAW];
+                fs_f vOld;
+#if FS_RESIDENT
+                const int g1 = (g + 1) & (FS_NB - 1);
+#pragma unroll
+                for (int d = 0; d < FS_DOF; d++) {
+                    const fs_f a0 = xs[d][g], a1 = xs[d][g1];
+                    xo[d] = wrap ? a1 : a0;
+                    xr[d] = fs_readlane(a1, r);
+                }
+                {
+                    const fs_f v0 = vc[g], v1 = vc[g1];
+                    vOld = wrap ? v1 : v0;
+                }
+#else
+                const int s = (g * FS_B + lane + (wrap ? FS_B : 0)) & (FS_N - 1);
+                fs_gather(path, s, xo);
+                fs_gather(path, ((g + 1) * FS_B + r) & (FS_N - 1), xr);
+                vOld = fs_potential_v(xo);
+#endif
                 // every lane draws the same count so buffered streams stay in step
                 fs_f u[FS_KDRAW];
 #pragma unroll
@@ -338,45 +400,72 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
                     z[2 * k] = sig * sn;
                     z[2 * k + 1] = sig * cs;
                 }
-                const fs_f vOld = fs_potential_v(xo);
 #pragma unroll
                 for (int d = 0; d < FS_DOF; d++) xn[d] = xo[d];
 #pragma unroll
                 for (int n = 1; n <= FS_S; n++) {
                     const int h = FS_SEG >> n;
                     const bool gen = (ip & (2 * h - 1)) == h;
-                    const bool rightEnd = lane + h == FS_B;
+                    const bool rightEnd = p + h == FS_B;
 #pragma unroll
                     for (int d = 0; d < FS_DOF; d++) {
-                        const fs_f L = fs_lane_read(xn[d], lane - h);
-                        fs_f R = fs_lane_read(xn[d], (lane + h) & 63);
+                        const fs_f L = fs_lane_read(xn[d], (lane - h) & (FS_B - 1));
+                        fs_f R = fs_lane_read(xn[d], (lane + h) & (FS_B - 1));
                         R = rightEnd ? xr[d] : R;
                         if (gen) xn[d] = 0.5 * (L + R) + z[d];
                     }
                 }
                 const fs_f vNew = fs_potential_v(xn);
-                fs_f dv = (ip != 0 && active) ? vNew - vOld : (fs_f)0.0;
-                dv = fs_group_sum<FS_SEG>(dv);
-                const fs_f dS = eps * dv;
-                const fs_f ua = fs_lane_read(u[0], segBase);
-                const bool acc = !(exp(-dS) < ua);
-                if (acc && ip == 0 && active) nacc++;
-                if (acc && ip != 0 && active) {
+                const fs_f dv = (ip != 0 && active) ? vNew - vOld : (fs_f)0.0;
+                // segment sums and decisions in position space (lane L = position L)
+                const fs_f dvp = fs_group_sum<FS_SEG>(fs_lane_read(dv, toPos));
+                const fs_f u0p = fs_lane_read(u[0], toPos);
+                const fs_f ua = fs_lane_read(u0p, lane & ~(FS_SEG - 1));
+                const bool accp = !(exp(-eps * dvp) < ua);
+                if (accp && (lane & (FS_SEG - 1)) == 0 && active) nacc++;
+                const bool acc = fs_lane_read((int)accp, p) != 0;
+                const bool put = acc && ip != 0 && active;
+#if FS_RESIDENT
+                if constexpr (FS_NB == 1) {
+#pragma unroll
+                    for (int d = 0; d < FS_DOF; d++) xs[d][0] = put ? xn[d] : xs[d][0];
+                    vc[0] = put ? vNew : vc[0];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < FS_DOF; d++) {
+                        const fs_f a0 = xs[d][g], a1 = xs[d][g1];
+                        xs[d][g] = (put && !wrap) ? xn[d] : a0;
+                        xs[d][g1] = (put && wrap) ? xn[d] : a1;
+                    }
+                    const fs_f v0 = vc[g], v1 = vc[g1];
+                    vc[g] = (put && !wrap) ? vNew : v0;
+                    vc[g1] = (put && wrap) ? vNew : v1;
+                }
+#else
+                if (put) {
 #pragma unroll
                     for (int d = 0; d < FS_DOF; d++) path[d * FS_N + s] = xn[d];
                 }
+#endif
             }
         }
 
-        // ---- measurement over all N slices ----
+        // ---- measurement over all N slices (lane j: slices j, j + B, ...) ----
 #if FS_NOPS > 0 || FS_BINS || FS_NCORR > 0
         fs_f part[FS_NOPS1];
 #pragma unroll
         for (int k = 0; k < FS_NOPS1; k++) part[k] = (fs_f)0.0;
         if (active) {
-            for (int t = lane; t < FS_N; t += FS_B) {
+#pragma unroll 1
+            for (int k = 0; k < FS_NB; k++) {
+                const int t = k * FS_B + lane;
                 fs_f x[FS_DOF];
+#if FS_RESIDENT
+#pragma unroll
+                for (int d = 0; d < FS_DOF; d++) x[d] = xs[d][k];
+#else
                 fs_gather(path, t, x);
+#endif
 #if FS_NOPS > 0
                 fs_operator_v(x, part);
 #endif
@@ -406,8 +495,9 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
         }
 #endif
 #if FS_NCORR > 0
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         {
             double *cb = reinterpret_cast<double *>(a.corMeans) + w * (long)(FS_NCORR * FS_CORLAGS * 2);
 #pragma unroll
@@ -425,12 +515,21 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
                 }
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #endif
 #endif
     }
 
     // ---- launch outputs ----
+#if FS_RESIDENT
+#pragma unroll 1
+    for (int k = 0; k < FS_NB; k++) {
+#pragma unroll
+        for (int d = 0; d < FS_DOF; d++) path[d * FS_N + k * 64 + lane] = xs[d][k];
+    }
+#endif
     const fs_u32 tot = fs_group_sum<64>(nacc);
     if (lane == 0) a.accepts[w] = tot;
 #if FS_NOPS > 0

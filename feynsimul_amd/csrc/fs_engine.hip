@@ -116,6 +116,10 @@ int validate(const fs_config *c)
         if ((1 << c->S) > 64) return fail(FS_ERR_INVALID, "wave sampler needs 2^S <= 64");
         if (c->n_corr > 0 && (c->cor_lags < 1 || c->cor_lags > c->N / 2))
             return fail(FS_ERR_INVALID, "cor_lags must be in [1, N/2]");
+        if (c->path_storage < FS_STORAGE_AUTO || c->path_storage > FS_STORAGE_REGISTERS)
+            return fail(FS_ERR_INVALID, "unknown path_storage %d", c->path_storage);
+        if (c->path_storage == FS_STORAGE_REGISTERS && (c->N < 64 || c->N / 64 > 16))
+            return fail(FS_ERR_INVALID, "register storage needs 64 <= N <= 1024");
     } else {
         return fail(FS_ERR_INVALID, "unknown sampler %d", c->sampler);
     }
@@ -158,6 +162,8 @@ std::string build_source(const fs_config *c, const char *system_src, int wpb)
     defi("FS_LOCAL", c->local_size > 0 ? c->local_size : 1);
     defi("FS_WPB", wpb);
     defi("FS_SAMPLER", c->sampler);
+    if (c->path_storage == FS_STORAGE_MEMORY) defi("FS_RESIDENT", 0);
+    if (c->path_storage == FS_STORAGE_REGISTERS) defi("FS_RESIDENT", 1);
     def("eps", lit(c->beta / (double)c->N));
     s += R"(
 #ifndef M_PI

@@ -118,7 +118,11 @@ class PIMCKernel:
             if self._PSAlpha is None:
                 raise NameError('PSALpha need to be set if enablePathShift is True')
         if self._enableRanlux:
-            if self._luxuaryFactor is None or self._luxuaryFactor < 0:
+            if self._luxuaryFactor is None:
+                raise NameError('luxuaryFactor needs to be an uint >= 0.')
+            if not isinstance(self._luxuaryFactor, (int, np.integer)):
+                raise NameError('luxuaryFactor needs to be an integer.')
+            if self._luxuaryFactor < 0:
                 raise NameError('luxuaryFactor needs to be an uint >= 0.')
             try:
                 # the reference increments the value it was given and uses the
@@ -126,7 +130,11 @@ class PIMCKernel:
                 self._luxuaryFactor += 1
             except TypeError:
                 raise NameError('luxuaryFactor needs to be an integer.')
-            if self._ranluxIntMax is None or self._ranluxIntMax <= 0:
+            if self._ranluxIntMax is None:
+                raise NameError('ranluxIntMax needs to be an uint > 0.')
+            if not isinstance(self._ranluxIntMax, (int, np.integer)):
+                raise NameError('ranluxIntMax needs to be an integer.')
+            if self._ranluxIntMax <= 0:
                 raise NameError('ranluxIntMax needs to be an uint > 0.')
             try:
                 self._ranluxIntMax += 1
@@ -243,6 +251,12 @@ class PIMCKernel:
         cfg.x_min = float(self._xMin) if self._enableBins else 0.0
         cfg.x_max = float(self._xMax) if self._enableBins else 1.0
         cfg.op_norm = 1.0 / float(self._operatorRuns * self._N * self._nbrOfWalkers / float(self._nbrOfThreads))
+        storage = getattr(ka, "pathStorage", None)
+        storages = {None: _native.FS_STORAGE_AUTO, "memory": _native.FS_STORAGE_MEMORY,
+                    "registers": _native.FS_STORAGE_REGISTERS}
+        if storage not in storages:
+            raise ValueError("pathStorage must be 'registers', 'memory' or None")
+        cfg.path_storage = storages[storage]
         self._cfg = cfg
         src = system_section(self._system, self._operators if cfg.n_ops else (),
                              self._correlators if cfg.n_corr else ())

@@ -20,6 +20,10 @@ arguments at the end, all optional:
     (default True).
 ``wavesPerBlock``
     launch shape of the wave sampler (default automatic).
+``pathStorage``
+    where the wave sampler keeps each walker's path during a launch:
+    ``"registers"``, ``"memory"`` or ``None`` (automatic: registers when the
+    path fits).
 """
 
 
@@ -58,7 +62,8 @@ class KernelArgs:
                  sampler=None,
                  correlatorLags=None,
                  fastMath=True,
-                 wavesPerBlock=None):
+                 wavesPerBlock=None,
+                 pathStorage=None):
         self.system = system
         self.nbrOfWalkers = nbrOfWalkers
         self.N = N
@@ -92,3 +97,4 @@ class KernelArgs:
         self.correlatorLags = correlatorLags
         self.fastMath = fastMath
         self.wavesPerBlock = wavesPerBlock
+        self.pathStorage = pathStorage

@@ -45,6 +45,11 @@ enum {
 
 enum { FS_SAMPLER_EXACT = 0, FS_SAMPLER_WAVE = 1 };
 
+/* Where the wave sampler keeps a walker's path during a launch: AUTO picks
+ * registers when N >= 64, N <= 1024 and the path plus its potential cache fit
+ * in 176 VGPRs, else memory. */
+enum { FS_STORAGE_AUTO = 0, FS_STORAGE_MEMORY = 1, FS_STORAGE_REGISTERS = 2 };
+
 /* One configuration = one compiled module (the reference rebuilds its
  * program for every change of these, kernel.py:213-382). */
 typedef struct fs_config {
@@ -71,6 +76,7 @@ typedef struct fs_config {
     double beta;
     double x_min, x_max;
     double op_norm;             /* exact sampler: 1/(operatorRuns N walkers/threads) */
+    int32_t path_storage;       /* wave sampler: FS_STORAGE_* */
 } fs_config;
 
 /* Device buffers of one launch; unused entries may be NULL.  Layouts:

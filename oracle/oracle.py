@@ -147,6 +147,7 @@ def _declare(lib):
     lib.fs_oracle_run_wave.restype = ctypes.c_int
     lib.fs_oracle_xs_stream.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
     lib.fs_oracle_ranlux_init.argtypes = [vp, ctypes.c_int, vp]
+    lib.fs_oracle_ranlux_expand.argtypes = [vp, ctypes.c_int, vp]
     lib.fs_oracle_ranlux_stream.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     lib.fs_oracle_ranlux_stream_double.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
     lib.fs_oracle_potential.argtypes = [vp]
@@ -210,6 +211,37 @@ def ranlux_init(lib, seeds: np.ndarray) -> np.ndarray:
     st = np.zeros(seeds.shape[0], dtype=RANLUX_DTYPE)
     lib.fs_oracle_ranlux_init(_ptr(seeds), seeds.shape[0], _ptr(st))
     return st
+
+
+def ranlux_expand(lib, seeds: np.ndarray) -> np.ndarray:
+    """ranluxcl_init's seed expansion without the warm-up cycles."""
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    st = np.zeros(seeds.shape[0], dtype=RANLUX_DTYPE)
+    lib.fs_oracle_ranlux_expand(_ptr(seeds), seeds.shape[0], _ptr(st))
+    return st
+
+
+def ranlux_std_stream(state, nskip: int, count: int, exe: str, pre: int = 0) -> np.ndarray:
+    """Integer outputs of libstdc++'s ranlux24_base / discard_block_engine loaded with `state`
+    (a RANLUX_DTYPE record at cycle position 0), after discarding `pre` values."""
+    assert int(state["stepnr"]) == 0
+    lags = [int(round(float(v) * 16777216.0)) for v in state["s"][::-1]]  # x_{i-24} .. x_{i-1}
+    carry = 1 if float(state["carry"]) != 0.0 else 0
+    txt = "%d %d %d %s %d\n" % (nskip, count, pre, " ".join(map(str, lags)), carry)
+    res = subprocess.run([exe], input=txt, capture_output=True, text=True, check=True)
+    return np.array([int(v) for v in res.stdout.split()], dtype=np.int64)
+
+
+def std_check_exe() -> str:
+    """Build (once) the libstdc++ RANLUX cross-check program."""
+    out = os.path.join(BUILD_DIR, "ranlux_std_check")
+    src = os.path.join(HERE, "ranlux_std_check.cpp")
+    if not os.path.exists(out) or os.path.getmtime(out) < os.path.getmtime(src):
+        os.makedirs(BUILD_DIR, exist_ok=True)
+        tmp = out + ".tmp%d" % os.getpid()
+        subprocess.run(["g++", "-O2", "-std=c++17", src, "-o", tmp], check=True)
+        os.replace(tmp, out)
+    return out
 
 
 def ranlux_stream(lib, states: np.ndarray, nskip: int, nquad: int, double=False) -> np.ndarray:

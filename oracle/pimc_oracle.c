@@ -161,7 +161,7 @@ static void fs_rl_next4_double(fs_ranlux *r, int nskip, double out[4])
 
 /* ranluxcl_init: xorshift64 (13, 7, 17) expansion of a 64-bit seed into 24
  * 24-bit fractions, then 32 x 24 warm-up steps (pyopencl-ranluxcl.cl:597-725). */
-static void fs_rl_init(uint64_t x, fs_ranlux *r)
+static void fs_rl_expand(uint64_t x, fs_ranlux *r)
 {
     const float inv = (float)(1.0 / 16777216.0);
     for (int blk = 0; blk < 3; blk++) {
@@ -184,6 +184,11 @@ static void fs_rl_init(uint64_t x, fs_ranlux *r)
     r->stepnr = 0;
     r->carry = 0.0f;
     if (r->s[23] == 0.0f) r->carry = FS_TWOM24;
+}
+
+static void fs_rl_init(uint64_t x, fs_ranlux *r)
+{
+    fs_rl_expand(x, r);
     for (int i = 0; i < 32; i++)
         for (int t = 0; t < 24; t++) fs_rl_step(r, t);
 }
@@ -238,6 +243,12 @@ FS_EXPORT void fs_oracle_xs_stream(const uint32_t *rows, int nrows, int ndraw,
 FS_EXPORT void fs_oracle_ranlux_init(const uint64_t *seeds, int n, fs_ranlux *states)
 {
     for (int i = 0; i < n; i++) fs_rl_init(seeds[i], &states[i]);
+}
+
+/* the seed expansion of ranluxcl_init without its warm-up cycles */
+FS_EXPORT void fs_oracle_ranlux_expand(const uint64_t *seeds, int n, fs_ranlux *states)
+{
+    for (int i = 0; i < n; i++) fs_rl_expand(seeds[i], &states[i]);
 }
 
 FS_EXPORT void fs_oracle_ranlux_stream(fs_ranlux *states, int n, int nskip, int nquad,
@@ -551,17 +562,19 @@ FS_EXPORT int fs_oracle_run_reference(const fs_oracle_cfg *c, FLOAT_TYPE *paths,
 /* of a walker is moved once per step; the tiling offset r in [0, 2^S) is    */
 /* shared), with a different assignment of random numbers:                   */
 /*  - walker w owns a group stream (row w of the group table) that supplies  */
-/*    r = int & (2^S - 1) once per step, and 64 lane streams (rows 64w + p);  */
+/*    r = int & (2^S - 1) once per step, and 64 lane streams (rows 64w + q);  */
 /*  - slices are handled in blocks of B = min(64, N) consecutive slices      */
-/*    starting at r + gB; lane p holds slice r + gB + p, segment p / 2^S,    */
-/*    segment position i = p % 2^S;                                          */
-/*  - every lane draws 2 ceil(DOF/2) uniforms per block; a lane with i != 0  */
-/*    uses them as pairs (u, v) and forms both Box-Muller normals             */
-/*    rho sin(2 pi u), rho cos(2 pi u) for coordinates 2k and 2k+1; the lane  */
-/*    with i == 0 (the fixed left end) keeps its first draw as the segment's  */
-/*    acceptance uniform;                                                    */
+/*    starting at r + gB; position p of a block is slice r + gB + p, in      */
+/*    segment p / 2^S at segment position i = p % 2^S;                       */
+/*  - every lane stream draws 2 ceil(DOF/2) uniforms per block; position p   */
+/*    takes the draws of stream (r + p) mod B, the residue of its slice, so  */
+/*    a slice keeps its stream whatever the offset; a position with i != 0   */
+/*    uses them as pairs (u, v) and forms both Box-Muller normals            */
+/*    rho sin(2 pi u), rho cos(2 pi u) for coordinates 2k and 2k+1; the      */
+/*    position with i == 0 (the fixed left end) keeps its first draw as the  */
+/*    segment's acceptance uniform;                                          */
 /*  - the action change of a segment is epsv * sum_i (V_new - V_old), summed   */
-/*    by an xor butterfly over the segment's lanes (endpoint contributes 0). */
+/*    by an xor butterfly over the segment's positions (the end adds 0).     */
 /* Per-walker outputs: accepted move count, FP64 operator means, FP64         */
 /* correlator bins.                                                          */
 /* ------------------------------------------------------------------------- */
@@ -625,17 +638,19 @@ FS_EXPORT int fs_oracle_run_wave(const fs_oracle_cfg *c, FLOAT_TYPE *paths, uint
                     int base = (int)r + g * B;
                     for (int p = 0; p < B; p++) gather(path, N, (base + p) % N, xo[p]);
                     gather(path, N, (base + B) % N, xr);
-                    /* every one of the 64 lanes draws KDRAW uniforms per block, so
-                     * buffered streams stay in step; interior lanes use them as
-                     * Box-Muller pairs, a segment's left-end lane keeps the first
-                     * as the acceptance uniform */
+                    /* every one of the 64 streams draws KDRAW uniforms per block, so
+                     * buffered streams stay in step; position p uses the stream of
+                     * its slice residue (r + p) mod B; interior positions use the
+                     * draws as Box-Muller pairs, a segment's left end keeps the
+                     * first as the acceptance uniform */
                     const int KDRAW = 2 * ((FS_DOF + 1) / 2);
+                    FLOAT_TYPE ust[64][FS_DOF + 1];
                     FLOAT_TYPE ufirst[64];
-                    for (int p = 0; p < 64; p++) {
-                        FLOAT_TYPE u[FS_DOF + 1];
-                        for (int k = 0; k < KDRAW; k++) u[k] = st_float(&lane[p]);
+                    for (int q = 0; q < 64; q++)
+                        for (int k = 0; k < KDRAW; k++) ust[q][k] = st_float(&lane[q]);
+                    for (int p = 0; p < B; p++) {
+                        const FLOAT_TYPE *u = ust[((int)r + p) % B];
                         ufirst[p] = u[0];
-                        if (p >= B) continue;
                         int ip = p & (SEG - 1);
                         memcpy(xn[p], xo[p], sizeof(xo[p]));
                         dv[p] = 0.0;

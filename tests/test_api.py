@@ -1,0 +1,175 @@
+"""Host-side boundary: KernelArgs / PIMCKernel validation and the C ABI (CPU only).
+
+Validation branches mirror FeynSimul/kernel.py:93-198 (exception types and
+messages); they are all raised before any device work, so they run here.
+The C-ABI tests load libfeynsimul.so, check every declared export and compile
+sampler modules for gfx950 with hiprtc (no GPU needed to compile).
+"""
+import inspect
+import os
+import re
+
+import pytest
+
+from feynsimul_amd.kernel_args import KernelArgs
+from feynsimul_amd.kernel import PIMCKernel, humanReadableSize
+from feynsimul_amd.physical_systems import HarmOsc, Lm2m2Cluster, AnharmOsc
+from feynsimul_amd.system_source import system_section
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _ka(**kw):
+    s = HarmOsc()
+    base = dict(system=s, nbrOfWalkers=64, N=64, beta=5.0, S=3, enableOperator=True, enableCorrelator=False,
+                enableBisection=True, enablePathShift=False, enableSingleNodeMove=False,
+                enableParallelizePath=True, enableGlobalPath=False, enableGlobalOldPath=False, enableBins=False,
+                metroStepsPerOperatorRun=2, operatorRuns=2, nbrOfWalkersPerWorkGroup=1,
+                operators=(s.energyOp,))
+    base.update(kw)
+    return KernelArgs(**base)
+
+
+def test_kernel_args_signature_matches_reference():
+    params = list(inspect.signature(KernelArgs.__init__).parameters)[1:]
+    ref = ["system", "nbrOfWalkers", "N", "beta", "enableOperator", "enableCorrelator", "enableBisection",
+           "enablePathShift", "enableSingleNodeMove", "enableBins", "nbrOfWalkersPerWorkGroup", "enableDouble",
+           "enableRanlux", "luxuaryFactor", "ranluxIntMax", "S", "alpha", "PSAlpha", "enableParallelizePath",
+           "metroStepsPerOperatorRun", "enableGlobalPath", "enableGlobalOldPath", "operators", "operatorRuns",
+           "correlators", "xMin", "xMax", "binResolutionPerDOF"]
+    assert params[: len(ref)] == ref
+    ka = _ka()
+    ka.enableBins = True  # mutable bag, as the tutorial uses it
+    assert ka.enableBins
+
+
+@pytest.mark.parametrize("kw,exc,msg", [
+    (dict(beta=5), TypeError, "beta need to be a float"),
+    (dict(operatorRuns=None), NameError, "operatorRuns need to be set"),
+    (dict(operators=None), NameError, "operators need to be set"),
+    (dict(metroStepsPerOperatorRun=None), NameError, "metroStepsPerOperatorRun need to be set"),
+    (dict(operators=["x1"]), TypeError, "operators need to be a tuple"),
+    (dict(enableCorrelator=True), NameError, "correlators need to be set"),
+    (dict(enableCorrelator=True, correlators="x1"), TypeError, "correlators need to be a tuple"),
+    (dict(enableSingleNodeMove=True, enableParallelizePath=False), NameError, "alpha need to be set"),
+    (dict(enablePathShift=True, enableParallelizePath=False), NameError, "PSALpha need to be set"),
+    (dict(enableRanlux=True), NameError, "luxuaryFactor needs to be an uint"),
+    (dict(enableRanlux=True, luxuaryFactor="2"), NameError, "luxuaryFactor"),
+    (dict(enableRanlux=True, luxuaryFactor=2), NameError, "ranluxIntMax needs to be an uint"),
+    (dict(enableRanlux=True, luxuaryFactor=4, ranluxIntMax=10), Exception, "nskip is negative"),
+    (dict(enableBins=True), NameError, "xMin need to be set"),
+    (dict(enableBins=True, xMin=-1.0), NameError, "xMax need to be set"),
+    (dict(enableBins=True, xMin=-1.0, xMax=1.0), NameError, "binResolutionPerDOF need to be set"),
+    (dict(nbrOfWalkers=63, nbrOfWalkersPerWorkGroup=4), Exception, "multiple"),
+    (dict(nbrOfWalkers=2, nbrOfWalkersPerWorkGroup=4), Exception, "multiple"),
+    (dict(N=48), Exception, "power of 2"),
+    (dict(S=None), NameError, "S need to be set"),
+    (dict(S=7), Exception, "2^S must be less than or equal to N"),
+    (dict(enableGlobalPath=None), NameError, "enableGlobalPath need to be set"),
+    (dict(enableGlobalOldPath=None), NameError, "enableGlobalOldPath need to be set"),
+    (dict(enableParallelizePath=None), NameError, "enableParallelizePath need to be set"),
+    (dict(enableSingleNodeMove=True, alpha=0.5), Exception, "Single node move not possible"),
+    (dict(enablePathShift=True, PSAlpha=0.5), Exception, "Shift path not possible"),
+    (dict(enableBisection=False, enableGlobalOldPath=True), Exception, "Old Path is not used"),
+])
+def test_validation_matches_reference(kw, exc, msg):
+    with pytest.raises(exc, match=re.escape(msg)):
+        PIMCKernel(_ka(**kw))
+
+
+def test_no_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    with pytest.raises(RuntimeError, match="needs a GPU"):
+        PIMCKernel(_ka())
+
+
+def test_human_readable_size():
+    assert humanReadableSize(592) == "592 B"
+    assert humanReadableSize(1024 ** 3) == "1 GiB"
+
+
+# ----------------------------------------------------------------------------
+# C ABI
+# ----------------------------------------------------------------------------
+def _declared_functions():
+    with open(os.path.join(ROOT, "include", "feynsimul.h")) as f:
+        txt = f.read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fs_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from feynsimul_amd import _native
+    L = _native.lib()
+    declared = _declared_functions()
+    assert set(declared) == set(_native.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.fs_abi_version() == _native.FS_ABI_VERSION
+
+
+def _cfg(system, N, S, sampler, nops=1, ncorr=0, **kw):
+    from feynsimul_amd import _native
+    c = _native.FsConfig()
+    c.abi_version, c.sampler, c.N, c.S, c.DOF = _native.FS_ABI_VERSION, sampler, N, S, system.DOF
+    c.walkers_per_group, c.n_walkers, c.parallelize_path, c.local_size = 1, 64, 1, 1
+    c.fast_math, c.ranlux_int_range, c.n_ops, c.n_corr, c.cor_lags = 1, 2 ** 31, nops, ncorr, N // 2
+    c.bin_res, c.beta, c.x_max, c.op_norm = 1, 10.0, 1.0, 1.0
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+@pytest.mark.parametrize("name", ["ho", "aho", "lm2m2-4"])
+def test_module_compiles_for_gfx950(name, sampler):
+    from feynsimul_amd import _native
+    system = {"ho": HarmOsc(), "aho": AnharmOsc(0.1), "lm2m2-4": Lm2m2Cluster(4)}[name]
+    ops = (system.energyOp,)
+    corr = ("x1",) if name != "lm2m2-4" else ()
+    e = _native.Engine(_cfg(system, 256, 4, sampler, ncorr=len(corr), bins=int(name == "ho"), bin_res=8,
+                            x_min=-2.0, x_max=2.0),
+                       system_section(system, ops, corr), compile_only=True)
+    assert "fs_sampler.h" in e.source
+    e.close()
+
+
+def test_compile_error_reports_build_log():
+    from feynsimul_amd import _native
+
+    class Broken:
+        DOF = 1
+        potential = "x1 +* 2"
+        userCode = ""
+    with pytest.raises(_native.FsError) as ei:
+        _native.Engine(_cfg(Broken(), 64, 3, 1), system_section(Broken(), ("x1",)), compile_only=True)
+    assert ei.value.code == _native.FS_ERR_COMPILE
+    assert "error" in str(ei.value)
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("N", 48, "power of 2"),
+    ("S", 7, "2^S"),
+    ("abi_version", 99, "ABI version"),
+    ("sampler", 5, "unknown sampler"),
+    ("cor_lags", 0, "cor_lags"),
+])
+def test_invalid_config_rejected(field, value, msg):
+    from feynsimul_amd import _native
+    c = _cfg(HarmOsc(), 64, 3, 1, ncorr=1)
+    setattr(c, field, value)
+    with pytest.raises(_native.FsError, match=re.escape(msg)) as ei:
+        _native.Engine(c, system_section(HarmOsc(), ("x1",), ("x1",)), compile_only=True)
+    assert ei.value.code == _native.FS_ERR_INVALID
+
+
+def test_device_entry_points_need_a_gpu():
+    import torch
+    from feynsimul_amd import _native
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    assert _native.device_count() == 0
+    rc = _native.lib().fs_seed_xorshift(8, 1, 0, 0, None)  # any non-null pointer; rejected before use
+    assert rc == _native.FS_ERR_NO_DEVICE

@@ -41,7 +41,7 @@ def _initial_paths(system, W, N, rng):
     return rng.uniform(-1.0, 1.0, (W, system.DOF * N))
 
 
-def _ka(system, W, N, S, beta, opRuns, steps, parallel, ranlux, double, sampler, Wg=1, seed=11):
+def _ka(system, W, N, S, beta, opRuns, steps, parallel, ranlux, double, sampler, Wg=1, seed=11, storage=None):
     return KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=S, enableOperator=True,
                       enableCorrelator=False, enableBisection=True, enablePathShift=False,
                       enableSingleNodeMove=False, enableParallelizePath=parallel, enableGlobalPath=False,
@@ -49,7 +49,7 @@ def _ka(system, W, N, S, beta, opRuns, steps, parallel, ranlux, double, sampler,
                       enableRanlux=ranlux, luxuaryFactor=1 if ranlux else None,
                       ranluxIntMax=2 ** 31 - 1 if ranlux else None, metroStepsPerOperatorRun=steps,
                       operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=Wg, operators=(system.energyOp,),
-                      seed=seed, sampler=sampler, fastMath=False)
+                      seed=seed, sampler=sampler, fastMath=False, pathStorage=storage)
 
 
 def _compare(gp, rp, acc, racc, walker_of_thread, tol, min_match=0.95):
@@ -123,25 +123,30 @@ def test_exact_sampler_matches_reference_order(gpu, case):
 
 
 CASES_WAVE = [
-    # name, W, N, S, beta, ranlux, double
-    ("ho", 64, 256, 4, 10.0, False, False),
-    ("ho", 32, 32, 3, 4.0, False, False),
-    ("ho", 32, 128, 6, 10.0, True, False),
-    ("aho", 32, 256, 5, 10.0, False, True),
-    ("lm2m2", 16, 256, 6, 10.0, False, False),
+    # name, W, N, S, beta, ranlux, double, path storage
+    ("ho", 64, 256, 4, 10.0, False, False, "registers"),
+    ("ho", 64, 256, 4, 10.0, False, False, "memory"),
+    ("ho", 32, 32, 3, 4.0, False, False, None),
+    ("ho", 32, 128, 6, 10.0, True, False, "registers"),
+    ("ho", 32, 128, 6, 10.0, True, False, "memory"),
+    ("aho", 32, 256, 5, 10.0, False, True, None),
+    ("aho", 16, 2048, 4, 20.0, False, False, None),
+    ("lm2m2", 16, 256, 6, 10.0, False, False, "registers"),
+    ("lm2m2", 16, 1024, 6, 10.0, False, False, "registers"),
+    ("lm2m2", 16, 1024, 6, 10.0, False, False, "memory"),
 ]
 
 
-@pytest.mark.parametrize("case", CASES_WAVE, ids=lambda c: "%s-W%d-N%d-S%d-rl%d-f64%d" % (
-    c[0], c[1], c[2], c[3], c[5], c[6]))
+@pytest.mark.parametrize("case", CASES_WAVE, ids=lambda c: "%s-W%d-N%d-S%d-rl%d-f64%d-%s" % (
+    c[0], c[1], c[2], c[3], c[5], c[6], c[7]))
 def test_wave_sampler_matches_oracle(gpu, case):
-    name, W, N, S, beta, rl, dbl = case
+    name, W, N, S, beta, rl, dbl, storage = case
     _case("wave " + str(case))
     system = _system(name)
     rng = np.random.default_rng(5)
     p0 = _initial_paths(system, W, N, rng)
     opRuns, steps = 3, 2
-    k = PIMCKernel(_ka(system, W, N, S, beta, opRuns, steps, True, rl, dbl, "wave"))
+    k = PIMCKernel(_ka(system, W, N, S, beta, opRuns, steps, True, rl, dbl, "wave", storage=storage))
     if rl:
         k.initRanlux()
         k.groupInitRanlux()

@@ -1,0 +1,324 @@
+"""Pins of the CPU oracle (oracle/pimc_oracle.c) -- CPU only.
+
+What fixes each oracle function, so that a plausible mistake fails a check:
+
+* xorshift128 (kernel.c:214-239): Marsaglia's published xor128 outputs for
+  his default seed; the FP32 randFloat end points ([0, 1] inclusive).
+* RANLUX SWB core and luxury skipping (pyopencl-ranluxcl.cl:363-557):
+  libstdc++'s std::ranlux24_base / std::discard_block_engine loaded with the
+  same lagged state produce the same 24-bit stream (small values checked on
+  their top 24 bits, the low-bit fill below 2^-12 is RANLUXCL's own).
+* ranluxcl_init (pyopencl-ranluxcl.cl:597-725): the expansion equals the
+  192-bit concatenation of three xorshift64(13, 7, 17) outputs cut into
+  24-bit fields, and the warm-up is 32 x 24 steps of the std engine.
+* ranluxcl64: 48-bit composition of two float outputs, checked against the
+  float stream.
+* bisection move + Metropolis rule (kernel.c:258-399): the reference's own
+  printed acceptance rates (doc/tutorial/index.rst:239-256); free particle
+  (V = 0) accepts every move; tiny paths against exact Gaussian covariances
+  and transfer-matrix quadrature, for both orderings.
+* HO and AHO estimators: closed forms and FP64 transfer matrices.
+* LM2M2 potential: literature values of the formula and finite-difference
+  derivatives of the virial operator.
+Parity unpinned: none of the functions above.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from feynsimul_amd.physical_systems import HarmOsc, AnharmOsc, Lm2m2Cluster
+from feynsimul_amd.physical_systems.lm2m2 import lm2m2_kelvin, mass_scale
+
+
+class _Free:
+    DOF = 1
+    potential = "0.0"
+    userCode = ""
+    energyOp = "x1 * x1"
+
+
+@pytest.fixture(scope="module")
+def ho_lib():
+    return O.build(HarmOsc(), 64, 5.0, operators=("1.0 * x1 * x1",))
+
+
+# ----------------------------------------------------------------------------
+# xorshift128
+# ----------------------------------------------------------------------------
+def test_xorshift_marsaglia_known_answer(ho_lib):
+    rows = np.array([[123456789, 362436069, 521288629, 88675123]], dtype=np.uint32)
+    ints, flt = O.xs_stream(ho_lib, rows, 5)
+    assert list(ints[0]) == [3701687786, 458299110, 2500872618, 3633119408, 516391518]
+    np.testing.assert_array_equal(flt[0], (ints[0].astype(np.float32) * np.float32(2.0 ** -32)))
+
+
+def test_xorshift_float_endpoints(ho_lib):
+    # state whose next w is 0 and one whose next w is 2^32 - 1: x = y = z = 0
+    rows = np.array([[0, 0, 0, 0], [0, 0, 0, 0xFFFFFFFF]], dtype=np.uint32)
+    ints, flt = O.xs_stream(ho_lib, rows, 1)
+    assert ints[0, 0] == 0 and flt[0, 0] == 0.0
+    assert ints[1, 0] == 0xFFFFFFFF ^ (0xFFFFFFFF >> 19)
+    # float32(w) rounds to 2^32 for w >= 2^32 - 128, so randFloat reaches 1.0
+    assert np.float32(np.uint32(0xFFFFFF80)) * np.float32(2.0 ** -32) == 1.0
+    assert flt[1, 0] == np.float32(np.uint32(ints[1, 0])) * np.float32(2.0 ** -32)
+
+
+def test_xorshift_double_conversion():
+    lib = O.build(HarmOsc(), 64, 5.0, operators=("1.0 * x1 * x1",), double=True)
+    rows = O.xorshift_rows(3, 0, 4)
+    ints, flt = O.xs_stream(lib, rows, 50)
+    np.testing.assert_array_equal(flt, ints.astype(np.float64) * 2.328306437080797e-10)
+
+
+def test_seed_rows_nonzero_and_distinct():
+    rows = O.xorshift_rows(0, 0, 4096)
+    assert (rows != 0).any(axis=1).all()
+    assert len({tuple(r) for r in rows}) == 4096
+
+
+# ----------------------------------------------------------------------------
+# RANLUX
+# ----------------------------------------------------------------------------
+def _top24(v):
+    return np.floor(v.astype(np.float64) * 16777216.0).astype(np.int64)
+
+
+@pytest.mark.parametrize("lux", [0, 1, 2, 3, 4])
+def test_ranlux_matches_libstdcxx(ho_lib, lux):
+    exe = O.std_check_exe()
+    seeds = O.ranlux_seeds(2024, 0, 3)
+    st = O.ranlux_init(ho_lib, seeds)
+    nquad = 250
+    for i in range(3):
+        std = O.ranlux_std_stream(st[i], O.nskip_of_lux(lux), nquad * 4, exe)
+        ours = O.ranlux_stream(ho_lib, st[i:i + 1].copy(), O.nskip_of_lux(lux), nquad)[0]
+        np.testing.assert_array_equal(_top24(ours), std)
+        small = ours < 2.0 ** -12
+        # above 2^-12 the output is exactly the 24-bit state value
+        np.testing.assert_array_equal(ours[~small].astype(np.float64) * 16777216.0, std[~small])
+
+
+def test_ranlux_small_value_fill(ho_lib):
+    # a state of zeros except the lag-10 entry produces tiny values that get the fill
+    st = np.zeros(1, dtype=O.RANLUX_DTYPE)
+    st["s"][0, 9] = 1.0 / 16777216.0  # s10
+    st["s"][0, 8] = 0.5               # s09, supplies the low bits
+    out = O.ranlux_stream(ho_lib, st, 0, 1)[0]
+    assert out[0] == np.float32(2.0 ** -24 + 2.0 ** -24 * 0.5)
+
+
+def test_ranlux_init_expansion_and_warmup(ho_lib):
+    exe = O.std_check_exe()
+    for x in (1, 0x123456789ABCDEF, (1 << 64) - 1):
+        seeds = np.array([x], dtype=np.uint64)
+        ex = O.ranlux_expand(ho_lib, seeds)[0]
+        # three xorshift64 (13, 7, 17) outputs, concatenated, cut into 8 fields per output triple
+        M = (1 << 64) - 1
+        v = x
+        fields = []
+        for _ in range(3):
+            words = []
+            for _ in range(3):
+                v ^= (v << 13) & M
+                v ^= v >> 7
+                v ^= (v << 17) & M
+                words.append(v)
+            big = (words[0] << 128) | (words[1] << 64) | words[2]
+            fields += [(big >> (192 - 24 * (k + 1))) & 0xFFFFFF for k in range(8)]
+        np.testing.assert_array_equal(np.round(ex["s"] * 16777216.0).astype(np.int64), fields)
+        assert float(ex["carry"]) == (2.0 ** -24 if fields[23] == 0 else 0.0)
+        warm = O.ranlux_init(ho_lib, seeds)[0]
+        # the warm-up is 32 x 24 SWB steps: a std engine that discards 768 values
+        # then emits 24 reproduces the warmed state's next 24 values
+        std = O.ranlux_std_stream(ex, 0, 24, exe, pre=768)
+        ours = O.ranlux_stream(ho_lib, np.array([warm]), 0, 6)[0]
+        np.testing.assert_array_equal(_top24(ours), std)
+
+
+def test_ranlux_double_composition():
+    lib64 = O.build(HarmOsc(), 64, 5.0, operators=("1.0 * x1 * x1",), double=True)
+    lib32 = O.build(HarmOsc(), 64, 5.0, operators=("1.0 * x1 * x1",))
+    st = O.ranlux_init(lib32, O.ranlux_seeds(9, 0, 2))
+    f = O.ranlux_stream(lib32, st.copy(), 76, 40)
+    d = O.ranlux_stream(lib64, st.copy(), 76, 20, double=True)
+    tiny = np.float32(2.0 ** -48)
+    f = f.astype(np.float64).reshape(2, -1, 2)
+    hi = np.where(f[..., 0] == tiny, 0.0, f[..., 0])
+    np.testing.assert_array_equal(d.reshape(2, -1), hi + f[..., 1] / 16777216.0)
+
+
+# ----------------------------------------------------------------------------
+# Sampler laws
+# ----------------------------------------------------------------------------
+def _ar_ref(N, S, beta, walkers=448, steps=10, runs=64, seed=1):
+    sysobj = HarmOsc()
+    lib = O.build(sysobj, N, beta, operators=(sysobj.energyOp,))
+    cfg = O.make_cfg(N, S, 1, walkers, beta, runs, steps, walkersPerGroup=4, nOps=1)
+    rng = np.random.default_rng(seed)
+    paths = np.ascontiguousarray(np.repeat(rng.uniform(-0.01, 0.01, (walkers, 1)), N, axis=1).astype(np.float32))
+    seeds = O.xorshift_rows(seed, 0, cfg.nThreads + 1)
+    O.run_reference(lib, cfg, paths, seeds=seeds)  # burn-in
+    out = O.run_reference(lib, cfg, paths, seeds=seeds)
+    T = cfg.nThreads // walkers
+    e = out["opMeans"].reshape(walkers, T).mean(axis=1)
+    return out["accepts"].mean() / (runs * steps), e.mean(), e.std() / math.sqrt(walkers)
+
+
+@pytest.mark.parametrize("N,S,lo,hi,e_exact", [
+    (8, 1, 0.5569 - 0.004, 0.5612 + 0.004, 0.312347556605),
+    (16, 2, 0.4741 - 0.004, 0.4792 + 0.004, 0.423999157335),
+    (64, 2, 0.8923 - 0.002, 0.8934 + 0.002, 0.494006019083),
+])
+def test_acceptance_matches_reference_tutorial(N, S, lo, hi, e_exact):
+    """doc/tutorial/index.rst:239-256: the reference's modN log at beta = 20."""
+    ar, e, se = _ar_ref(N, S, 20.0)
+    assert lo <= ar <= hi, "acceptance %.4f outside the reference's [%.4f, %.4f]" % (ar, lo, hi)
+    assert abs(O.ho_discrete_x2(N, 20.0) - e_exact) < 1e-11
+    assert abs(e - e_exact) < 4 * se
+
+
+def test_free_particle_always_accepts():
+    lib = O.build(_Free(), 32, 4.0, operators=("x1 * x1",))
+    cfg = O.make_cfg(32, 3, 1, 16, 4.0, 3, 5, nOps=1)
+    paths = np.zeros((16, 32), np.float32)
+    out = O.run_reference(lib, cfg, paths, seeds=O.xorshift_rows(5, 0, cfg.nThreads + 1))
+    assert (out["accepts"] == 15).all()
+    lane = O.xorshift_rows(6, 0, 16 * 64)
+    grp = O.xorshift_rows(6, 16 * 64, 16)
+    out = O.run_wave(lib, O.make_cfg(32, 3, 1, 16, 4.0, 3, 5, nOps=1), np.zeros((16, 32), np.float32),
+                     lane_seeds=lane, group_seeds=grp)
+    assert (out["accepts"] == 15 * (32 // 8)).all()
+
+
+def test_free_bridge_midpoint_law():
+    """V = 0, one level: the midpoint of a segment is N(mean of ends, sigma_1^2),
+    sigma_1^2 = 2^S eps / 4 (kernel.c:613-618)."""
+    N, S, beta, W = 2, 1, 3.0, 20000
+    lib = O.build(_Free(), N, beta, operators=("x1 * x1",))
+    cfg = O.make_cfg(N, S, 1, W, beta, 1, 1, parallelizePath=False, walkersPerGroup=1, nOps=1)
+    paths = np.zeros((W, N), np.float32)
+    O.run_reference(lib, cfg, paths, seeds=O.xorshift_rows(8, 0, W + 1))
+    # after one move one slice is still 0 and the other is the Levy midpoint
+    moved = paths.sum(axis=1).astype(np.float64)
+    var = 2 * (beta / N) / 4
+    assert abs(moved.mean()) < 4 * math.sqrt(var / W)
+    assert abs(moved.var() / var - 1) < 4 * math.sqrt(2.0 / W)
+    # chi-square on 20 equiprobable bins of the standard normal
+    from scipy import stats
+    z = moved / math.sqrt(var)
+    edges = stats.norm.ppf(np.linspace(0, 1, 21))
+    counts, _ = np.histogram(z, edges)
+    chi2 = ((counts - W / 20) ** 2 / (W / 20)).sum()
+    assert chi2 < stats.chi2.ppf(0.999, 19)
+
+
+def _cov_exact(N, beta, k=1.0):
+    eps = beta / N
+    A = np.zeros((N, N))
+    for i in range(N):
+        A[i, i] = 2 / eps + eps * k
+        A[i, (i + 1) % N] -= 1 / eps
+        A[i, (i - 1) % N] -= 1 / eps
+    return np.linalg.inv(A)
+
+
+@pytest.mark.parametrize("mode", ["reference", "wave"])
+def test_tiny_path_covariance(mode):
+    """N = 4, S = 2: the stationary law is the Gaussian of the discretised action."""
+    N, S, beta, W = 4, 2, 2.0, 4096
+    sysobj = HarmOsc()
+    lib = O.build(sysobj, N, beta, operators=("x1 * x1",))
+    cfg = O.make_cfg(N, S, 1, W, beta, 1, 40, nOps=1)
+    rng = np.random.default_rng(0)
+    paths = np.ascontiguousarray(rng.normal(0, 0.5, (W, N)).astype(np.float32))
+    acc = np.zeros((N, N))
+    if mode == "reference":
+        seeds = O.xorshift_rows(1, 0, cfg.nThreads + 1)
+        step = lambda: O.run_reference(lib, cfg, paths, seeds=seeds)
+    else:
+        lane = O.xorshift_rows(1, 0, W * 64)
+        grp = O.xorshift_rows(1, W * 64, W)
+        step = lambda: O.run_wave(lib, cfg, paths, lane_seeds=lane, group_seeds=grp)
+    step()
+    n = 0
+    for _ in range(30):
+        step()
+        p = paths.astype(np.float64)
+        acc += p.T @ p / W
+        n += 1
+    cov = acc / n
+    C = _cov_exact(N, beta)
+    # 4096 x 30 samples, strongly correlated in time; a 5% band catches wrong widths/actions
+    np.testing.assert_allclose(cov, C, rtol=0.05, atol=0.01)
+
+
+@pytest.mark.parametrize("mode", ["reference", "wave"])
+def test_anharmonic_vs_transfer_matrix(mode):
+    N, S, beta, W = 16, 2, 4.0, 2048
+    sysobj = AnharmOsc(0.5)
+    lib = O.build(sysobj, N, beta, operators=(sysobj.positionSquaredOp,))
+    cfg = O.make_cfg(N, S, 1, W, beta, 20, 10, nOps=1)
+    paths = np.zeros((W, N), np.float32)
+    if mode == "reference":
+        seeds = O.xorshift_rows(2, 0, cfg.nThreads + 1)
+        step = lambda: O.run_reference(lib, cfg, paths, seeds=seeds)
+        per = lambda out: out["opMeans"].reshape(W, -1).mean(axis=1)
+    else:
+        lane = O.xorshift_rows(2, 0, W * 64)
+        grp = O.xorshift_rows(2, W * 64, W)
+        step = lambda: O.run_wave(lib, cfg, paths, lane_seeds=lane, group_seeds=grp)
+        per = lambda out: out["opMeans"]
+    step()
+    e = per(step())
+    x2, _, _ = O.transfer_matrix_1d(sysobj.V, N, beta, L=6.0, M=900)
+    se = e.std() / math.sqrt(W)
+    assert abs(e.mean() - x2) < 4 * se, (e.mean(), x2, se)
+
+
+# ----------------------------------------------------------------------------
+# Closed forms and systems
+# ----------------------------------------------------------------------------
+def test_ho_closed_forms():
+    assert abs(O.ho_discrete_x2(256, 10.0) - 0.499950082044) < 1e-11
+    assert abs(O.ho_discrete_x2(128, 11.0) - 0.499555802927) < 1e-11
+    c = O.ho_discrete_correlator(256, 10.0)
+    assert abs(c[0] - O.ho_discrete_x2(256, 10.0)) < 1e-14
+    # large-beta decay rate = the discretised gap arccosh(1 + eps^2/2)/eps
+    eps = 10.0 / 256
+    gap = math.acosh(1 + eps * eps / 2) / eps
+    assert abs(gap - 0.99993643) < 1e-7
+
+
+def test_transfer_matrix_reproduces_ho():
+    x2, _, cor = O.transfer_matrix_1d(lambda x: 0.5 * x * x, 32, 6.0)
+    assert abs(x2 - O.ho_discrete_x2(32, 6.0)) < 1e-9
+    np.testing.assert_allclose(cor, O.ho_discrete_correlator(32, 6.0)[:17], atol=1e-9)
+
+
+def test_lm2m2_literature_values():
+    assert abs(lm2m2_kelvin(2.9695) - (-10.97)) < 1e-5
+    assert abs(lm2m2_kelvin(3.5) - (-6.284940)) < 2e-5
+    assert abs(lm2m2_kelvin(5.0) - (-0.728570)) < 2e-5
+    assert abs(lm2m2_kelvin(10.0) - (-0.010414)) < 2e-6
+    assert abs(mass_scale(4.0026) ** 2 - 1 / 12.119282) < 2e-6
+
+
+def test_lm2m2_system_strings_in_oracle():
+    """The generated potential and virial strings, compiled by gcc in FP64."""
+    sysobj = Lm2m2Cluster(3)
+    lib = O.build(sysobj, 64, 10.0, operators=(sysobj.energyOp, sysobj.meanPairDistanceOp), double=True)
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        r_atoms = rng.normal(0, 2.5, (3, 3))
+        x = np.ascontiguousarray((r_atoms * sysobj.massScale).reshape(9))
+        dists = [np.linalg.norm(r_atoms[i] - r_atoms[j]) for i, j in sysobj.pairs]
+        V = sum(lm2m2_kelvin(d) for d in dists)
+        assert abs(lib.fs_oracle_potential(x.ctypes.data) - V) < 1e-9 * max(1, abs(V))
+        acc = np.zeros(3)
+        lib.fs_oracle_operators(x.ctypes.data, acc.ctypes.data)
+        h = 1e-6
+        vir = sum(lm2m2_kelvin(d) + 0.5 * d * (lm2m2_kelvin(d + h) - lm2m2_kelvin(d - h)) / (2 * h) for d in dists)
+        assert abs(acc[0] - vir) < 1e-5 * max(1, abs(vir))
+        assert abs(acc[1] - np.mean(dists)) < 1e-9
