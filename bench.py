@@ -7,7 +7,8 @@ distance.  One step = one launch of 32 sweeps over every chain followed by
 one measurement.  value = slice-updates/s (one slice-update = one interior
 slice of a segment, all 9 coordinates, proposed by one bisection move).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage registers|memory]
+       python bench.py --copy-calibration   (1 GiB device copy, to scale rocprofv3 HBM counters)
 """
 import argparse
 import json
@@ -87,6 +88,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of 1 GiB (5 times), to scale HBM counters")
     ap.add_argument("--storage", choices=["registers", "memory"], default=None,
                     help="where the wave sampler keeps each path during a launch (default: automatic)")
     args = ap.parse_args()
@@ -97,6 +100,14 @@ def main():
     import numpy as np
     import torch
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5}), flush=True)
+        return
     system, k = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
     rng = np.random.default_rng(0)
     k.setPaths(initial_paths(system, rng))

@@ -14,7 +14,8 @@ FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3
 FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
 FS_STORAGE_AUTO, FS_STORAGE_MEMORY, FS_STORAGE_REGISTERS = 0, 1, 2
 
-EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_build_log",
+EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_path_storage",
+           "fs_build_log",
            "fs_source", "fs_run", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
            "fs_ranlux_draw")
 
@@ -61,6 +62,7 @@ def lib():
     L.fs_device_count.argtypes = [ctypes.POINTER(ctypes.c_int)]
     L.fs_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp)]
     L.fs_destroy.argtypes = [vp]
+    L.fs_path_storage.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     L.fs_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_source.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_run.argtypes = [vp, ctypes.POINTER(FsBuffers), i32, i32, i32, vp]
@@ -113,6 +115,12 @@ class Engine:
             raise FsError(rc, msg)
         self.cfg = cfg
         self.build_log = _text(L.fs_build_log, self._h)
+
+    @property
+    def path_storage(self):
+        v = ctypes.c_int(0)
+        check(lib().fs_path_storage(self._h, ctypes.byref(v)))
+        return v.value
 
     @property
     def source(self):

@@ -293,9 +293,10 @@ template <int G, typename T> FS_DEV T fs_group_sum(T v)
 // them, so a launch touches HBM once on entry and once on exit; the block
 // index is uniform, so the slot of a block is a register-indexed move.
 // Otherwise every block is loaded from and stored to the path in memory.
-#ifndef FS_RESIDENT
-#define FS_RESIDENT (FS_N >= 64 && FS_NB <= 16 && FS_NB * (FS_DOF + 1) * (FS_DOUBLE ? 2 : 1) <= 176)
-#endif
+// FS_RESIDENT (1: registers, 0: memory) is decided by the host.
+// In memory storage the potential at every slice is cached in potCache
+// ([walkers][N], refreshed at the start of each launch), so a move evaluates
+// the potential only at its proposed slices, as in register storage.
 #if FS_RESIDENT
 typedef fs_f fs_slots __attribute__((ext_vector_type(FS_NB)));
 #endif
@@ -344,6 +345,17 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
         for (int d = 0; d < FS_DOF; d++) xs[d][k] = x[d];
         vc[k] = fs_potential_v(x);
     }
+#else
+    // lane j only ever touches the cache entries of slices congruent to j mod B
+    fs_f *pot = reinterpret_cast<fs_f *>(a.potCache) + w * (long)FS_N;
+    if (active) {
+#pragma unroll 1
+        for (int k = 0; k < FS_NB; k++) {
+            fs_f x[FS_DOF];
+            fs_gather(path, k * FS_B + lane, x);
+            pot[k * FS_B + lane] = fs_potential_v(x);
+        }
+    }
 #endif
 
     for (int i = 0; i < a.operatorRuns; i++) {
@@ -381,7 +393,7 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
                 const int s = (g * FS_B + lane + (wrap ? FS_B : 0)) & (FS_N - 1);
                 fs_gather(path, s, xo);
                 fs_gather(path, ((g + 1) * FS_B + r) & (FS_N - 1), xr);
-                vOld = fs_potential_v(xo);
+                vOld = pot[s];
 #endif
                 // every lane draws the same count so buffered streams stay in step
                 fs_f u[FS_KDRAW];
@@ -389,16 +401,11 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
                 for (int k = 0; k < FS_KDRAW; k++) u[k] = rng.uniform();
 #pragma unroll
                 for (int k = 0; k < FS_KPAIR; k++) {
-                    fs_f sn = (fs_f)0.0, cs = (fs_f)0.0;
+                    // v == 0 gives a zero displacement, as kernel.c:346-349 does
                     const fs_f v = u[2 * k + 1];
-                    if (v != (fs_f)0.0) {
-                        const fs_f rho = sqrt(-2.0 * log(v));
-                        const fs_f ph = 2.0 * M_PI * u[2 * k];
-                        sn = rho * sin(ph);
-                        cs = rho * cos(ph);
-                    }
-                    z[2 * k] = sig * sn;
-                    z[2 * k + 1] = sig * cs;
+                    const fs_f rho = v != (fs_f)0.0 ? sig * sqrt(-2.0 * log(v)) : (fs_f)0.0;
+                    z[2 * k] = rho * fs_sin2pi(u[2 * k]);
+                    z[2 * k + 1] = rho * fs_cos2pi(u[2 * k]);
                 }
 #pragma unroll
                 for (int d = 0; d < FS_DOF; d++) xn[d] = xo[d];
@@ -445,6 +452,7 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
                 if (put) {
 #pragma unroll
                     for (int d = 0; d < FS_DOF; d++) path[d * FS_N + s] = xn[d];
+                    pot[s] = vNew;
                 }
 #endif
             }

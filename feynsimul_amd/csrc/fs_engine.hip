@@ -128,6 +128,19 @@ int validate(const fs_config *c)
     return FS_OK;
 }
 
+// Register storage of the wave sampler: lane j keeps N/64 slices of DOF
+// coordinates plus their potential for the whole launch.  Chosen automatically
+// only while that costs at most 64 VGPRs, which keeps several wavefronts per
+// SIMD; larger paths are streamed through the cache hierarchy instead.
+bool resident_storage(const fs_config *c)
+{
+    if (c->sampler != FS_SAMPLER_WAVE || c->N < 64 || c->N / 64 > 16) return false;
+    if (c->path_storage == FS_STORAGE_REGISTERS) return true;
+    if (c->path_storage == FS_STORAGE_MEMORY) return false;
+    const int words = c->double_precision ? 2 : 1;
+    return (c->N / 64) * (c->DOF + 1) * words <= 64;
+}
+
 std::string build_source(const fs_config *c, const char *system_src, int wpb)
 {
     std::string s;
@@ -162,8 +175,7 @@ std::string build_source(const fs_config *c, const char *system_src, int wpb)
     defi("FS_LOCAL", c->local_size > 0 ? c->local_size : 1);
     defi("FS_WPB", wpb);
     defi("FS_SAMPLER", c->sampler);
-    if (c->path_storage == FS_STORAGE_MEMORY) defi("FS_RESIDENT", 0);
-    if (c->path_storage == FS_STORAGE_REGISTERS) defi("FS_RESIDENT", 1);
+    defi("FS_RESIDENT", resident_storage(c) ? 1 : 0);
     def("eps", lit(c->beta / (double)c->N));
     s += R"(
 #ifndef M_PI
@@ -185,8 +197,16 @@ std::string build_source(const fs_config *c, const char *system_src, int wpb)
 #define half_sqrt sqrt
 #define mad(a, b, c) ((a) * (b) + (c))
 #define pown(a, n) pow((a), (FLOAT_TYPE)(n))
-#pragma clang force_cuda_host_device begin
 )";
+    // sin(2 pi x) and cos(2 pi x): the hardware sine/cosine take their argument
+    // in revolutions; used when relaxed math is allowed in FP32
+    if (c->fast_math && !c->double_precision)
+        s += "__device__ __forceinline__ float fs_sin2pi(float x) { return __builtin_amdgcn_sinf(x); }\n"
+             "__device__ __forceinline__ float fs_cos2pi(float x) { return __builtin_amdgcn_cosf(x); }\n";
+    else
+        s += "__device__ __forceinline__ FLOAT_TYPE fs_sin2pi(FLOAT_TYPE x) { return sin(2.0 * M_PI * x); }\n"
+             "__device__ __forceinline__ FLOAT_TYPE fs_cos2pi(FLOAT_TYPE x) { return cos(2.0 * M_PI * x); }\n";
+    s += "#pragma clang force_cuda_host_device begin\n";
     s += system_src ? system_src : "";
     s += "\n#pragma clang force_cuda_host_device end\n#include \"fs_sampler.h\"\n";
     return s;
@@ -321,6 +341,15 @@ int fs_destroy(fs_engine *e)
     return FS_OK;
 }
 
+int fs_path_storage(const fs_engine *e, int *storage)
+{
+    if (!e || !storage) return fail(FS_ERR_INVALID, "engine or storage is NULL");
+    *storage = e->cfg.sampler == FS_SAMPLER_WAVE
+                   ? (resident_storage(&e->cfg) ? FS_STORAGE_REGISTERS : FS_STORAGE_MEMORY)
+                   : FS_STORAGE_MEMORY;
+    return FS_OK;
+}
+
 int fs_build_log(const fs_engine *e, char *buf, size_t len, size_t *needed)
 {
     if (!e) return fail(FS_ERR_INVALID, "engine is NULL");
@@ -346,6 +375,8 @@ int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t met
     if (c.n_ops > 0 && !b->op_means) return fail(FS_ERR_INVALID, "op_means required");
     if (c.n_corr > 0 && !b->cor_means) return fail(FS_ERR_INVALID, "cor_means required");
     if (c.bins && !b->bins) return fail(FS_ERR_INVALID, "bins required");
+    if (c.sampler == FS_SAMPLER_WAVE && !resident_storage(&c) && !b->pot_cache)
+        return fail(FS_ERR_INVALID, "pot_cache required by the wave sampler with memory storage");
 
     fs_kargs a{};
     a.paths = b->paths;

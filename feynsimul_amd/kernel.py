@@ -306,6 +306,9 @@ class PIMCKernel:
             if exact:
                 self._seedsSnap = self._zeros((rows, 4), t.int32)
         self._oldPath = self._zeros((threads, (2 ** self._S - 1) * DOF), ft) if exact else None
+        self._potCache = None
+        if not exact and self._engine.path_storage == _native.FS_STORAGE_MEMORY:
+            self._potCache = self._zeros((W, N), ft)
         self._opTotals = None
         self._series = None
         if nOps:
@@ -345,6 +348,7 @@ class PIMCKernel:
         b.series = p(self._series)
         b.cor_means = p(self._correlatorValues)
         b.bins = p(self._binCounts)
+        b.pot_cache = p(self._potCache)
         return b
 
     def _thread_layout(self):
@@ -511,7 +515,8 @@ class PIMCKernel:
         """Bytes of device memory held by this kernel (kernel.py:660-684)."""
         tot = 0
         for x in (self._paths, self._accepts, self._seeds, self._seedsSnap, self._rl, self._grl, self._oldPath,
-                  self._operatorValues, self._opTotals, self._series, self._correlatorValues, self._binCounts):
+                  self._operatorValues, self._opTotals, self._series, self._correlatorValues, self._binCounts,
+                  self._potCache):
             if x is not None:
                 tot += x.numel() * x.element_size()
         return tot

@@ -61,48 +61,45 @@ def _user_code(qscale):
     B = 2.0 * math.pi / (p["xa2"] - p["xa1"])
     return r"""
 /* LM2M2 He-He potential in K; r2 is a squared pair distance in mass-scaled
- * coordinates, q = r / r_m is the reduced distance. */
+ * coordinates, q = r / r_m is the reduced distance.  Written without
+ * branches: the damping exponent max(D/q - 1, 0) gives exactly 1 for q >= D,
+ * and the add-on A_a (sin(B (q - x_a1) - pi/2) + 1) = A_a (1 - cos(2 pi w)),
+ * w = (q - x_a1) / (x_a2 - x_a1), is selected on [x_a1, x_a2]. */
 FLOAT_TYPE fs_sqr(FLOAT_TYPE a) { return a * a; }
 FLOAT_TYPE fs_lm2m2_q(FLOAT_TYPE q)
 {
-    const FLOAT_TYPE iq2 = 1.0 / (q * q);
+    const FLOAT_TYPE iq = 1.0 / q;
+    const FLOAT_TYPE iq2 = iq * iq;
     const FLOAT_TYPE disp = iq2 * iq2 * iq2 * (%(c6)r + iq2 * (%(c8)r + iq2 * %(c10)r));
-    FLOAT_TYPE damp = 1.0;
-    if (q < %(D)r) {
-        const FLOAT_TYPE t = %(D)r / q - 1.0;
-        damp = q > %(qmin)r ? exp(-t * t) : 0.0;
-    }
-    FLOAT_TYPE v = %(A)r * exp(q * (%(beta)r * q - %(alpha)r)) - damp * disp;
-    if (q >= %(xa1)r && q <= %(xa2)r)
-        v += %(Aa)r * (sin(%(B)r * (q - %(xa1)r) - M_PI_2) + 1.0);
-    return %(eps_k)r * v;
+    const FLOAT_TYPE t = fmax(%(D)r * iq - 1.0, 0.0);
+    const FLOAT_TYPE damp = q > %(qmin)r ? exp(-t * t) : 0.0;
+    const FLOAT_TYPE w = (q - %(xa1)r) * %(iwidth)r;
+    const FLOAT_TYPE va = (w >= 0.0 && w <= 1.0) ? %(Aa)r * (1.0 - fs_cos2pi(w)) : 0.0;
+    return %(eps_k)r * (%(A)r * exp(q * (%(beta)r * q - %(alpha)r)) - damp * disp + va);
 }
-/* r dV/dr in K */
-FLOAT_TYPE fs_lm2m2_rdvdr_q(FLOAT_TYPE q)
+/* V(q) + r dV/dr / 2 in K, the pair term of the virial energy estimator */
+FLOAT_TYPE fs_lm2m2_virial_q(FLOAT_TYPE q)
 {
-    const FLOAT_TYPE iq2 = 1.0 / (q * q);
+    const FLOAT_TYPE iq = 1.0 / q;
+    const FLOAT_TYPE iq2 = iq * iq;
     const FLOAT_TYPE iq6 = iq2 * iq2 * iq2;
     const FLOAT_TYPE disp = iq6 * (%(c6)r + iq2 * (%(c8)r + iq2 * %(c10)r));
     const FLOAT_TYPE qdisp = -iq6 * (6.0 * %(c6)r + iq2 * (8.0 * %(c8)r + iq2 * 10.0 * %(c10)r));
-    FLOAT_TYPE damp = 1.0, qdamp = 0.0;
-    if (q < %(D)r) {
-        const FLOAT_TYPE t = %(D)r / q - 1.0;
-        damp = q > %(qmin)r ? exp(-t * t) : 0.0;
-        qdamp = damp * 2.0 * t * (%(D)r / q);
-    }
-    FLOAT_TYPE g = %(A)r * exp(q * (%(beta)r * q - %(alpha)r)) * q * (2.0 * %(beta)r * q - %(alpha)r)
-                   - (qdamp * disp + damp * qdisp);
-    if (q >= %(xa1)r && q <= %(xa2)r)
-        g += %(Aa)r * %(B)r * q * cos(%(B)r * (q - %(xa1)r) - M_PI_2);
-    return %(eps_k)r * g;
+    const FLOAT_TYPE t = fmax(%(D)r * iq - 1.0, 0.0);
+    const FLOAT_TYPE damp = q > %(qmin)r ? exp(-t * t) : 0.0;
+    const FLOAT_TYPE qdamp = damp * 2.0 * t * (%(D)r * iq);
+    const FLOAT_TYPE rep = %(A)r * exp(q * (%(beta)r * q - %(alpha)r));
+    const FLOAT_TYPE w = (q - %(xa1)r) * %(iwidth)r;
+    const bool win = w >= 0.0 && w <= 1.0;
+    const FLOAT_TYPE va = win ? %(Aa)r * (1.0 - fs_cos2pi(w)) : 0.0;
+    const FLOAT_TYPE qva = win ? %(Aa)r * %(B)r * q * fs_sin2pi(w) : 0.0;
+    const FLOAT_TYPE v = rep - damp * disp + va;
+    const FLOAT_TYPE qdv = rep * q * (2.0 * %(beta)r * q - %(alpha)r) - (qdamp * disp + damp * qdisp) + qva;
+    return %(eps_k)r * (v + 0.5 * qdv);
 }
 FLOAT_TYPE fs_lm2m2_r2(FLOAT_TYPE r2) { return fs_lm2m2_q(sqrt(r2) * %(qscale)r); }
-FLOAT_TYPE fs_lm2m2_virial_r2(FLOAT_TYPE r2)
-{
-    const FLOAT_TYPE q = sqrt(r2) * %(qscale)r;
-    return fs_lm2m2_q(q) + 0.5 * fs_lm2m2_rdvdr_q(q);
-}
-""" % dict(p, B=B, qmin=0.0005 / p["rm"], qscale=qscale)
+FLOAT_TYPE fs_lm2m2_virial_r2(FLOAT_TYPE r2) { return fs_lm2m2_virial_q(sqrt(r2) * %(qscale)r); }
+""" % dict(p, B=B, iwidth=1.0 / (p["xa2"] - p["xa1"]), qmin=0.0005 / p["rm"], qscale=qscale)
 
 
 class Lm2m2Cluster:

@@ -46,8 +46,8 @@ enum {
 enum { FS_SAMPLER_EXACT = 0, FS_SAMPLER_WAVE = 1 };
 
 /* Where the wave sampler keeps a walker's path during a launch: AUTO picks
- * registers when N >= 64, N <= 1024 and the path plus its potential cache fit
- * in 176 VGPRs, else memory. */
+ * registers when 64 <= N <= 1024 and the path plus its potential cache take
+ * at most 64 VGPRs per lane, else memory (then pot_cache is required). */
 enum { FS_STORAGE_AUTO = 0, FS_STORAGE_MEMORY = 1, FS_STORAGE_REGISTERS = 2 };
 
 /* One configuration = one compiled module (the reference rebuilds its
@@ -89,7 +89,8 @@ typedef struct fs_config {
  *           then group rows); rl [walkers*64]; grl [walkers];
  *           op_means f64[walkers][n_ops]; op_totals f64[walkers][n_ops][2];
  *           series f32[measurements][walkers][n_ops];
- *           cor_means f64[walkers][n_corr][cor_lags][2]
+ *           cor_means f64[walkers][n_corr][cor_lags][2];
+ *           pot_cache FLOAT[walkers][N] (memory storage)
  *   bins u32[bin_res^DOF] */
 typedef struct fs_buffers {
     void *paths;
@@ -119,6 +120,8 @@ int fs_device_count(int *count);
  * module unless `compile_only` is set (then only the build log is kept). */
 int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs_engine **out);
 int fs_destroy(fs_engine *e);
+/* The path storage the module was built with (FS_STORAGE_REGISTERS or _MEMORY). */
+int fs_path_storage(const fs_engine *e, int *storage);
 /* Copies the build log (NUL-terminated, truncated to len) and returns its full length in *needed. */
 int fs_build_log(const fs_engine *e, char *buf, size_t len, size_t *needed);
 /* Copies the generated module source (for verbose dumps). */

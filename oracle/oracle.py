@@ -36,6 +36,7 @@ _libs: dict[str, ctypes.CDLL] = {}
 # OpenCL C conveniences the reference's system strings may use
 # (physical_systems/__init__.py:6-9 names native_* functions).
 _OPENCL_PRELUDE = r"""
+#include <stdbool.h>
 #include <math.h>
 #include <tgmath.h>
 #ifndef M_PI
@@ -58,6 +59,8 @@ _OPENCL_PRELUDE = r"""
 #define rsqrt(a) (1.0 / sqrt(a))
 #define mad(a, b, c) ((a) * (b) + (c))
 #define pown(a, n) pow((a), (FLOAT_TYPE)(n))
+#define fs_sin2pi(a) sin(2.0 * M_PI * (a))
+#define fs_cos2pi(a) cos(2.0 * M_PI * (a))
 """
 
 
