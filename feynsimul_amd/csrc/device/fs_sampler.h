@@ -370,7 +370,33 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
             fs_f sig = sigtab[0];
 #pragma unroll
             for (int n = 1; n <= FS_S; n++) sig = (lvl == n) ? sigtab[n] : sig;
+            // The Levy bisection of a segment, written in closed form: the
+            // midpoint recursion x_m = (x_{m-h} + x_{m+h}) / 2 + sigma_n z_m
+            // resolves to the linear interpolation of the two ends plus, for
+            // each level n, the level-n node covering position i times the tent
+            // 1 - |i - m_n| / h_n (zero at nodes of coarser levels and at the
+            // ends).  Same bridge, no per-level masking.
+            const int segL = p - ip;
+            const int laneL = (segL + r) & (FS_B - 1);
+            const int laneR = (segL + FS_SEG + r) & (FS_B - 1);  // lane r when the end is the block end
+            const fs_f frac = (fs_f)ip * ((fs_f)1.0 / (fs_f)FS_SEG);
+            int srcN[FS_S + 1];
+            fs_f wN[FS_S + 1];
+#pragma unroll
+            for (int n = 1; n <= FS_S; n++) {
+                const int h = FS_SEG >> n;
+                const int m = (ip & ~(2 * h - 1)) + h;
+                srcN[n] = (segL + m + r) & (FS_B - 1);
+                wN[n] = (fs_f)1.0 - (fs_f)abs(ip - m) * ((fs_f)1.0 / (fs_f)h);
+            }
 
+#if !FS_RESIDENT
+            // block g + 1 is loaded while block g is moved: its position 0 is the
+            // right end of block g, which no move of this step changes
+            fs_f xnext[FS_DOF];
+            int snext = (lane + (wrap ? FS_B : 0)) & (FS_N - 1);
+            fs_gather(path, snext, xnext);
+#endif
 #if FS_RESIDENT
 #pragma unroll 1
 #endif
@@ -390,9 +416,13 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
                     vOld = wrap ? v1 : v0;
                 }
 #else
-                const int s = (g * FS_B + lane + (wrap ? FS_B : 0)) & (FS_N - 1);
-                fs_gather(path, s, xo);
-                fs_gather(path, ((g + 1) * FS_B + r) & (FS_N - 1), xr);
+                const int s = snext;
+#pragma unroll
+                for (int d = 0; d < FS_DOF; d++) xo[d] = xnext[d];
+                snext = ((g + 1) * FS_B + lane + (wrap ? FS_B : 0)) & (FS_N - 1);
+                fs_gather(path, snext, xnext);
+#pragma unroll
+                for (int d = 0; d < FS_DOF; d++) xr[d] = fs_readlane(xnext[d], r);
                 vOld = pot[s];
 #endif
                 // every lane draws the same count so buffered streams stay in step
@@ -408,19 +438,13 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
                     z[2 * k + 1] = rho * fs_cos2pi(u[2 * k]);
                 }
 #pragma unroll
-                for (int d = 0; d < FS_DOF; d++) xn[d] = xo[d];
+                for (int d = 0; d < FS_DOF; d++) {
+                    const fs_f xL = fs_lane_read(xo[d], laneL);
+                    const fs_f xR = fs_lane_read(p == 0 ? xr[d] : xo[d], laneR);
+                    fs_f x = xL + (xR - xL) * frac;
 #pragma unroll
-                for (int n = 1; n <= FS_S; n++) {
-                    const int h = FS_SEG >> n;
-                    const bool gen = (ip & (2 * h - 1)) == h;
-                    const bool rightEnd = p + h == FS_B;
-#pragma unroll
-                    for (int d = 0; d < FS_DOF; d++) {
-                        const fs_f L = fs_lane_read(xn[d], (lane - h) & (FS_B - 1));
-                        fs_f R = fs_lane_read(xn[d], (lane + h) & (FS_B - 1));
-                        R = rightEnd ? xr[d] : R;
-                        if (gen) xn[d] = 0.5 * (L + R) + z[d];
-                    }
+                    for (int n = 1; n <= FS_S; n++) x += wN[n] * fs_lane_read(z[d], srcN[n]);
+                    xn[d] = x;
                 }
                 const fs_f vNew = fs_potential_v(xn);
                 const fs_f dv = (ip != 0 && active) ? vNew - vOld : (fs_f)0.0;
