@@ -30,6 +30,19 @@ BYTES_PER_UPDATE = 8.0 * DOF * SEG / (SEG - 1)
 # counted once): 26 per coordinate + 2 potential evaluations of 3 LM2M2 pairs
 VALU_PER_UPDATE = 26 * DOF + 2 * (39 * 3 + 1)
 HBM_PEAK_GBS = 8000.0
+# measured HBM bytes per launch of this workload (rocprofv3 FETCH_SIZE/WRITE_SIZE
+# scaled by a 1 GiB copy), kept with the profile summary it comes from
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fs_wave_sweep_lm2m2_he3_n1024.md")
+
+
+def measured_traffic():
+    import re
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read(), re.M)
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s, one FP32 op per lane per clock
 
 
@@ -147,7 +160,8 @@ def main():
                    "path_storage": args.storage or "auto",
                    "chains": W, "slices": N, "bisection_level": S},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic() if args.storage in (None, "memory") else None,
                      "kernel": "fs_wave_sweep", "avg_launch_ms": kern_s * 1e3,
                      "bytes_per_update": BYTES_PER_UPDATE, "updates_per_launch": updates},
         "valu": {"ops_per_update": VALU_PER_UPDATE, "achieved": VALU_PER_UPDATE * updates / kern_s,
