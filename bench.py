@@ -3,8 +3,8 @@
 Workload (one GPU): 3 He atoms in 3D with the LM2M2 pair potential, mass
 4.0026 u, N = 1024 slices, beta = 10 1/K, 16384 chains, bisection level 6,
 FP32 coordinates, xorshift128 streams, operators virial energy and mean pair
-distance.  One step = one launch of 32 sweeps over every chain followed by
-one measurement.  value = slice-updates/s (one slice-update = one interior
+distance, and a 256-bin pair-distance histogram.  One step = one launch of
+32 sweeps over every chain followed by one measurement.  value = slice-updates/s (one slice-update = one interior
 slice of a segment, all 9 coordinates, proposed by one bisection move).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage registers|memory]
@@ -20,6 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 N, S, W, BETA, DOF = 1024, 6, 16384, 10.0, 9
+HIST_MAX_A = 40.0  # pair-distance histogram: 256 bins on [0, 40) angstrom
 SWEEPS_PER_STEP = 32
 SEG = 1 << S
 UPDATES_PER_SWEEP = W * (N // SEG) * (SEG - 1)
@@ -46,7 +47,7 @@ def measured_traffic():
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s, one FP32 op per lane per clock
 
 
-def make_kernel(steps_per_launch, seed, storage=None):
+def make_kernel(steps_per_launch, seed, storage=None, operator_runs=1):
     from feynsimul_amd.kernel_args import KernelArgs
     from feynsimul_amd.kernel import PIMCKernel
     from feynsimul_amd.physical_systems import Lm2m2Cluster
@@ -55,9 +56,12 @@ def make_kernel(steps_per_launch, seed, storage=None):
                     enableCorrelator=False, enableBisection=True, enablePathShift=False,
                     enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
                     enableGlobalOldPath=False, enableBins=False, enableDouble=False,
-                    metroStepsPerOperatorRun=steps_per_launch, operatorRuns=1, nbrOfWalkersPerWorkGroup=1,
+                    metroStepsPerOperatorRun=steps_per_launch, operatorRuns=operator_runs,
+                    nbrOfWalkersPerWorkGroup=1,
                     operators=(system.energyOp, system.meanPairDistanceOp), seed=seed, sampler="wave",
-                    pathStorage=storage)
+                    pathStorage=storage,
+                    histogramObservables=tuple(system.pairDistance(i, j) for i, j in system.pairs),
+                    histogramRange=(0.0, HIST_MAX_A), histogramBins=256)
     return system, PIMCKernel(ka)
 
 
@@ -65,6 +69,32 @@ def initial_paths(system, rng, walkers=W, sigma=3.0):
     import numpy as np
     pos = system.initialPositions(walkers, rng, side=3.5, sigma=sigma)
     return np.repeat(pos[:, :, None], N, axis=2).reshape(walkers, DOF * N)
+
+
+def ess_phase(paths, storage, burn_launches, meas_launches, runs_per_launch=16):
+    """ESS/s of the virial energy E0: series of per-chain slice means, one every
+    32 sweeps, after burn-in; tau_int by Sokal windowing over all chains."""
+    import torch
+    from feynsimul_amd.stats import effective_sample_size
+    _, k = make_kernel(SWEEPS_PER_STEP, seed=777, storage=storage, operator_runs=runs_per_launch)
+    k.setPaths(paths)
+    for _ in range(burn_launches):
+        k.run(wait=False)
+    k.setSeriesCapacity(meas_launches * runs_per_launch)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(meas_launches):
+        k.run(wait=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    r = effective_sample_size(k.getSeries()[:, :, 0])
+    sweeps = meas_launches * runs_per_launch * SWEEPS_PER_STEP
+    return {"metric": "ESS/s of E0 (virial energy), plain bisection sampler", "value": r["ess"] / dt,
+            "unit": "effective samples/s", "E0_K": r["mean"], "sigma_K": r["se"],
+            "tau_int_measurements": r["tau_int"], "tau_int_sweeps": r["tau_int"] * SWEEPS_PER_STEP,
+            "ess": r["ess"], "seconds": dt, "sweeps": sweeps,
+            "burn_in_sweeps": burn_launches * runs_per_launch * SWEEPS_PER_STEP,
+            "seconds_to_sigma_1e-3_K": dt * (r["se"] / 1e-3) ** 2}
 
 
 def cpu_baseline(target_s=12.0):
@@ -101,6 +131,9 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ess", action="store_true", help="skip the ESS/s phase")
+    ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the ESS phase")
+    ap.add_argument("--ess-launches", type=int, default=64, help="ESS launches of 16 x 32 sweeps")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (5 times), to scale HBM counters")
     ap.add_argument("--storage", choices=["registers", "memory"], default=None,
@@ -156,7 +189,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic: seeded Gaussian blobs (sigma 3 A) around a random triangle of side 3.5 A",
         "config": {"workload": "LM2M2 He3, 3D Cartesian, N=1024, beta=10/K, 16384 chains, S=6, xorshift128, "
-                               "32 sweeps + 1 measurement per step", "sampler": "wave",
+                               "32 sweeps + 1 measurement (energy, mean pair distance, 256-bin pair-distance "
+                               "histogram) per step", "sampler": "wave",
                    "path_storage": args.storage or "auto",
                    "chains": W, "slices": N, "bisection_level": S},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -169,6 +203,8 @@ def main():
         "acceptance": k.getAcceptanceRate(),
         "energy_K": E,
     }
+    if not args.no_ess:
+        line["ess"] = ess_phase(k.getPaths(), args.storage, args.ess_burn, args.ess_launches)
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -30,13 +30,14 @@ class FsConfig(ctypes.Structure):
         ("n_corr", ctypes.c_int32), ("cor_lags", ctypes.c_int32), ("bins", ctypes.c_int32),
         ("bin_res", ctypes.c_int32), ("waves_per_block", ctypes.c_int32), ("beta", ctypes.c_double),
         ("x_min", ctypes.c_double), ("x_max", ctypes.c_double), ("op_norm", ctypes.c_double),
-        ("path_storage", ctypes.c_int32)]
+        ("path_storage", ctypes.c_int32), ("n_hist_obs", ctypes.c_int32), ("hist_bins", ctypes.c_int32),
+        ("hist_lo", ctypes.c_double), ("hist_hi", ctypes.c_double)]
 
 
 class FsBuffers(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "paths", "accepts", "seeds", "seeds_snap", "rl", "grl", "old_paths", "op_means", "op_totals", "series",
-        "cor_means", "bins", "pot_cache")]
+        "cor_means", "bins", "pot_cache", "obs_hist")]
 
 
 class FsError(RuntimeError):

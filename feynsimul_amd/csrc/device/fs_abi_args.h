@@ -18,10 +18,11 @@ struct fs_kargs {
     float *series;        // wave, optional: per-measurement slice means [measurement][walkers][nOps]
     void *corMeans;       // exact: FLOAT_TYPE [walkers][nCorr][N/2]; wave: double [walkers][nCorr][lags][2]
     unsigned *bins;       // histogram counts
-    void *potCache;       // wave, optional: FLOAT_TYPE [walkers][N] potential at every slice
+    void *potCache;       // wave, memory storage: FLOAT_TYPE [walkers][N] potential at every slice
+    unsigned *obsHist;    // wave: histogram of the scalar observables [hist_bins]
     long long nWalkers;
     int operatorRuns;
     int metroSteps;
     int seriesOffset;     // index of the first measurement of this launch in `series`
-    int flags;            // bit 0: potCache valid on entry
+    int flags;            // reserved (0)
 };

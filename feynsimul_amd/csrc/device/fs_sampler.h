@@ -10,7 +10,9 @@
 // Configuration macros: FS_N, FS_S, FS_DOF, FS_NOPS, FS_NCORR, FS_CORLAGS,
 // FLOAT_TYPE, FS_DOUBLE, FS_RANLUX, FS_NSKIP, FS_ISCALE, FS_BINS, FS_BINRES,
 // FS_XMIN, FS_XMAX, FS_INVBIN, FS_OPNORM, FS_PARALLEL, FS_W, FS_LOCAL,
-// FS_WPB (waves per block of the wave sampler), FS_SAMPLER (0 exact, 1 wave).
+// FS_WPB (waves per block of the wave sampler), FS_SAMPLER (0 exact, 1 wave),
+// FS_RESIDENT (wave path storage), FS_NHOBS, FS_HBINS, FS_HLO, FS_HINV
+// (observable histogram of the wave sampler).
 //
 // Two samplers share the move:
 //
@@ -329,9 +331,17 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
 #pragma unroll
     for (int k = 0; k < FS_NOPS1; k++) opS[k] = opC[k] = 0.0;
 
-#if FS_NCORR > 0
+    // dynamic LDS: per wavefront, correlator staging then a private histogram
     extern __shared__ char fs_smem[];
+#if FS_NCORR > 0
     fs_f *cbuf = reinterpret_cast<fs_f *>(fs_smem) + (threadIdx.x >> 6) * (FS_N * FS_NCORR);
+#endif
+#if FS_NHOBS > 0
+    unsigned *hbuf = reinterpret_cast<unsigned *>(fs_smem + FS_WPB * (FS_N * FS_NCORR * (int)sizeof(fs_f))) +
+                     (threadIdx.x >> 6) * FS_HBINS;
+    for (int b = lane; b < FS_HBINS; b += 64) hbuf[b] = 0u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 #endif
 
 #if FS_RESIDENT
@@ -483,7 +493,7 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
         }
 
         // ---- measurement over all N slices (lane j: slices j, j + B, ...) ----
-#if FS_NOPS > 0 || FS_BINS || FS_NCORR > 0
+#if FS_NOPS > 0 || FS_BINS || FS_NCORR > 0 || FS_NHOBS > 0
         fs_f part[FS_NOPS1];
 #pragma unroll
         for (int k = 0; k < FS_NOPS1; k++) part[k] = (fs_f)0.0;
@@ -503,6 +513,17 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
 #endif
 #if FS_BINS
                 fs_histogram(x, a.bins);
+#endif
+#if FS_NHOBS > 0
+                {
+                    fs_f ob[FS_NHOBS];
+                    fs_hist_obs_v(x, ob);
+#pragma unroll
+                    for (int o = 0; o < FS_NHOBS; o++) {
+                        const fs_f v = (ob[o] - (fs_f)FS_HLO) * (fs_f)FS_HINV;
+                        if (v >= (fs_f)0.0 && v < (fs_f)FS_HBINS) atomicAdd(hbuf + (int)v, 1u);
+                    }
+                }
 #endif
 #if FS_NCORR > 0
                 fs_f prod[FS_NCORR1];
@@ -555,6 +576,15 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs
     }
 
     // ---- launch outputs ----
+#if FS_NHOBS > 0
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int b = lane; b < FS_HBINS; b += 64) {
+        const unsigned c = hbuf[b];
+        if (c) atomicAdd(a.obsHist + b, c);
+    }
+#endif
 #if FS_RESIDENT
 #pragma unroll 1
     for (int k = 0; k < FS_NB; k++) {

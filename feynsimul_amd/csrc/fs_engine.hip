@@ -116,6 +116,9 @@ int validate(const fs_config *c)
         if ((1 << c->S) > 64) return fail(FS_ERR_INVALID, "wave sampler needs 2^S <= 64");
         if (c->n_corr > 0 && (c->cor_lags < 1 || c->cor_lags > c->N / 2))
             return fail(FS_ERR_INVALID, "cor_lags must be in [1, N/2]");
+        if (c->n_hist_obs < 0 || (c->n_hist_obs > 0 && (c->hist_bins < 1 || c->hist_bins > 8192 ||
+                                                      !(c->hist_hi > c->hist_lo))))
+            return fail(FS_ERR_INVALID, "observable histogram needs 1 <= hist_bins <= 8192 and hist_hi > hist_lo");
         if (c->path_storage < FS_STORAGE_AUTO || c->path_storage > FS_STORAGE_REGISTERS)
             return fail(FS_ERR_INVALID, "unknown path_storage %d", c->path_storage);
         if (c->path_storage == FS_STORAGE_REGISTERS && (c->N < 64 || c->N / 64 > 16))
@@ -175,6 +178,10 @@ std::string build_source(const fs_config *c, const char *system_src, int wpb)
     defi("FS_LOCAL", c->local_size > 0 ? c->local_size : 1);
     defi("FS_WPB", wpb);
     defi("FS_SAMPLER", c->sampler);
+    defi("FS_NHOBS", c->sampler == FS_SAMPLER_WAVE ? c->n_hist_obs : 0);
+    defi("FS_HBINS", c->hist_bins > 0 ? c->hist_bins : 1);
+    def("FS_HLO", lit(c->hist_lo));
+    def("FS_HINV", lit(c->n_hist_obs > 0 ? (double)c->hist_bins / (c->hist_hi - c->hist_lo) : 1.0));
     defi("FS_RESIDENT", resident_storage(c) ? 1 : 0);
     def("eps", lit(c->beta / (double)c->N));
     s += R"(
@@ -256,12 +263,15 @@ int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs
     e->cfg = *cfg;
     const size_t fsize = cfg->double_precision ? 8 : 4;
     int wpb = cfg->waves_per_block > 0 ? cfg->waves_per_block : 4;
-    if (cfg->sampler == FS_SAMPLER_WAVE && cfg->n_corr > 0) {
-        const size_t per = (size_t)cfg->N * cfg->n_corr * fsize;
+    if (cfg->sampler == FS_SAMPLER_WAVE) {
+        // per wavefront: correlator staging, then a private observable histogram
+        const size_t cor = cfg->n_corr > 0 ? (size_t)cfg->N * cfg->n_corr * fsize : 0;
+        const size_t hist = cfg->n_hist_obs > 0 ? (size_t)cfg->hist_bins * 4 : 0;
+        const size_t per = cor + hist;
         while (wpb > 1 && per * wpb > 65536) wpb--;
         if (per > 65536) {
             delete e;
-            return fail(FS_ERR_INVALID, "correlator staging needs %zu bytes of LDS per wavefront", per);
+            return fail(FS_ERR_INVALID, "measurement staging needs %zu bytes of LDS per wavefront", per);
         }
         e->wave_smem = per * wpb;
     }
@@ -375,6 +385,8 @@ int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t met
     if (c.n_ops > 0 && !b->op_means) return fail(FS_ERR_INVALID, "op_means required");
     if (c.n_corr > 0 && !b->cor_means) return fail(FS_ERR_INVALID, "cor_means required");
     if (c.bins && !b->bins) return fail(FS_ERR_INVALID, "bins required");
+    if (c.sampler == FS_SAMPLER_WAVE && c.n_hist_obs > 0 && !b->obs_hist)
+        return fail(FS_ERR_INVALID, "obs_hist required");
     if (c.sampler == FS_SAMPLER_WAVE && !resident_storage(&c) && !b->pot_cache)
         return fail(FS_ERR_INVALID, "pot_cache required by the wave sampler with memory storage");
 
@@ -392,6 +404,7 @@ int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t met
     a.corMeans = b->cor_means;
     a.bins = static_cast<unsigned *>(b->bins);
     a.potCache = b->pot_cache;
+    a.obsHist = static_cast<unsigned *>(b->obs_hist);
     a.nWalkers = c.n_walkers;
     a.operatorRuns = operator_runs;
     a.metroSteps = metro_steps;

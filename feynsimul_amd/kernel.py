@@ -212,6 +212,15 @@ class PIMCKernel:
             raise Exception("work-group of %d items exceeds the device limit of 1024"
                             % (T * self._nbrOfWalkersPerWorkGroup))
         self._sampler = sampler
+        self._histObs = tuple(getattr(ka, "histogramObservables", None) or ())
+        if self._histObs:
+            if sampler != "wave":
+                raise Exception("histogramObservables need the wave sampler")
+            rng_ = getattr(ka, "histogramRange", None)
+            if rng_ is None or len(rng_) != 2 or not rng_[1] > rng_[0]:
+                raise Exception("histogramRange must be (lo, hi) with hi > lo")
+            self._histRange = (float(rng_[0]), float(rng_[1]))
+            self._histBins = int(getattr(ka, "histogramBins", 256))
         lags = getattr(ka, "correlatorLags", None)
         self._corLags = int(lags) if lags else self._N // 2
         if self._enableCorrelator and not (1 <= self._corLags <= self._N // 2):
@@ -257,9 +266,13 @@ class PIMCKernel:
         if storage not in storages:
             raise ValueError("pathStorage must be 'registers', 'memory' or None")
         cfg.path_storage = storages[storage]
+        if self._histObs:
+            cfg.n_hist_obs = len(self._histObs)
+            cfg.hist_bins = self._histBins
+            cfg.hist_lo, cfg.hist_hi = self._histRange
         self._cfg = cfg
         src = system_section(self._system, self._operators if cfg.n_ops else (),
-                             self._correlators if cfg.n_corr else ())
+                             self._correlators if cfg.n_corr else (), self._histObs)
         self._engine = _native.Engine(cfg, src)
         if verbose:
             with open("cleaned.hip", "w") as f:
@@ -330,6 +343,8 @@ class PIMCKernel:
             self._binCounts = self._zeros((self._binResolutionPerDOF ** DOF,), t.int32)
         else:
             self._binCounts = None
+        self._obsHist = self._zeros((self._histBins,), t.int32) if self._histObs else None
+        self._obsRuns = 0
 
     def _buffers(self):
         b = _native.FsBuffers()
@@ -349,6 +364,7 @@ class PIMCKernel:
         b.cor_means = p(self._correlatorValues)
         b.bins = p(self._binCounts)
         b.pot_cache = p(self._potCache)
+        b.obs_hist = p(self._obsHist)
         return b
 
     def _thread_layout(self):
@@ -381,6 +397,7 @@ class PIMCKernel:
         self._ranRuns += 1
         self._corRuns += opRuns
         self._binRuns += opRuns
+        self._obsRuns += opRuns
         if self._series is not None:
             self._seriesCount += opRuns
         self._lastRunLength = (opRuns, M)
@@ -497,6 +514,24 @@ class PIMCKernel:
         if self._binCounts is not None:
             self._binCounts.zero_()
         self._binRuns = 0
+        if self._obsHist is not None:
+            self._obsHist.zero_()
+        self._obsRuns = 0
+
+    def getObservableHistogram(self):
+        """(bin centres, density, counts) of the histogram of ``histogramObservables``.
+
+        The density integrates to the fraction of observable values inside the
+        range: counts / (bin width x walkers x slices x measurements x observables).
+        """
+        if self._obsHist is None:
+            raise Exception("no histogramObservables were configured")
+        counts = self._obsHist.cpu().numpy().view(np.uint32).astype(np.float64)
+        lo, hi = self._histRange
+        width = (hi - lo) / self._histBins
+        centres = lo + width * (np.arange(self._histBins) + 0.5)
+        tot = self._nbrOfWalkers * self._N * max(self._obsRuns, 1) * len(self._histObs)
+        return centres, counts / (width * tot), counts
 
     def getRunTime(self):
         """Device time of the last run in seconds (kernel.py:638-646)."""
@@ -516,7 +551,7 @@ class PIMCKernel:
         tot = 0
         for x in (self._paths, self._accepts, self._seeds, self._seedsSnap, self._rl, self._grl, self._oldPath,
                   self._operatorValues, self._opTotals, self._series, self._correlatorValues, self._binCounts,
-                  self._potCache):
+                  self._potCache, self._obsHist):
             if x is not None:
                 tot += x.numel() * x.element_size()
         return tot

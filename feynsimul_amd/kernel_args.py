@@ -24,6 +24,11 @@ arguments at the end, all optional:
     where the wave sampler keeps each walker's path during a launch:
     ``"registers"``, ``"memory"`` or ``None`` (automatic: registers when the
     path fits).
+``histogramObservables``, ``histogramRange``, ``histogramBins``
+    wave sampler: a tuple of scalar expressions in ``x1..xDOF`` (pair
+    distances, say) whose values at every slice are binned, at every
+    measurement, into one histogram of ``histogramBins`` bins over
+    ``histogramRange = (lo, hi)`` (``getObservableHistogram``).
 """
 
 
@@ -63,7 +68,10 @@ class KernelArgs:
                  correlatorLags=None,
                  fastMath=True,
                  wavesPerBlock=None,
-                 pathStorage=None):
+                 pathStorage=None,
+                 histogramObservables=None,
+                 histogramRange=None,
+                 histogramBins=256):
         self.system = system
         self.nbrOfWalkers = nbrOfWalkers
         self.N = N
@@ -98,3 +106,6 @@ class KernelArgs:
         self.fastMath = fastMath
         self.wavesPerBlock = wavesPerBlock
         self.pathStorage = pathStorage
+        self.histogramObservables = histogramObservables
+        self.histogramRange = histogramRange
+        self.histogramBins = histogramBins

@@ -77,6 +77,9 @@ typedef struct fs_config {
     double x_min, x_max;
     double op_norm;             /* exact sampler: 1/(operatorRuns N walkers/threads) */
     int32_t path_storage;       /* wave sampler: FS_STORAGE_* */
+    int32_t n_hist_obs;         /* wave sampler: scalar observables binned into one histogram */
+    int32_t hist_bins;          /* bins of that histogram */
+    double hist_lo, hist_hi;    /* its range; values outside are not counted */
 } fs_config;
 
 /* Device buffers of one launch; unused entries may be NULL.  Layouts:
@@ -106,6 +109,7 @@ typedef struct fs_buffers {
     void *cor_means;
     void *bins;
     void *pot_cache;
+    void *obs_hist;             /* u32[hist_bins] (wave sampler, n_hist_obs > 0) */
 } fs_buffers;
 
 typedef struct fs_engine fs_engine;

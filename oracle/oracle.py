@@ -64,7 +64,7 @@ _OPENCL_PRELUDE = r"""
 """
 
 
-def system_header(system, N: int, beta: float, operators=(), correlators=(), double=False) -> str:
+def system_header(system, N: int, beta: float, operators=(), correlators=(), double=False, observables=()) -> str:
     """Generated C header for one (system, N, beta, operators) configuration."""
     dof = int(system.DOF)
     args = ", ".join("FLOAT_TYPE x%d" % (i + 1) for i in range(dof))
@@ -76,6 +76,7 @@ def system_header(system, N: int, beta: float, operators=(), correlators=(), dou
     lines.append("#define FS_DOF %d" % dof)
     lines.append("#define FS_NOPS %d" % max(1, len(operators)))
     lines.append("#define FS_NCORR %d" % max(1, len(correlators)))
+    lines.append("#define FS_NHOBS %d" % max(1, len(observables)))
     lines.append(_OPENCL_PRELUDE)
     lines.append("#define eps %s" % ("%1.17e" % (beta / float(N))))
     lines.append(getattr(system, "userCode", "") or "")
@@ -84,6 +85,9 @@ def system_header(system, N: int, beta: float, operators=(), correlators=(), dou
     lines.append("static inline void fs_operator(%s, FLOAT_TYPE *opAccum)\n{\n%s}" % (args, body))
     body = "".join("    corProd[%d] *= %s;\n" % (i, cc) for i, cc in enumerate(correlators))
     lines.append("static inline void fs_correlator(%s, FLOAT_TYPE *corProd)\n{\n%s}" % (args, body))
+    body = "".join("    obs[%d] = %s;\n" % (i, o) for i, o in enumerate(observables))
+    lines.append("static inline void fs_hist_obs(%s, FLOAT_TYPE *obs)\n{\n%s}" % (args, body))
+    lines.append("static inline void fs_hist_obs_v(const FLOAT_TYPE *x, FLOAT_TYPE *o) { fs_hist_obs(%s, o); }" % call)
     lines.append("static inline FLOAT_TYPE fs_potential_v(const FLOAT_TYPE *x) { return potential(%s); }" % call)
     lines.append("static inline void fs_operator_v(const FLOAT_TYPE *x, FLOAT_TYPE *a) { fs_operator(%s, a); }" % call)
     lines.append("static inline void fs_correlator_v(const FLOAT_TYPE *x, FLOAT_TYPE *p) { fs_correlator(%s, p); }"
@@ -92,9 +96,9 @@ def system_header(system, N: int, beta: float, operators=(), correlators=(), dou
 
 
 def build(system, N: int, beta: float, operators=(), correlators=(), double=False,
-          fast_math=False, native=False) -> ctypes.CDLL:
+          fast_math=False, native=False, observables=()) -> ctypes.CDLL:
     """Compile (or reuse) the oracle for one configuration and load it."""
-    hdr = system_header(system, N, beta, operators, correlators, double)
+    hdr = system_header(system, N, beta, operators, correlators, double, observables)
     with open(SRC, "rb") as f:
         src_hash = hashlib.sha1(f.read()).hexdigest()
     flags = ["-O2", "-fPIC", "-shared", "-fopenmp", "-fgnu89-inline", "-std=gnu11",
@@ -133,8 +137,9 @@ class OracleCfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "N", "S", "DOF", "nWalkers", "walkersPerGroup", "parallelizePath", "ranlux", "nskip",
         "operatorRuns", "metroSteps", "enableOperator", "nOps", "enableCorrelator", "nCorr",
-        "corLags", "enableBins", "binRes", "localSize", "nThreads")] + [
-        (n, ctypes.c_double) for n in ("epsilon", "opNorm", "ranluxIntScale", "xMin", "xMax", "invBinSize")]
+        "corLags", "enableBins", "binRes", "localSize", "nThreads", "nHistObs", "histBins")] + [
+        (n, ctypes.c_double) for n in ("epsilon", "opNorm", "ranluxIntScale", "xMin", "xMax", "invBinSize",
+                                       "histLo", "histInv")]
 
 
 RANLUX_DTYPE = np.dtype([("s", "<f4", (24,)), ("carry", "<f4"), ("dummy", "<f4"),
@@ -146,7 +151,7 @@ def _declare(lib):
     vp = ctypes.c_void_p
     lib.fs_oracle_run_reference.argtypes = [ctypes.POINTER(OracleCfg)] + [vp] * 8 + [ctypes.c_int]
     lib.fs_oracle_run_reference.restype = ctypes.c_int
-    lib.fs_oracle_run_wave.argtypes = [ctypes.POINTER(OracleCfg)] + [vp] * 8 + [ctypes.c_int]
+    lib.fs_oracle_run_wave.argtypes = [ctypes.POINTER(OracleCfg)] + [vp] * 9 + [ctypes.c_int]
     lib.fs_oracle_run_wave.restype = ctypes.c_int
     lib.fs_oracle_xs_stream.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
     lib.fs_oracle_ranlux_init.argtypes = [vp, ctypes.c_int, vp]
@@ -272,7 +277,7 @@ def nskip_of_lux(lux: int) -> int:
 def make_cfg(N, S, DOF, nWalkers, beta, operatorRuns, metroSteps, *, walkersPerGroup=1,
              parallelizePath=True, ranlux=False, nskip=0, ranluxIntMax=2 ** 31 - 1, nOps=0,
              nCorr=0, corLags=None, bins=False, binRes=1, xMin=0.0, xMax=1.0, localSize=None,
-             nThreads=None) -> OracleCfg:
+             nThreads=None, nHistObs=0, histBins=1, histRange=(0.0, 1.0)) -> OracleCfg:
     c = OracleCfg()
     c.N, c.S, c.DOF = N, S, DOF
     c.nWalkers, c.walkersPerGroup = nWalkers, walkersPerGroup
@@ -291,6 +296,9 @@ def make_cfg(N, S, DOF, nWalkers, beta, operatorRuns, metroSteps, *, walkersPerG
     c.ranluxIntScale = float(np.float32(ranluxIntMax + 1))
     c.xMin, c.xMax = xMin, xMax
     c.invBinSize = float(binRes) / float(xMax - xMin)
+    c.nHistObs, c.histBins = nHistObs, histBins
+    c.histLo = float(histRange[0])
+    c.histInv = float(histBins) / float(histRange[1] - histRange[0])
     return c
 
 
@@ -318,12 +326,13 @@ def run_wave(lib, cfg: OracleCfg, paths, lane_seeds=None, group_seeds=None, lane
     opm = np.zeros(cfg.nWalkers * max(cfg.nOps, 1), dtype=np.float64)
     if cor_bins is None:
         cor_bins = np.zeros(cfg.nWalkers * max(cfg.nCorr, 1) * cfg.corLags, dtype=np.float64)
+    hist = np.zeros(max(cfg.histBins, 1), dtype=np.uint32)
     rc = lib.fs_oracle_run_wave(ctypes.byref(cfg), _ptr(paths), _ptr(accepts), _ptr(lane_seeds),
                                 _ptr(group_seeds), _ptr(lane_rl), _ptr(group_rl), _ptr(opm), _ptr(cor_bins),
-                                nthreads)
+                                _ptr(hist), nthreads)
     if rc != 0:
         raise RuntimeError("oracle wave run failed (%d)" % rc)
-    return dict(accepts=accepts, opMeans=opm, corBins=cor_bins)
+    return dict(accepts=accepts, opMeans=opm, corBins=cor_bins, obsHist=hist)
 
 
 # --------------------------------------------------------------------------

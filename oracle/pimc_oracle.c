@@ -288,10 +288,13 @@ typedef struct {
     int32_t enableBins, binRes;
     int32_t localSize;       /* work-group size of the one-walker-per-item mode */
     int32_t nThreads;        /* work-items in the launch */
+    int32_t nHistObs;        /* wave ordering: observables binned into one histogram */
+    int32_t histBins;
     double epsilon;          /* beta / N */
     double opNorm;
     double ranluxIntScale;   /* float(ranluxIntMax + 1) */
     double xMin, xMax, invBinSize;
+    double histLo, histInv;
 } fs_oracle_cfg;
 
 typedef struct {
@@ -587,7 +590,7 @@ static inline int lvl_of(int i, int S)
 
 FS_EXPORT int fs_oracle_run_wave(const fs_oracle_cfg *c, FLOAT_TYPE *paths, uint32_t *accepts,
                                  uint32_t *laneSeeds, uint32_t *groupSeeds, fs_ranlux *laneRl,
-                                 fs_ranlux *groupRl, double *opMeans, double *corBins,
+                                 fs_ranlux *groupRl, double *opMeans, double *corBins, uint32_t *obsHist,
                                  int nthreads)
 {
     const int N = c->N, S = c->S, SEG = 1 << S;
@@ -730,6 +733,21 @@ FS_EXPORT int fs_oracle_run_wave(const fs_oracle_cfg *c, FLOAT_TYPE *paths, uint
                     double tk = opS[q] + yk;
                     opC[q] = (tk - opS[q]) - yk;
                     opS[q] = tk;
+                }
+            }
+            if (c->nHistObs > 0) {
+                /* histogram of scalar observables at every slice */
+                FLOAT_TYPE x[FS_DOF], ob[FS_NHOBS];
+                for (int t = 0; t < N; t++) {
+                    gather(path, N, t, x);
+                    fs_hist_obs_v(x, ob);
+                    for (int o = 0; o < c->nHistObs; o++) {
+                        FLOAT_TYPE v = (ob[o] - (FLOAT_TYPE)c->histLo) * (FLOAT_TYPE)c->histInv;
+                        if (v >= 0.0 && v < (FLOAT_TYPE)c->histBins) {
+#pragma omp atomic
+                            obsHist[(int)v] += 1u;
+                        }
+                    }
                 }
             }
             if (c->enableCorrelator) {

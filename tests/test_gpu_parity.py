@@ -201,3 +201,30 @@ def test_system_section_matches_oracle(gpu):
         acc = np.zeros(2)
         lib.fs_oracle_operators(xo[i].ctypes.data, acc.ctypes.data)
         assert abs(acc[0] - ops[i, 0]) <= 1e-9 * max(1.0, abs(acc[0]))
+
+
+@pytest.mark.parametrize("storage", ["memory", "registers"])
+def test_wave_observable_histogram_matches_oracle(gpu, storage):
+    """256-bin pair-distance histogram of the He3 cluster (the observables of every slice at every
+    measurement) against the oracle's restatement of the same launch."""
+    _case("wave pair-distance histogram " + storage)
+    system = Lm2m2Cluster(3)
+    W, N, S, beta = 16, 256, 6, 10.0
+    obs = tuple(system.pairDistance(i, j) for i, j in system.pairs)
+    rng = np.random.default_rng(9)
+    p0 = _initial_paths(system, W, N, rng)
+    ka = _ka(system, W, N, S, beta, 3, 2, True, False, False, "wave", storage=storage)
+    ka.histogramObservables, ka.histogramRange, ka.histogramBins = obs, (0.0, 12.0), 256
+    k = PIMCKernel(ka)
+    k.setPaths(p0)
+    k.run()
+    lib = O.build(system, N, beta, operators=(system.energyOp,), observables=obs)
+    cfg = O.make_cfg(N, S, system.DOF, W, beta, 3, 2, nOps=1, nHistObs=3, histBins=256, histRange=(0.0, 12.0))
+    paths = np.ascontiguousarray(p0.astype(np.float32))
+    ref = O.run_wave(lib, cfg, paths, lane_seeds=O.xorshift_rows(11, 0, W * 64),
+                     group_seeds=O.xorshift_rows(11, W * 64, W))
+    _, dens, counts = k.getObservableHistogram()
+    assert counts.sum() == ref["obsHist"].sum() or abs(counts.sum() - ref["obsHist"].sum()) <= 3 * N
+    # walkers whose Metropolis decisions flipped move a few slices; allow that many counts to move
+    assert np.abs(counts - ref["obsHist"]).sum() <= 0.02 * counts.sum() + 6
+    assert counts.sum() > 0.5 * W * N * 3 * 3
