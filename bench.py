@@ -97,6 +97,50 @@ def ess_phase(paths, storage, burn_launches, meas_launches, runs_per_launch=16):
             "seconds_to_sigma_1e-3_K": dt * (r["se"] / 1e-3) ** 2}
 
 
+def ragged_bench(steps, warmup):
+    """Batched ragged LM2M2 systems in one persistent launch (32 sweeps per launch), autotuned."""
+    import numpy as np
+    import torch
+    from feynsimul_amd.kernel_args import KernelArgs
+    from feynsimul_amd.physical_systems import Lm2m2Cluster
+    from feynsimul_amd.batch import BatchedPIMC
+    from feynsimul_amd.autotune import autotune_batch
+    rng = np.random.default_rng(5)
+    kas, starts = [], []
+    for n in (3, 4, 6, 8):
+        system = Lm2m2Cluster(n)
+        for NN in (256, 512, 1024, 2048):
+            kas.append(KernelArgs(system=system, nbrOfWalkers=2048, N=NN, beta=BETA, S=6, enableOperator=True,
+                                  enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                                  enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                                  enableGlobalOldPath=False, enableBins=False, metroStepsPerOperatorRun=SWEEPS_PER_STEP,
+                                  operatorRuns=1, nbrOfWalkersPerWorkGroup=1, operators=(system.energyOp,),
+                                  seed=n * 4096 + NN, sampler="wave"))
+            pos = system.initialPositions(2048, rng, side=3.5, sigma=3.0)
+            starts.append(np.repeat(pos[:, :, None], NN, axis=2).reshape(2048, system.DOF * NN))
+    best, trials = autotune_batch(kas, initialPaths=starts, use_cache=False)
+    batch = BatchedPIMC(kas, wavesPerBlock=best["wavesPerBlock"], order=best["order"])
+    for k, p in zip(batch.kernels, starts):
+        k.setPaths(p)
+    for _ in range(warmup):
+        batch.run()
+    graph = batch.captureGraph(steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.replay(graph)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    upd = batch.sliceUpdatesPerLaunch() * steps
+    return {"metric": "path-slice updates/s, ragged batch in one persistent launch", "value": upd / wall,
+            "unit": "slice-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic: seeded Gaussian blobs (sigma 3 A)",
+            "config": {"workload": "LM2M2 clusters n in {3,4,6,8} x N in {256,512,1024,2048}, 2048 chains each "
+                                   "(32768 chains), S=6, beta=10/K, 32 sweeps per step, hipGraph replay",
+                       "autotune": best, "trials": trials},
+            "acceptance": [round(k.getAcceptanceRate(), 4) for k in batch.kernels]}
+
+
 def cpu_baseline(target_s=12.0):
     """Reference-order CPU restatement (oracle) on this host's cores, bounded sample."""
     import numpy as np
@@ -132,6 +176,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the ESS/s phase")
+    ap.add_argument("--workload", choices=["he3", "ragged"], default="he3",
+                    help="he3: the headline workload; ragged: LM2M2 n in {3,4,6,8} x N in {256..2048}, "
+                         "2048 chains each, in one autotuned persistent launch")
     ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the ESS phase")
     ap.add_argument("--ess-launches", type=int, default=64, help="ESS launches of 16 x 32 sweeps")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -153,6 +200,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5}), flush=True)
+        return
+    if args.workload == "ragged":
+        print(json.dumps(ragged_bench(args.steps, args.warmup)), flush=True)
         return
     system, k = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
     rng = np.random.default_rng(0)

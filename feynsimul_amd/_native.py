@@ -15,7 +15,8 @@ FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
 FS_STORAGE_AUTO, FS_STORAGE_MEMORY, FS_STORAGE_REGISTERS = 0, 1, 2
 
 EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_path_storage",
-           "fs_build_log",
+           "fs_batch_create", "fs_batch_destroy", "fs_batch_path_storage", "fs_batch_build_log", "fs_batch_run",
+           "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
            "fs_source", "fs_run", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
            "fs_ranlux_draw")
 
@@ -64,6 +65,16 @@ def lib():
     L.fs_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp)]
     L.fs_destroy.argtypes = [vp]
     L.fs_path_storage.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.fs_batch_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.POINTER(ctypes.c_char_p), i32, i32, ctypes.c_int,
+                                  ctypes.POINTER(vp)]
+    L.fs_batch_destroy.argtypes = [vp]
+    L.fs_batch_path_storage.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int)]
+    L.fs_batch_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.fs_batch_run.argtypes = [vp, ctypes.POINTER(FsBuffers), ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32, vp]
+    L.fs_batch_capture.argtypes = [vp, ctypes.POINTER(FsBuffers), ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32,
+                                   i32, vp, ctypes.POINTER(vp)]
+    L.fs_graph_launch.argtypes = [vp, vp]
+    L.fs_graph_destroy.argtypes = [vp]
     L.fs_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_source.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_run.argtypes = [vp, ctypes.POINTER(FsBuffers), i32, i32, i32, vp]
@@ -136,6 +147,80 @@ class Engine:
     def close(self):
         if self._h.value:
             lib().fs_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """One compiled batched module (owner of the fs_batch handle)."""
+
+    def __init__(self, cfgs, system_srcs, waves_per_block=0, compile_only=False):
+        n = len(cfgs)
+        self._h = ctypes.c_void_p()
+        arr = (FsConfig * n)(*cfgs)
+        srcs = (ctypes.c_char_p * n)(*[x.encode() for x in system_srcs])
+        L = lib()
+        rc = L.fs_batch_create(arr, srcs, n, int(waves_per_block), int(compile_only), ctypes.byref(self._h))
+        if rc != FS_OK:
+            msg = L.fs_last_error().decode(errors="replace")
+            if self._h.value:
+                self.build_log = _text(L.fs_batch_build_log, self._h)
+                L.fs_batch_destroy(self._h)
+                self._h = ctypes.c_void_p()
+            raise FsError(rc, msg)
+        self.ntypes = n
+        self.build_log = _text(L.fs_batch_build_log, self._h)
+
+    def path_storage(self, t):
+        v = ctypes.c_int(0)
+        check(lib().fs_batch_path_storage(self._h, t, ctypes.byref(v)))
+        return v.value
+
+    @staticmethod
+    def _order(order, n):
+        return None if order is None else (ctypes.c_int32 * n)(*order)
+
+    def run(self, bufs, order, operator_runs, metro_steps, series_offset, grid_blocks, stream):
+        arr = (FsBuffers * self.ntypes)(*bufs)
+        check(lib().fs_batch_run(self._h, arr, self._order(order, self.ntypes), operator_runs, metro_steps,
+                                 series_offset, grid_blocks, stream))
+
+    def capture(self, bufs, order, operator_runs, metro_steps, series_offset, n_launches, grid_blocks, stream):
+        arr = (FsBuffers * self.ntypes)(*bufs)
+        g = ctypes.c_void_p()
+        check(lib().fs_batch_capture(self._h, arr, self._order(order, self.ntypes), operator_runs, metro_steps,
+                                     series_offset, n_launches, grid_blocks, stream, ctypes.byref(g)))
+        return Graph(g)
+
+    def close(self):
+        if self._h.value:
+            lib().fs_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """An instantiated HIP graph of batched launches."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def launch(self, stream):
+        check(lib().fs_graph_launch(self._h, stream))
+
+    def close(self):
+        if self._h.value:
+            lib().fs_graph_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

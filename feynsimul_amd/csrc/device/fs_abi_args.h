@@ -26,3 +26,18 @@ struct fs_kargs {
     int seriesOffset;     // index of the first measurement of this launch in `series`
     int flags;            // reserved (0)
 };
+
+// Persistent batched launch over several system types (one wave per walker).
+// Walkers are numbered in dispatch order: type typeOf[d] owns numbers
+// [start[d], start[d + 1]); wavefronts take numbers from *counter until
+// start[ntypes] is reached.  The run length is the same for every type.
+struct fs_batch_args {
+    const fs_kargs *types;     // [ntypes], indexed by type id
+    const long long *start;    // [ntypes + 1], prefix sums of walkers in dispatch order
+    const int *typeOf;         // [ntypes], type id of each dispatch slot
+    unsigned long long *counter;
+    int ntypes;
+    int operatorRuns;
+    int metroSteps;
+    int seriesOffset;
+};

@@ -1,0 +1,380 @@
+// fs_wave.h -- the wave-parallel Metropolis bisection sampler (gfx950).
+//
+// One wavefront owns one walker for a whole launch.  A block of 64
+// consecutive slices is spread one slice per lane, so the 2^S - 1 midpoints
+// of a segment, the potential evaluations and the action sum all run in
+// parallel across lanes and a segment never leaves registers.  Lane j always
+// holds the slices congruent to j mod 64 and draws from its own stream; with
+// tiling offset r it plays block position (j - r) mod 64.  The offset of
+// every step comes from a per-walker group stream (the ParallelizePath scheme
+// of FeynSimul/GPUsrc/kernel.c:713-716).  The proposal and the acceptance
+// rule are the reference's (kernel.c:258-399); only the assignment of random
+// numbers differs.
+//
+// Everything system-specific comes from a trait class Sys:
+//   constants N, S, DOF, NOPS, NCORR, CORLAGS, NHOBS, HBINS, RESIDENT, BINS,
+//   BINRES and EPS, HLO, HINV, XMIN, XMAX, INVBIN;
+//   V(x), ops(x, acc), corr(x, prod), hobs(x, out) over one slice x[DOF].
+// A module for one system instantiates fs_wave_walker once; the batched
+// scheduler instantiates it once per system type.
+//
+// Path storage: with RESIDENT, lane j keeps slices j, j + 64, ... and their
+// potential in registers for the whole launch (HBM touched on entry and
+// exit); otherwise blocks stream through the cache hierarchy, the next block
+// is prefetched, and the potential at every slice is cached in potCache
+// ([walkers][N], refreshed at the start of each launch), so a move evaluates
+// the potential only at its proposed slices in both storages.
+#pragma once
+
+#include "fs_abi_args.h"
+#include "fs_rng.h"
+#include "fs_stream.h"
+
+template <typename T> FS_DEV T fs_lane_read(T v, int src) { return __shfl(v, src, 64); }
+
+FS_DEV float fs_readlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+FS_DEV double fs_readlane(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over aligned groups of G lanes (xor butterfly); every lane of a group
+// ends with the same bits.
+template <int G, typename T> FS_DEV T fs_group_sum(T v)
+{
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Bytes of LDS one wavefront uses for a system: correlator staging, then a
+// private histogram of the scalar observables.
+template <class Sys> constexpr int fs_wave_lds_bytes()
+{
+    return Sys::N * Sys::NCORR * (int)sizeof(FLOAT_TYPE) + (Sys::NHOBS > 0 ? Sys::HBINS * 4 : 0);
+}
+
+template <class Sys>
+FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char *lds)
+{
+    typedef FLOAT_TYPE F;
+    constexpr int N = Sys::N, S = Sys::S, DOF = Sys::DOF;
+    constexpr int SEG = 1 << S;
+    constexpr int B = N < 64 ? N : 64;
+    constexpr int NB = N / B;
+    constexpr int KPAIR = (DOF + 1) / 2;
+    constexpr int KDRAW = 2 * KPAIR;
+    constexpr int NOPS1 = Sys::NOPS > 0 ? Sys::NOPS : 1;
+    constexpr int NCORR1 = Sys::NCORR > 0 ? Sys::NCORR : 1;
+    static_assert(SEG <= B, "2^S must not exceed min(N, 64)");
+    static_assert(!Sys::RESIDENT || (N >= 64 && NB <= 16), "register storage needs 64 <= N <= 1024");
+
+    F *path = reinterpret_cast<F *>(a.paths) + w * (long)(N * DOF);
+    const bool active = lane < B;
+    F sigtab[S + 1];
+#pragma unroll
+    for (int n = 1; n <= S; n++) sigtab[n] = sqrt(((F)SEG * (F)Sys::EPS / ((F)2.0 * (F)(1 << n))));
+    sigtab[0] = sigtab[1];
+
+    // lane j draws from stream 64 w + j, i.e. the stream follows the slice residue
+    fs_stream rng, grp;
+#if FS_RANLUX
+    rng.load(a.rl, w * 64 + lane);
+    grp.load(a.grl, w);
+#else
+    rng.load(a.seeds, w * 64 + lane);
+    grp.load(a.seeds, a.nWalkers * 64 + w);
+#endif
+
+    fs_u32 nacc = 0;
+    double opS[NOPS1], opC[NOPS1];
+#pragma unroll
+    for (int k = 0; k < NOPS1; k++) opS[k] = opC[k] = 0.0;
+
+    F *cbuf = reinterpret_cast<F *>(lds);
+    unsigned *hbuf = reinterpret_cast<unsigned *>(lds + N * Sys::NCORR * (int)sizeof(F));
+    if constexpr (Sys::NHOBS > 0) {
+        for (int b = lane; b < Sys::HBINS; b += 64) hbuf[b] = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    typedef F slots_t __attribute__((ext_vector_type(NB)));
+    slots_t xs[Sys::RESIDENT ? DOF : 1];
+    slots_t vc;
+    F *pot = reinterpret_cast<F *>(a.potCache) + w * (long)N;
+    if constexpr (Sys::RESIDENT) {
+#pragma unroll 1
+        for (int k = 0; k < NB; k++) {
+            F x[DOF];
+#pragma unroll
+            for (int d = 0; d < DOF; d++) x[d] = path[d * N + k * 64 + lane];
+#pragma unroll
+            for (int d = 0; d < DOF; d++) xs[d][k] = x[d];
+            vc[k] = Sys::V(x);
+        }
+    } else if (active) {
+        // lane j only ever touches the cache entries of slices congruent to j mod B
+#pragma unroll 1
+        for (int k = 0; k < NB; k++) {
+            F x[DOF];
+#pragma unroll
+            for (int d = 0; d < DOF; d++) x[d] = path[d * N + k * B + lane];
+            pot[k * B + lane] = Sys::V(x);
+        }
+    }
+
+    for (int i = 0; i < a.operatorRuns; i++) {
+        for (int j = 0; j < a.metroSteps; j++) {
+            const int r = (int)(grp.integer() & (fs_u32)(SEG - 1));
+            // this lane plays block position p; its slice is r + gB + p
+            const int p = active ? ((lane - r) & (B - 1)) : lane;
+            const int ip = p & (SEG - 1);
+            const bool wrap = lane < r;              // slice lies in the next block of slots
+            const int toPos = (lane + r) & (B - 1);  // lane that plays position `lane`
+            const int lvl = ip ? S - __builtin_ctz((unsigned)ip) : 0;
+            F sig = sigtab[0];
+#pragma unroll
+            for (int n = 1; n <= S; n++) sig = (lvl == n) ? sigtab[n] : sig;
+            // The Levy bisection of a segment, written in closed form: the
+            // midpoint recursion x_m = (x_{m-h} + x_{m+h}) / 2 + sigma_n z_m
+            // resolves to the linear interpolation of the two ends plus, for
+            // each level n, the level-n node covering position i times the tent
+            // 1 - |i - m_n| / h_n (zero at nodes of coarser levels and at the
+            // ends).  Same bridge, no per-level masking.
+            const int segL = p - ip;
+            const int laneL = (segL + r) & (B - 1);
+            const int laneR = (segL + SEG + r) & (B - 1);  // lane r when the end is the block end
+            const F frac = (F)ip * ((F)1.0 / (F)SEG);
+            int srcN[S + 1];
+            F wN[S + 1];
+#pragma unroll
+            for (int n = 1; n <= S; n++) {
+                const int h = SEG >> n;
+                const int m = (ip & ~(2 * h - 1)) + h;
+                srcN[n] = (segL + m + r) & (B - 1);
+                wN[n] = (F)1.0 - (F)abs(ip - m) * ((F)1.0 / (F)h);
+            }
+
+            // memory storage: block g + 1 is loaded while block g is moved; its
+            // position 0 is the right end of block g, which no move of this step changes
+            F xnext[DOF];
+            int snext = (lane + (wrap ? B : 0)) & (N - 1);
+            if constexpr (!Sys::RESIDENT) {
+#pragma unroll
+                for (int d = 0; d < DOF; d++) xnext[d] = path[d * N + snext];
+            }
+#pragma unroll 1
+            for (int g = 0; g < NB; g++) {
+                F xo[DOF], xn[DOF], xr[DOF], z[KDRAW];
+                F vOld;
+                const int g1 = (g + 1) & (NB - 1);
+                int s = 0;
+                if constexpr (Sys::RESIDENT) {
+#pragma unroll
+                    for (int d = 0; d < DOF; d++) {
+                        const F a0 = xs[d][g], a1 = xs[d][g1];
+                        xo[d] = wrap ? a1 : a0;
+                        xr[d] = fs_readlane(a1, r);
+                    }
+                    const F v0 = vc[g], v1 = vc[g1];
+                    vOld = wrap ? v1 : v0;
+                } else {
+                    s = snext;
+#pragma unroll
+                    for (int d = 0; d < DOF; d++) xo[d] = xnext[d];
+                    snext = ((g + 1) * B + lane + (wrap ? B : 0)) & (N - 1);
+#pragma unroll
+                    for (int d = 0; d < DOF; d++) xnext[d] = path[d * N + snext];
+#pragma unroll
+                    for (int d = 0; d < DOF; d++) xr[d] = fs_readlane(xnext[d], r);
+                    vOld = pot[s];
+                }
+                // every lane draws the same count so buffered streams stay in step
+                F u[KDRAW];
+#pragma unroll
+                for (int k = 0; k < KDRAW; k++) u[k] = rng.uniform();
+#pragma unroll
+                for (int k = 0; k < KPAIR; k++) {
+                    // v == 0 gives a zero displacement, as kernel.c:346-349 does
+                    const F v = u[2 * k + 1];
+                    const F rho = v != (F)0.0 ? sig * sqrt(-2.0 * log(v)) : (F)0.0;
+                    z[2 * k] = rho * fs_sin2pi(u[2 * k]);
+                    z[2 * k + 1] = rho * fs_cos2pi(u[2 * k]);
+                }
+#pragma unroll
+                for (int d = 0; d < DOF; d++) {
+                    const F xL = fs_lane_read(xo[d], laneL);
+                    const F xR = fs_lane_read(p == 0 ? xr[d] : xo[d], laneR);
+                    F x = xL + (xR - xL) * frac;
+#pragma unroll
+                    for (int n = 1; n <= S; n++) x += wN[n] * fs_lane_read(z[d], srcN[n]);
+                    xn[d] = x;
+                }
+                const F vNew = Sys::V(xn);
+                const F dv = (ip != 0 && active) ? vNew - vOld : (F)0.0;
+                // segment sums and decisions in position space (lane L = position L)
+                const F dvp = fs_group_sum<SEG>(fs_lane_read(dv, toPos));
+                const F u0p = fs_lane_read(u[0], toPos);
+                const F ua = fs_lane_read(u0p, lane & ~(SEG - 1));
+                const bool accp = !(exp(-(F)Sys::EPS * dvp) < ua);
+                if (accp && (lane & (SEG - 1)) == 0 && active) nacc++;
+                const bool acc = fs_lane_read((int)accp, p) != 0;
+                const bool put = acc && ip != 0 && active;
+                if constexpr (Sys::RESIDENT) {
+                    if constexpr (NB == 1) {
+#pragma unroll
+                        for (int d = 0; d < DOF; d++) xs[d][0] = put ? xn[d] : xs[d][0];
+                        vc[0] = put ? vNew : vc[0];
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < DOF; d++) {
+                            const F a0 = xs[d][g], a1 = xs[d][g1];
+                            xs[d][g] = (put && !wrap) ? xn[d] : a0;
+                            xs[d][g1] = (put && wrap) ? xn[d] : a1;
+                        }
+                        const F v0 = vc[g], v1 = vc[g1];
+                        vc[g] = (put && !wrap) ? vNew : v0;
+                        vc[g1] = (put && wrap) ? vNew : v1;
+                    }
+                } else if (put) {
+#pragma unroll
+                    for (int d = 0; d < DOF; d++) path[d * N + s] = xn[d];
+                    pot[s] = vNew;
+                }
+            }
+        }
+
+        // ---- measurement over all N slices (lane j: slices j, j + B, ...) ----
+        if constexpr (Sys::NOPS > 0 || Sys::BINS || Sys::NCORR > 0 || Sys::NHOBS > 0) {
+            F part[NOPS1];
+#pragma unroll
+            for (int k = 0; k < NOPS1; k++) part[k] = (F)0.0;
+            if (active) {
+#pragma unroll 1
+                for (int k = 0; k < NB; k++) {
+                    const int t = k * B + lane;
+                    F x[DOF];
+                    if constexpr (Sys::RESIDENT) {
+#pragma unroll
+                        for (int d = 0; d < DOF; d++) x[d] = xs[d][k];
+                    } else {
+#pragma unroll
+                        for (int d = 0; d < DOF; d++) x[d] = path[d * N + t];
+                    }
+                    if constexpr (Sys::NOPS > 0) Sys::ops(x, part);
+                    if constexpr (Sys::BINS) {
+                        // DOF-dimensional grid histogram over raw coordinates (kernel.c:478-506)
+                        int idx = 0, inc = 1;
+                        bool ok = true;
+#pragma unroll
+                        for (int q = 0; q < DOF; q++) {
+                            ok = ok && x[q] >= (F)Sys::XMIN && x[q] < (F)Sys::XMAX;
+                            idx += (int)((fs_u32)((x[q] - (F)Sys::XMIN) * (F)Sys::INVBIN)) * inc;
+                            inc *= Sys::BINRES;
+                        }
+                        if (ok) atomicAdd(a.bins + idx, 1u);
+                    }
+                    if constexpr (Sys::NHOBS > 0) {
+                        F ob[Sys::NHOBS > 0 ? Sys::NHOBS : 1];
+                        Sys::hobs(x, ob);
+#pragma unroll
+                        for (int o = 0; o < Sys::NHOBS; o++) {
+                            const F v = (ob[o] - (F)Sys::HLO) * (F)Sys::HINV;
+                            if (v >= (F)0.0 && v < (F)Sys::HBINS) atomicAdd(hbuf + (int)v, 1u);
+                        }
+                    }
+                    if constexpr (Sys::NCORR > 0) {
+                        F prod[NCORR1];
+#pragma unroll
+                        for (int c = 0; c < NCORR1; c++) prod[c] = (F)1.0;
+                        Sys::corr(x, prod);
+#pragma unroll
+                        for (int c = 0; c < Sys::NCORR; c++) cbuf[c * N + t] = prod[c];
+                    }
+                }
+            }
+            if constexpr (Sys::NOPS > 0) {
+#pragma unroll
+                for (int k = 0; k < Sys::NOPS; k++) {
+                    const F m = fs_group_sum<B>(part[k]);
+                    const double y = (double)m - opC[k];
+                    const double t = opS[k] + y;
+                    opC[k] = (t - opS[k]) - y;
+                    opS[k] = t;
+                    if (a.series != nullptr && lane == 0)
+                        a.series[((long)(a.seriesOffset + i) * a.nWalkers + w) * Sys::NOPS + k] = (float)(m / (F)N);
+                }
+            }
+            if constexpr (Sys::NCORR > 0) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                double *cb = reinterpret_cast<double *>(a.corMeans) + w * (long)(Sys::NCORR * Sys::CORLAGS * 2);
+#pragma unroll
+                for (int c = 0; c < Sys::NCORR; c++) {
+                    const F *cv = cbuf + c * N;
+                    for (int tau = lane; tau < Sys::CORLAGS; tau += 64) {
+                        F acc = (F)0.0;
+                        for (int t = 0; t < N; t++) acc += cv[t] * cv[(t + tau) & (N - 1)];
+                        double *bin = cb + (c * Sys::CORLAGS + tau) * 2;
+                        const double sum = bin[0], comp = bin[1];
+                        const double y = (double)acc - comp;
+                        const double t2 = sum + y;
+                        bin[1] = (t2 - sum) - y;
+                        bin[0] = t2;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+    }
+
+    // ---- launch outputs ----
+    if constexpr (Sys::NHOBS > 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        for (int b = lane; b < Sys::HBINS; b += 64) {
+            const unsigned c = hbuf[b];
+            if (c) atomicAdd(a.obsHist + b, c);
+        }
+    }
+    if constexpr (Sys::RESIDENT) {
+#pragma unroll 1
+        for (int k = 0; k < NB; k++) {
+#pragma unroll
+            for (int d = 0; d < DOF; d++) path[d * N + k * 64 + lane] = xs[d][k];
+        }
+    }
+    const fs_u32 tot = fs_group_sum<64>(nacc);
+    if (lane == 0) a.accepts[w] = tot;
+    if constexpr (Sys::NOPS > 0) {
+        if (lane == 0) {
+            double *opm = reinterpret_cast<double *>(a.opMeans);
+#pragma unroll
+            for (int k = 0; k < Sys::NOPS; k++) {
+                opm[w * Sys::NOPS + k] = opS[k] / ((double)a.operatorRuns * (double)N);
+                if (a.opTotals != nullptr) {
+                    double *tk = a.opTotals + (w * Sys::NOPS + k) * 2;
+                    const double y = (opS[k] - opC[k]) - tk[1];
+                    const double t = tk[0] + y;
+                    tk[1] = (t - tk[0]) - y;
+                    tk[0] = t;
+                }
+            }
+        }
+    }
+#if FS_RANLUX
+    rng.store(a.rl, w * 64 + lane);
+    if (lane == 0) grp.store(a.grl, w);
+#else
+    rng.store(a.seeds, w * 64 + lane);
+    if (lane == 0) grp.store(a.seeds, a.nWalkers * 64 + w);
+#endif
+}

@@ -10,6 +10,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -28,6 +29,12 @@ const char kArgsHeader[] = {
     , 0};
 const char kSamplerHeader[] = {
 #embed "device/fs_sampler.h"
+    , 0};
+const char kWaveHeader[] = {
+#embed "device/fs_wave.h"
+    , 0};
+const char kStreamHeader[] = {
+#embed "device/fs_stream.h"
     , 0};
 
 thread_local std::string g_error;
@@ -144,46 +151,26 @@ bool resident_storage(const fs_config *c)
     return (c->N / 64) * (c->DOF + 1) * words <= 64;
 }
 
-std::string build_source(const fs_config *c, const char *system_src, int wpb)
+int nskip_of(const fs_config *c)
+{
+    const int l = c->ranlux_lux;
+    return c->ranlux ? (l == 0 ? 0 : l == 1 ? 24 : l == 2 ? 76 : l == 3 ? 200 : l == 4 ? 380 : l - 24) : 0;
+}
+
+// Definitions shared by every module: precision, random-stream kind, the
+// OpenCL C conveniences the reference's system strings may use
+// (physical_systems/__init__.py:6-9) and the fast sin(2 pi x) helpers.
+std::string common_prelude(const fs_config *c)
 {
     std::string s;
     auto def = [&](const char *k, const std::string &v) { s += "#define " + std::string(k) + " " + v + "\n"; };
-    auto defi = [&](const char *k, long long v) { def(k, std::to_string(v)); };
-    s += "// generated by fs_create\n";
     def("FLOAT_TYPE", c->double_precision ? "double" : "float");
-    defi("FS_DOUBLE", c->double_precision ? 1 : 0);
-    defi("FS_N", c->N);
-    defi("FS_S", c->S);
-    defi("FS_DOF", c->DOF);
-    defi("FS_NOPS", c->n_ops);
-    defi("FS_NCORR", c->n_corr);
-    defi("FS_CORLAGS", c->cor_lags > 0 ? c->cor_lags : c->N / 2);
-    defi("FS_RANLUX", c->ranlux ? 1 : 0);
-    {
-        const int l = c->ranlux_lux;
-        const int nskip = l == 0 ? 0 : l == 1 ? 24 : l == 2 ? 76 : l == 3 ? 200 : l == 4 ? 380 : l - 24;
-        defi("FS_NSKIP", c->ranlux ? nskip : 0);
-        char b[64];
-        snprintf(b, sizeof(b), "%.9Ef", (double)c->ranlux_int_range);  // as kernel.py:334 formats it
-        def("FS_ISCALE", b);
-    }
-    defi("FS_BINS", c->bins ? 1 : 0);
-    defi("FS_BINRES", c->bin_res > 0 ? c->bin_res : 1);
-    def("FS_XMIN", lit(c->x_min));
-    def("FS_XMAX", lit(c->x_max));
-    def("FS_INVBIN", lit(c->bins ? (double)c->bin_res / (c->x_max - c->x_min) : 1.0));
-    def("FS_OPNORM", lit(c->op_norm));
-    defi("FS_PARALLEL", c->parallelize_path ? 1 : 0);
-    defi("FS_W", c->walkers_per_group);
-    defi("FS_LOCAL", c->local_size > 0 ? c->local_size : 1);
-    defi("FS_WPB", wpb);
-    defi("FS_SAMPLER", c->sampler);
-    defi("FS_NHOBS", c->sampler == FS_SAMPLER_WAVE ? c->n_hist_obs : 0);
-    defi("FS_HBINS", c->hist_bins > 0 ? c->hist_bins : 1);
-    def("FS_HLO", lit(c->hist_lo));
-    def("FS_HINV", lit(c->n_hist_obs > 0 ? (double)c->hist_bins / (c->hist_hi - c->hist_lo) : 1.0));
-    defi("FS_RESIDENT", resident_storage(c) ? 1 : 0);
-    def("eps", lit(c->beta / (double)c->N));
+    def("FS_DOUBLE", c->double_precision ? "1" : "0");
+    def("FS_RANLUX", c->ranlux ? "1" : "0");
+    def("FS_NSKIP", std::to_string(nskip_of(c)));
+    char b[64];
+    snprintf(b, sizeof(b), "%.9Ef", (double)c->ranlux_int_range);  // as kernel.py:334 formats it
+    def("FS_ISCALE", b);
     s += R"(
 #ifndef M_PI
 #define M_PI 3.14159265358979323846
@@ -213,6 +200,86 @@ std::string build_source(const fs_config *c, const char *system_src, int wpb)
     else
         s += "__device__ __forceinline__ FLOAT_TYPE fs_sin2pi(FLOAT_TYPE x) { return sin(2.0 * M_PI * x); }\n"
              "__device__ __forceinline__ FLOAT_TYPE fs_cos2pi(FLOAT_TYPE x) { return cos(2.0 * M_PI * x); }\n";
+    return s;
+}
+
+// Trait class of one system for fs_wave_walker (see fs_wave.h).
+std::string wave_trait(const fs_config *c)
+{
+    const bool res = resident_storage(c);
+    std::string s = "struct sys {\n    static constexpr int N = " + std::to_string(c->N) +
+                    ", S = " + std::to_string(c->S) + ", DOF = " + std::to_string(c->DOF) +
+                    ", NOPS = " + std::to_string(c->n_ops) + ", NCORR = " + std::to_string(c->n_corr) +
+                    ", CORLAGS = " + std::to_string(c->cor_lags > 0 ? c->cor_lags : c->N / 2) +
+                    ", NHOBS = " + std::to_string(c->n_hist_obs) +
+                    ", HBINS = " + std::to_string(c->hist_bins > 0 ? c->hist_bins : 1) +
+                    ", RESIDENT = " + std::to_string(res ? 1 : 0) + ", BINS = " + std::to_string(c->bins ? 1 : 0) +
+                    ", BINRES = " + std::to_string(c->bin_res > 0 ? c->bin_res : 1) + ";\n";
+    s += "    static constexpr FLOAT_TYPE EPS = " + lit(c->beta / (double)c->N) + ", HLO = " + lit(c->hist_lo) +
+         ", HINV = " + lit(c->n_hist_obs > 0 ? (double)c->hist_bins / (c->hist_hi - c->hist_lo) : 1.0) +
+         ", XMIN = " + lit(c->x_min) + ", XMAX = " + lit(c->x_max) +
+         ", INVBIN = " + lit(c->bins ? (double)c->bin_res / (c->x_max - c->x_min) : 1.0) + ";\n";
+    s += "    static FS_DEV FLOAT_TYPE V(const FLOAT_TYPE *x) { return fs_potential_v(x); }\n"
+         "    static FS_DEV void ops(const FLOAT_TYPE *x, FLOAT_TYPE *acc) { fs_operator_v(x, acc); }\n"
+         "    static FS_DEV void corr(const FLOAT_TYPE *x, FLOAT_TYPE *prod) { fs_correlator_v(x, prod); }\n"
+         "    static FS_DEV void hobs(const FLOAT_TYPE *x, FLOAT_TYPE *out) { fs_hist_obs_v(x, out); }\n};\n";
+    return s;
+}
+
+size_t wave_lds_bytes(const fs_config *c)
+{
+    const size_t fsize = c->double_precision ? 8 : 4;
+    return (c->n_corr > 0 ? (size_t)c->N * c->n_corr * fsize : 0) +
+           (c->n_hist_obs > 0 ? (size_t)c->hist_bins * 4 : 0);
+}
+
+std::vector<std::string> compile_options(const fs_config *c, const std::string &arch)
+{
+    std::vector<std::string> opts = {"--offload-arch=" + arch, "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                     "-Xclang", "-packed-fp32-ops", "-fno-math-errno", "-ffp-contract=fast"};
+    if (!c->double_precision) {
+        opts.push_back("-Xclang");
+        opts.push_back("-cl-single-precision-constant");
+    }
+    if (c->fast_math) {
+        opts.push_back("-freciprocal-math");
+        opts.push_back("-fno-signed-zeros");
+        opts.push_back("-fno-trapping-math");
+        opts.push_back("-fapprox-func");
+    }
+    return opts;
+}
+
+std::string build_source(const fs_config *c, const char *system_src, int wpb)
+{
+    std::string s;
+    auto def = [&](const char *k, const std::string &v) { s += "#define " + std::string(k) + " " + v + "\n"; };
+    auto defi = [&](const char *k, long long v) { def(k, std::to_string(v)); };
+    s += "// generated by fs_create\n";
+    s += common_prelude(c);
+    defi("FS_N", c->N);
+    defi("FS_S", c->S);
+    defi("FS_DOF", c->DOF);
+    defi("FS_NOPS", c->n_ops);
+    defi("FS_NCORR", c->n_corr);
+    defi("FS_CORLAGS", c->cor_lags > 0 ? c->cor_lags : c->N / 2);
+    defi("FS_BINS", c->bins ? 1 : 0);
+    defi("FS_BINRES", c->bin_res > 0 ? c->bin_res : 1);
+    def("FS_XMIN", lit(c->x_min));
+    def("FS_XMAX", lit(c->x_max));
+    def("FS_INVBIN", lit(c->bins ? (double)c->bin_res / (c->x_max - c->x_min) : 1.0));
+    def("FS_OPNORM", lit(c->op_norm));
+    defi("FS_PARALLEL", c->parallelize_path ? 1 : 0);
+    defi("FS_W", c->walkers_per_group);
+    defi("FS_LOCAL", c->local_size > 0 ? c->local_size : 1);
+    defi("FS_WPB", wpb);
+    defi("FS_SAMPLER", c->sampler);
+    defi("FS_NHOBS", c->sampler == FS_SAMPLER_WAVE ? c->n_hist_obs : 0);
+    defi("FS_HBINS", c->hist_bins > 0 ? c->hist_bins : 1);
+    def("FS_HLO", lit(c->hist_lo));
+    def("FS_HINV", lit(c->n_hist_obs > 0 ? (double)c->hist_bins / (c->hist_hi - c->hist_lo) : 1.0));
+    defi("FS_RESIDENT", resident_storage(c) ? 1 : 0);
+    def("eps", lit(c->beta / (double)c->N));
     s += "#pragma clang force_cuda_host_device begin\n";
     s += system_src ? system_src : "";
     s += "\n#pragma clang force_cuda_host_device end\n#include \"fs_sampler.h\"\n";
@@ -261,13 +328,10 @@ int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs
 
     fs_engine *e = new fs_engine();
     e->cfg = *cfg;
-    const size_t fsize = cfg->double_precision ? 8 : 4;
     int wpb = cfg->waves_per_block > 0 ? cfg->waves_per_block : 4;
     if (cfg->sampler == FS_SAMPLER_WAVE) {
         // per wavefront: correlator staging, then a private observable histogram
-        const size_t cor = cfg->n_corr > 0 ? (size_t)cfg->N * cfg->n_corr * fsize : 0;
-        const size_t hist = cfg->n_hist_obs > 0 ? (size_t)cfg->hist_bins * 4 : 0;
-        const size_t per = cor + hist;
+        const size_t per = wave_lds_bytes(cfg);
         while (wpb > 1 && per * wpb > 65536) wpb--;
         if (per > 65536) {
             delete e;
@@ -279,25 +343,13 @@ int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs
     e->source = build_source(cfg, system_src, wpb);
 
     hiprtcProgram prog;
-    const char *hdrs[] = {kRngHeader, kArgsHeader, kSamplerHeader};
-    const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_sampler.h"};
-    if (hiprtcCreateProgram(&prog, e->source.c_str(), "fs_module.hip", 3, hdrs, names) != HIPRTC_SUCCESS) {
+    const char *hdrs[] = {kRngHeader, kArgsHeader, kSamplerHeader, kWaveHeader, kStreamHeader};
+    const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_sampler.h", "fs_wave.h", "fs_stream.h"};
+    if (hiprtcCreateProgram(&prog, e->source.c_str(), "fs_module.hip", 5, hdrs, names) != HIPRTC_SUCCESS) {
         delete e;
         return fail(FS_ERR_COMPILE, "hiprtcCreateProgram failed");
     }
-    const std::string arch = "--offload-arch=" + (ndev ? device_arch() : std::string("gfx950"));
-    std::vector<std::string> opts = {arch, "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
-                                     "-packed-fp32-ops", "-fno-math-errno", "-ffp-contract=fast"};
-    if (!cfg->double_precision) {
-        opts.push_back("-Xclang");
-        opts.push_back("-cl-single-precision-constant");
-    }
-    if (cfg->fast_math) {
-        opts.push_back("-freciprocal-math");
-        opts.push_back("-fno-signed-zeros");
-        opts.push_back("-fno-trapping-math");
-        opts.push_back("-fapprox-func");
-    }
+    std::vector<std::string> opts = compile_options(cfg, ndev ? device_arch() : std::string("gfx950"));
     std::vector<const char *> copts;
     for (auto &o : opts) copts.push_back(o.c_str());
     const hiprtcResult cr = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
@@ -447,6 +499,340 @@ int fs_eval_system(fs_engine *e, const void *x, void *pot, void *ops, int64_t n,
     const unsigned grid = (unsigned)((n + 255) / 256);
     FS_HIP(hipModuleLaunchKernel(e->fn_eval, grid, 1, 1, 256, 1, 1, 0, reinterpret_cast<hipStream_t>(stream), args,
                                  nullptr));
+    return FS_OK;
+}
+
+// ---- batched systems ----
+}  // extern "C"
+
+struct fs_batch {
+    std::vector<fs_config> cfgs;
+    std::string source;
+    std::string log;
+    std::vector<char> code;
+    hipModule_t module = nullptr;
+    hipFunction_t fn_sweep = nullptr;
+    hipFunction_t fn_reset = nullptr;
+    int wpb = 4;
+    size_t smem = 0;
+    int max_blocks = 0;
+    // device tables and the host copies last uploaded
+    fs_kargs *d_types = nullptr;
+    long long *d_start = nullptr;
+    int *d_type_of = nullptr;
+    unsigned long long *d_counter = nullptr;
+    std::vector<fs_kargs> h_types;
+    std::vector<long long> h_start;
+    std::vector<int> h_type_of;
+};
+
+struct fs_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+std::string batch_source(const std::vector<fs_config> &cfgs, const char *const *srcs, int wpb, size_t lds)
+{
+    std::string s = "// generated by fs_batch_create\n";
+    s += common_prelude(&cfgs[0]);
+    s += "#define FS_WPB " + std::to_string(wpb) + "\n#define FS_LDS_PER_WAVE " + std::to_string(lds) + "\n";
+    s += "#include \"fs_wave.h\"\n";
+    for (size_t k = 0; k < cfgs.size(); k++) {
+        s += "namespace fs_ty" + std::to_string(k) + " {\n#define eps " + lit(cfgs[k].beta / (double)cfgs[k].N) +
+             "\n#pragma clang force_cuda_host_device begin\n";
+        s += srcs[k] ? srcs[k] : "";
+        s += "\n#pragma clang force_cuda_host_device end\n" + wave_trait(&cfgs[k]) + "#undef eps\n}\n";
+    }
+    s += R"(
+extern "C" __global__ void fs_batch_reset(unsigned long long *counter) { *counter = 0ull; }
+
+// Persistent sweep: each wavefront takes walker numbers from the counter and
+// runs that walker's whole launch with its type's specialisation.
+extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_batch_sweep(fs_batch_args b)
+{
+    const int lane = threadIdx.x & 63;
+    extern __shared__ char fs_smem[];
+    char *lds = fs_smem + (threadIdx.x >> 6) * FS_LDS_PER_WAVE;
+    const long long total = b.start[b.ntypes];
+    for (;;) {
+        unsigned long long item = 0;
+        if (lane == 0) item = atomicAdd(b.counter, 1ull);
+        item = __shfl(item, 0, 64);
+        if ((long long)item >= total) break;
+        int d = 0;
+        while (d + 1 < b.ntypes && (long long)item >= b.start[d + 1]) d++;
+        const int t = b.typeOf[d];
+        fs_kargs a = b.types[t];
+        a.operatorRuns = b.operatorRuns;
+        a.metroSteps = b.metroSteps;
+        a.seriesOffset = b.seriesOffset;
+        const long w = (long)((long long)item - b.start[d]);
+        switch (t) {
+)";
+    for (size_t k = 0; k < cfgs.size(); k++)
+        s += "        case " + std::to_string(k) + ": fs_wave_walker<fs_ty" + std::to_string(k) +
+             "::sys>(a, w, lane, lds); break;\n";
+    s += "        default: break;\n        }\n    }\n}\n";
+    return s;
+}
+
+int batch_tables(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
+                 int32_t metro_steps, int32_t series_offset)
+{
+    const int nt = (int)b->cfgs.size();
+    std::vector<fs_kargs> types(nt);
+    for (int t = 0; t < nt; t++) {
+        const fs_config &c = b->cfgs[t];
+        const fs_buffers &x = bufs[t];
+        if (!x.paths || !x.accepts) return fail(FS_ERR_INVALID, "type %d: paths and accepts are required", t);
+        if (c.ranlux ? (!x.rl || !x.grl) : !x.seeds) return fail(FS_ERR_INVALID, "type %d: random states missing", t);
+        if (c.n_ops > 0 && !x.op_means) return fail(FS_ERR_INVALID, "type %d: op_means required", t);
+        if (c.n_corr > 0 && !x.cor_means) return fail(FS_ERR_INVALID, "type %d: cor_means required", t);
+        if (c.bins && !x.bins) return fail(FS_ERR_INVALID, "type %d: bins required", t);
+        if (c.n_hist_obs > 0 && !x.obs_hist) return fail(FS_ERR_INVALID, "type %d: obs_hist required", t);
+        if (!resident_storage(&c) && !x.pot_cache) return fail(FS_ERR_INVALID, "type %d: pot_cache required", t);
+        fs_kargs a{};
+        a.paths = x.paths;
+        a.accepts = static_cast<unsigned *>(x.accepts);
+        a.seeds = static_cast<unsigned *>(x.seeds);
+        a.rl = static_cast<fs_ranlux *>(x.rl);
+        a.grl = static_cast<fs_ranlux *>(x.grl);
+        a.opMeans = x.op_means;
+        a.opTotals = static_cast<double *>(x.op_totals);
+        a.series = static_cast<float *>(x.series);
+        a.corMeans = x.cor_means;
+        a.bins = static_cast<unsigned *>(x.bins);
+        a.potCache = x.pot_cache;
+        a.obsHist = static_cast<unsigned *>(x.obs_hist);
+        a.nWalkers = c.n_walkers;
+        a.operatorRuns = operator_runs;
+        a.metroSteps = metro_steps;
+        a.seriesOffset = series_offset;
+        types[t] = a;
+    }
+    std::vector<int> type_of(nt);
+    std::vector<char> seen(nt, 0);
+    for (int d = 0; d < nt; d++) {
+        const int t = order ? order[d] : d;
+        if (t < 0 || t >= nt || seen[t]) return fail(FS_ERR_INVALID, "order must be a permutation of the types");
+        seen[t] = 1;
+        type_of[d] = t;
+    }
+    std::vector<long long> start(nt + 1, 0);
+    for (int d = 0; d < nt; d++) start[d + 1] = start[d] + b->cfgs[type_of[d]].n_walkers;
+    // upload only what changed; the tables are tiny, and an unchanged set needs no copy
+    if (memcmp(types.data(), b->h_types.data(), sizeof(fs_kargs) * nt) != 0) {
+        FS_HIP(hipMemcpy(b->d_types, types.data(), sizeof(fs_kargs) * nt, hipMemcpyHostToDevice));
+        b->h_types = types;
+    }
+    if (start != b->h_start || type_of != b->h_type_of) {
+        FS_HIP(hipMemcpy(b->d_start, start.data(), sizeof(long long) * (nt + 1), hipMemcpyHostToDevice));
+        FS_HIP(hipMemcpy(b->d_type_of, type_of.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
+        b->h_start = start;
+        b->h_type_of = type_of;
+    }
+    return FS_OK;
+}
+
+int batch_launch(fs_batch *b, int32_t operator_runs, int32_t metro_steps, int32_t series_offset,
+                 int32_t grid_blocks, hipStream_t st)
+{
+    void *rargs[] = {&b->d_counter};
+    FS_HIP(hipModuleLaunchKernel(b->fn_reset, 1, 1, 1, 1, 1, 1, 0, st, rargs, nullptr));
+    fs_batch_args ba{};
+    ba.types = b->d_types;
+    ba.start = b->d_start;
+    ba.typeOf = b->d_type_of;
+    ba.counter = b->d_counter;
+    ba.ntypes = (int)b->cfgs.size();
+    ba.operatorRuns = operator_runs;
+    ba.metroSteps = metro_steps;
+    ba.seriesOffset = series_offset;
+    void *args[] = {&ba};
+    const unsigned grid = (unsigned)(grid_blocks > 0 ? grid_blocks : b->max_blocks);
+    FS_HIP(hipModuleLaunchKernel(b->fn_sweep, grid, 1, 1, 64 * b->wpb, 1, 1, (unsigned)b->smem, st, args, nullptr));
+    return FS_OK;
+}
+
+void batch_free(fs_batch *b)
+{
+    if (!b) return;
+    if (b->d_types) (void)hipFree(b->d_types);
+    if (b->d_start) (void)hipFree(b->d_start);
+    if (b->d_type_of) (void)hipFree(b->d_type_of);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->module) (void)hipModuleUnload(b->module);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32_t ntypes, int32_t waves_per_block,
+                    int compile_only, fs_batch **out)
+{
+    if (!out || !cfgs || !system_srcs) return fail(FS_ERR_INVALID, "cfgs, system_srcs and out are required");
+    *out = nullptr;
+    if (ntypes < 1 || ntypes > 64) return fail(FS_ERR_INVALID, "ntypes must be in [1, 64]");
+    for (int t = 0; t < ntypes; t++) {
+        if (int rc = validate(&cfgs[t])) return rc;
+        const fs_config &c = cfgs[t], &c0 = cfgs[0];
+        if (c.sampler != FS_SAMPLER_WAVE) return fail(FS_ERR_INVALID, "type %d: batches need the wave sampler", t);
+        if (c.double_precision != c0.double_precision || c.ranlux != c0.ranlux ||
+            (c.ranlux && c.ranlux_lux != c0.ranlux_lux) || c.fast_math != c0.fast_math ||
+            c.ranlux_int_range != c0.ranlux_int_range)
+            return fail(FS_ERR_INVALID, "type %d: precision, RNG and math settings must match type 0", t);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) ndev = 0;
+    if (!compile_only && ndev == 0) return fail(FS_ERR_NO_DEVICE, "no GPU visible");
+    fs_batch *b = new fs_batch();
+    b->cfgs.assign(cfgs, cfgs + ntypes);
+    size_t lds = 0;
+    for (auto &c : b->cfgs) lds = std::max(lds, wave_lds_bytes(&c));
+    lds = (lds + 15) & ~(size_t)15;
+    int wpb = waves_per_block > 0 ? waves_per_block : 4;
+    while (wpb > 1 && lds * wpb > 65536) wpb--;
+    if (lds > 65536) {
+        batch_free(b);
+        return fail(FS_ERR_INVALID, "measurement staging needs %zu bytes of LDS per wavefront", lds);
+    }
+    b->wpb = wpb;
+    b->smem = lds * wpb;
+    b->source = batch_source(b->cfgs, system_srcs, wpb, lds);
+    hiprtcProgram prog;
+    const char *hdrs[] = {kRngHeader, kArgsHeader, kWaveHeader, kStreamHeader};
+    const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_wave.h", "fs_stream.h"};
+    if (hiprtcCreateProgram(&prog, b->source.c_str(), "fs_batch.hip", 4, hdrs, names) != HIPRTC_SUCCESS) {
+        batch_free(b);
+        return fail(FS_ERR_COMPILE, "hiprtcCreateProgram failed");
+    }
+    std::vector<std::string> opts = compile_options(&cfgs[0], ndev ? device_arch() : std::string("gfx950"));
+    std::vector<const char *> copts;
+    for (auto &o : opts) copts.push_back(o.c_str());
+    const hiprtcResult cr = hiprtcCompileProgram(prog, (int)copts.size(), copts.data());
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    if (ls > 1) {
+        std::vector<char> log(ls + 1, 0);
+        hiprtcGetProgramLog(prog, log.data());
+        b->log = log.data();
+    }
+    if (cr != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        g_error = "hiprtc compilation failed:\n" + b->log;
+        *out = b;  // the caller can read the build log, then destroy
+        return FS_ERR_COMPILE;
+    }
+    size_t cs = 0;
+    hiprtcGetCodeSize(prog, &cs);
+    b->code.resize(cs);
+    hiprtcGetCode(prog, b->code.data());
+    hiprtcDestroyProgram(&prog);
+    if (!compile_only) {
+        hipError_t e = hipModuleLoadData(&b->module, b->code.data());
+        if (e == hipSuccess) e = hipModuleGetFunction(&b->fn_sweep, b->module, "fs_batch_sweep");
+        if (e == hipSuccess) e = hipModuleGetFunction(&b->fn_reset, b->module, "fs_batch_reset");
+        if (e == hipSuccess) e = hipMalloc(&b->d_types, sizeof(fs_kargs) * ntypes);
+        if (e == hipSuccess) e = hipMalloc(&b->d_start, sizeof(long long) * (ntypes + 1));
+        if (e == hipSuccess) e = hipMalloc(&b->d_type_of, sizeof(int) * ntypes);
+        if (e == hipSuccess) e = hipMalloc(&b->d_counter, sizeof(unsigned long long));
+        int per_cu = 0, dev = 0, cus = 0;
+        if (e == hipSuccess)
+            e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn_sweep, 64 * wpb, b->smem);
+        if (e == hipSuccess) e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e != hipSuccess) {
+            batch_free(b);
+            return fail(FS_ERR_HIP, "batch module setup failed: %s", hipGetErrorString(e));
+        }
+        b->max_blocks = std::max(1, per_cu) * std::max(1, cus);
+    }
+    *out = b;
+    return FS_OK;
+}
+
+int fs_batch_destroy(fs_batch *b)
+{
+    batch_free(b);
+    return FS_OK;
+}
+
+int fs_batch_path_storage(const fs_batch *b, int32_t type, int *storage)
+{
+    if (!b || !storage || type < 0 || type >= (int)b->cfgs.size()) return fail(FS_ERR_INVALID, "bad type");
+    *storage = resident_storage(&b->cfgs[type]) ? FS_STORAGE_REGISTERS : FS_STORAGE_MEMORY;
+    return FS_OK;
+}
+
+int fs_batch_build_log(const fs_batch *b, char *buf, size_t len, size_t *needed)
+{
+    if (!b) return fail(FS_ERR_INVALID, "batch is NULL");
+    return copy_out(b->log, buf, len, needed);
+}
+
+int fs_batch_run(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
+                 int32_t metro_steps, int32_t series_offset, int32_t grid_blocks, void *stream)
+{
+    if (!b || !bufs) return fail(FS_ERR_INVALID, "batch and buffers are required");
+    if (!b->module) return fail(FS_ERR_INVALID, "batch was created compile-only");
+    if (operator_runs < 0 || metro_steps < 0) return fail(FS_ERR_INVALID, "negative run length");
+    if (int rc = batch_tables(b, bufs, order, operator_runs, metro_steps, series_offset)) return rc;
+    return batch_launch(b, operator_runs, metro_steps, series_offset, grid_blocks,
+                        reinterpret_cast<hipStream_t>(stream));
+}
+
+int fs_batch_capture(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
+                     int32_t metro_steps, int32_t series_offset, int32_t n_launches, int32_t grid_blocks,
+                     void *stream, fs_graph **out)
+{
+    if (!b || !bufs || !out) return fail(FS_ERR_INVALID, "batch, buffers and out are required");
+    *out = nullptr;
+    if (!b->module) return fail(FS_ERR_INVALID, "batch was created compile-only");
+    if (!stream) return fail(FS_ERR_INVALID, "capture needs a non-default stream");
+    if (n_launches < 1 || operator_runs < 0 || metro_steps < 0) return fail(FS_ERR_INVALID, "bad run length");
+    if (int rc = batch_tables(b, bufs, order, operator_runs, metro_steps, series_offset)) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    FS_HIP(hipStreamSynchronize(st));
+    FS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int rc = FS_OK;
+    for (int k = 0; k < n_launches && rc == FS_OK; k++)
+        rc = batch_launch(b, operator_runs, metro_steps, series_offset + k * operator_runs, grid_blocks, st);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &graph);
+    if (rc != FS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    if (ec != hipSuccess) return fail(FS_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+    fs_graph *g = new fs_graph();
+    g->graph = graph;
+    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        delete g;
+        return fail(FS_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    }
+    *out = g;
+    return FS_OK;
+}
+
+int fs_graph_launch(fs_graph *g, void *stream)
+{
+    if (!g || !g->exec) return fail(FS_ERR_INVALID, "graph is NULL");
+    FS_HIP(hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream)));
+    return FS_OK;
+}
+
+int fs_graph_destroy(fs_graph *g)
+{
+    if (!g) return FS_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
     return FS_OK;
 }
 

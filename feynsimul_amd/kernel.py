@@ -65,7 +65,7 @@ class _Queue:
 class PIMCKernel:
     """Compiles the sampler for a KernelArgs configuration and runs it on the GPU."""
 
-    def __init__(self, ka, verbose=False):
+    def __init__(self, ka, verbose=False, _batch_member=False):
         self._enableBins = ka.enableBins
         self._enableOperator = ka.enableOperator
         self._enableCorrelator = ka.enableCorrelator
@@ -271,20 +271,12 @@ class PIMCKernel:
             cfg.hist_bins = self._histBins
             cfg.hist_lo, cfg.hist_hi = self._histRange
         self._cfg = cfg
-        src = system_section(self._system, self._operators if cfg.n_ops else (),
-                             self._correlators if cfg.n_corr else (), self._histObs)
-        self._engine = _native.Engine(cfg, src)
-        if verbose:
-            with open("cleaned.hip", "w") as f:
-                f.write(self._engine.source)
-            print("Saved the generated module source in cleaned.hip")
-
-        # ---- device buffers (kernel.py:386-457) ----
+        self._systemSource = system_section(self._system, self._operators if cfg.n_ops else (),
+                                            self._correlators if cfg.n_corr else (), self._histObs)
         seed = getattr(ka, "seed", None)
         if seed is None:
             seed = int.from_bytes(os.urandom(8), "little")
         self._seed = int(seed) & ((1 << 64) - 1)
-        self._allocate()
         self._ev0 = torch.cuda.Event(enable_timing=True)
         self._ev1 = torch.cuda.Event(enable_timing=True)
         self._ranRuns = 0
@@ -292,6 +284,20 @@ class PIMCKernel:
         self._binRuns = 0
         self._seriesCount = 0
         self._lastRunLength = None
+        self._engine = None
+        if _batch_member:
+            return  # a BatchedPIMC compiles every member in one module, then calls _attach
+        self._engine = _native.Engine(cfg, self._systemSource)
+        if verbose:
+            with open("cleaned.hip", "w") as f:
+                f.write(self._engine.source)
+            print("Saved the generated module source in cleaned.hip")
+        self._attach(self._engine.path_storage)
+
+    def _attach(self, storage):
+        """Allocates the device buffers (kernel.py:386-457) for the given path storage."""
+        self._storage = storage
+        self._allocate()
 
     # ------------------------------------------------------------------
     def _zeros(self, shape, dtype):
@@ -320,7 +326,7 @@ class PIMCKernel:
                 self._seedsSnap = self._zeros((rows, 4), t.int32)
         self._oldPath = self._zeros((threads, (2 ** self._S - 1) * DOF), ft) if exact else None
         self._potCache = None
-        if not exact and self._engine.path_storage == _native.FS_STORAGE_MEMORY:
+        if not exact and self._storage == _native.FS_STORAGE_MEMORY:
             self._potCache = self._zeros((W, N), ft)
         self._opTotals = None
         self._series = None
@@ -383,6 +389,8 @@ class PIMCKernel:
     # ------------------------------------------------------------------
     def run(self, wait=True):
         """Runs operatorRuns x metroStepsPerOperatorRun steps (kernel.py:459-506)."""
+        if self._engine is None:
+            raise Exception("this kernel is a member of a BatchedPIMC; run the batch")
         opRuns, M = int(self._operatorRuns), int(self._metroStepsPerOperatorRun)
         offset = 0
         if self._series is not None:
@@ -586,7 +594,17 @@ class PIMCKernel:
         return False
 
     def getBuildLog(self):
-        return self._engine.build_log
+        return self._engine.build_log if self._engine is not None else ""
+
+    def _account_run(self, opRuns, M):
+        """Bookkeeping of one launch run by a BatchedPIMC on this member's buffers."""
+        self._ranRuns += 1
+        self._corRuns += opRuns
+        self._binRuns += opRuns
+        self._obsRuns += opRuns
+        if self._series is not None:
+            self._seriesCount += opRuns
+        self._lastRunLength = (opRuns, M)
 
     def getMetroStepsPerRun(self):
         """Metropolis moves per run (kernel.py:787-801)."""

@@ -149,6 +149,31 @@ int fs_xorshift_draw(const void *rows, void *out_u32, void *out_f32, void *out_f
 /* nquad four-value RANLUX draws per state (advancing the states); lux 0..4. */
 int fs_ranlux_draw(void *tab, void *out_f32, void *out_f64, int64_t n, int32_t nquad, int32_t lux, void *stream);
 
+/* ---- batched systems: one persistent launch over several configurations ----
+ * No reference counterpart as a call: the reference compiles one program per
+ * (system, N, S) and its run loop relaunches them one after another
+ * (kernel.py:346-382, pimc_utils.py:230-260).  Every configuration must use
+ * the wave sampler and agree on precision, RNG kind and luxury; buffers are
+ * given per type in type order.  `order` lists type ids in dispatch order
+ * (NULL: as given); `grid_blocks` 0 = enough blocks to fill the device. */
+typedef struct fs_batch fs_batch;
+typedef struct fs_graph fs_graph;
+int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32_t ntypes,
+                    int32_t waves_per_block, int compile_only, fs_batch **out);
+int fs_batch_destroy(fs_batch *b);
+int fs_batch_path_storage(const fs_batch *b, int32_t type, int *storage);
+int fs_batch_build_log(const fs_batch *b, char *buf, size_t len, size_t *needed);
+int fs_batch_run(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
+                 int32_t metro_steps, int32_t series_offset, int32_t grid_blocks, void *stream);
+/* Records n_launches consecutive fs_batch_run launches (measurement series
+ * offsets advancing by operator_runs) into a HIP graph; replay with
+ * fs_graph_launch.  Buffers and order are fixed at capture. */
+int fs_batch_capture(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
+                     int32_t metro_steps, int32_t series_offset, int32_t n_launches, int32_t grid_blocks,
+                     void *stream, fs_graph **out);
+int fs_graph_launch(fs_graph *g, void *stream);
+int fs_graph_destroy(fs_graph *g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -359,6 +359,17 @@ def ho_discrete_correlator(N: int, beta: float, k: float = 1.0) -> np.ndarray:
     return np.mean(np.cos(2.0 * np.pi * j[None, :] * tau / N) / a[None, :], axis=1)
 
 
+def transfer_matrix_gap(V, N: int, beta: float, L: float = 8.0, M: int = 1200) -> float:
+    """Gap E1 - E0 of the discretised action: ln(lambda_0 / lambda_1) / eps of its transfer matrix."""
+    eps = beta / N
+    x = np.linspace(-L, L, M)
+    h = x[1] - x[0]
+    v = V(x)
+    K = np.exp(-(x[:, None] - x[None, :]) ** 2 / (2 * eps) - eps * (v[:, None] + v[None, :]) / 2) * h
+    w = np.sort(np.linalg.eigvalsh(K))[::-1]
+    return float(np.log(w[0] / w[1]) / eps)
+
+
 def transfer_matrix_1d(V, N: int, beta: float, L: float = 8.0, M: int = 1200):
     """FP64 transfer-matrix quadrature of the primitive action for a 1D potential.
 

@@ -116,3 +116,70 @@ def test_aho_energy_vs_transfer_matrix(gpu):
     ref, _, _ = O.transfer_matrix_1d(system.V, N, beta)
     print("<x^2> = %.6f +- %.6f, transfer matrix %.6f" % (x2, se, ref))
     assert abs(x2 - ref) < 4 * se
+
+
+@pytest.mark.parametrize("ranlux", [False, True])
+def test_aho_correlator_and_gap(gpu, ranlux):
+    """Anharmonic oscillator x^2/2 + 0.1 x^4: two-time correlator <x(0) x(tau)> in FP64 Kahan bins
+    against the FP64 transfer matrix of the discretised action, with xorshift128 and with RANLUX
+    (luxury 2), and the gap from the correlator's decay."""
+    _case("AHO correlator ranlux=%s" % ranlux)
+    system = AnharmOsc(0.1)
+    W, N, S, beta, lags = 4096, 512, 4, 20.0, 96
+    rng = np.random.default_rng(3)
+    burn = PIMCKernel(_ka(system, W, N, S, beta, 1, 600, "wave", ranlux=ranlux))
+    if ranlux:
+        burn.initRanlux()
+        burn.groupInitRanlux()
+    burn.setPaths(rng.uniform(-1, 1, (W, N)))
+    burn.run()
+    k = PIMCKernel(_ka(system, W, N, S, beta, 40, 8, "wave", corr=True, seed=13, lags=lags, ranlux=ranlux,
+                       ops=(system.positionSquaredOp,)))
+    if ranlux:
+        k.initRanlux()
+        k.groupInitRanlux()
+    k.setPaths(burn.getPaths())
+    k.run()
+    mean, se = k.getCorrelator()
+    x2, _, cor = O.transfer_matrix_1d(system.V, N, beta, L=7.0, M=1000)
+    z = np.abs(mean[0] - cor[:lags]) / se[0]
+    print("max z %.2f, C(0) %.5f vs %.5f" % (z.max(), mean[0][0], cor[0]))
+    assert z.max() < 5.5
+    # C(0) is the slice mean of x^2 over the same measurements
+    assert abs(mean[0][0] - k.getOperators()[0].mean()) < 1e-5 * cor[0]
+    # effective gap -d ln C / d tau over a window where the first odd state dominates
+    gap = O.transfer_matrix_gap(system.V, N, beta, L=7.0, M=1000)
+    eps = beta / N
+    t1, t2 = 30, 60
+    g = -np.log(mean[0][t2] / mean[0][t1]) / ((t2 - t1) * eps)
+    g_tm = -np.log(cor[t2] / cor[t1]) / ((t2 - t1) * eps)
+    print("gap from correlator %.4f, same window of the transfer matrix %.4f, eigen gap %.4f" % (g, g_tm, gap))
+    assert abs(g - g_tm) < 0.03 * g_tm
+    assert abs(g_tm - gap) < 0.05 * gap
+
+
+def test_aho_full_size_ranlux_launch(gpu):
+    """The correlator/RNG stress workload at full size: 1D AHO, N=4096, beta=40, 65536 chains, RANLUX
+    luxury 2, 512 correlator lags.  Checked through size-independent identities: C(0) equals the
+    slice mean of x^2, C is bounded by C(0), streams advance, acceptance is a probability."""
+    import torch
+    _case("AHO full size RANLUX")
+    system = AnharmOsc(0.1)
+    W, N, S, beta, lags = 65536, 4096, 4, 40.0, 512
+    k = PIMCKernel(_ka(system, W, N, S, beta, 2, 2, "wave", corr=True, seed=21, lags=lags, ranlux=True,
+                       ops=(system.positionSquaredOp,)))
+    k.initRanlux()
+    k.groupInitRanlux()
+    st0 = k._rl[:1024].clone()
+    rng = np.random.default_rng(4)
+    k.setPaths(rng.uniform(-1, 1, (W, N)).astype(np.float32))
+    k.run()
+    mean, se = k.getCorrelator()
+    assert np.isfinite(mean).all() and np.isfinite(se).all()
+    x2 = k.getOperators()[0]
+    assert abs(mean[0][0] - x2.mean()) < 1e-5 * abs(x2.mean())
+    assert (np.abs(mean[0]) <= mean[0][0] * (1 + 1e-6)).all()
+    assert 0.0 < k.getAcceptanceRate() < 1.0
+    assert not torch.equal(st0, k._rl[:1024])
+    del k
+    torch.cuda.empty_cache()
