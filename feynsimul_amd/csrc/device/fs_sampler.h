@@ -165,9 +165,9 @@ extern "C" __global__ void fs_exact_sweep(fs_kargs a)
 #endif
         for (int j = 0; j < a.metroSteps; j++) {
 #if FS_PARALLEL
-            const fs_u32 start = ((grp.integer() & (fs_u32)(FS_SEG - 1)) + (fs_u32)(FS_SEG * lid0)) % (fs_u32)FS_N;
+            const fs_u32 start = (grp.below((fs_u32)FS_SEG) + (fs_u32)(FS_SEG * lid0)) % (fs_u32)FS_N;
 #else
-            const fs_u32 start = (grp.integer() & (fs_u32)(FS_N - 1)) % (fs_u32)FS_N;
+            const fs_u32 start = grp.below((fs_u32)FS_N);
 #endif
             fs_exact_move(path, oldPath, start, own, accepts);
 #if FS_PARALLEL

@@ -18,6 +18,11 @@ struct fs_stream {
     FS_DEV void store(fs_ranlux *tab, long idx) const { fs_rl_store(tab, idx, g.st); }
     FS_DEV fs_f uniform() { return g.next_float(); }
     FS_DEV fs_u32 integer() { return g.next_int((fs_f)FS_ISCALE); }
+    // Uniform index in [0, n), n a power of two <= 2^24: the top log2(n) bits
+    // of one buffered uniform.  ranluxIntWrapper's (uint)(u * 2^31) & (n - 1)
+    // is 0 whenever u lies on the 2^-24 grid (kernel.c:206, 715), which would
+    // freeze the segment offsets; u * n is exact and below n.
+    FS_DEV fs_u32 below(fs_u32 n) { return (fs_u32)(g.next_float() * (fs_f)n); }
 #else
     fs_xs128 g;
     FS_DEV void load(const fs_u32 *rows, long idx)
@@ -31,6 +36,8 @@ struct fs_stream {
     }
     FS_DEV fs_f uniform() { return fs_xs_float<fs_f>(g); }
     FS_DEV fs_u32 integer() { return fs_xs_int(g); }
+    // kernel.c:715: the low bits of one xorshift word
+    FS_DEV fs_u32 below(fs_u32 n) { return fs_xs_int(g) & (n - 1u); }
 #endif
 };
 

@@ -129,7 +129,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
 
     for (int i = 0; i < a.operatorRuns; i++) {
         for (int j = 0; j < a.metroSteps; j++) {
-            const int r = (int)(grp.integer() & (fs_u32)(SEG - 1));
+            const int r = (int)grp.below((fs_u32)SEG);
             // this lane plays block position p; its slice is r + gB + p
             const int p = active ? ((lane - r) & (B - 1)) : lane;
             const int ip = p & (SEG - 1);

@@ -623,7 +623,7 @@ int batch_tables(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int3
     std::vector<long long> start(nt + 1, 0);
     for (int d = 0; d < nt; d++) start[d + 1] = start[d] + b->cfgs[type_of[d]].n_walkers;
     // upload only what changed; the tables are tiny, and an unchanged set needs no copy
-    if (memcmp(types.data(), b->h_types.data(), sizeof(fs_kargs) * nt) != 0) {
+    if (b->h_types.size() != types.size() || memcmp(types.data(), b->h_types.data(), sizeof(fs_kargs) * nt) != 0) {
         FS_HIP(hipMemcpy(b->d_types, types.data(), sizeof(fs_kargs) * nt, hipMemcpyHostToDevice));
         b->h_types = types;
     }

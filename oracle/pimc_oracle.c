@@ -314,6 +314,15 @@ static inline uint32_t st_int(fs_stream *s)
     return s->ranlux ? rl_wrap_int(&s->rl, s->iscale) : fs_xs_int(&s->xs);
 }
 
+/* Offset draw in [0, n), n a power of two.  xorshift: the low bits of one word
+ * (kernel.c:715).  RANLUX: the top log2(n) bits of one uniform; the reference's
+ * st_int(..) & (n - 1) is 0 for every u on the 2^-24 grid (kernel.c:206), so
+ * its offsets never move. */
+static inline uint32_t st_below(fs_stream *s, uint32_t n)
+{
+    return s->ranlux ? (uint32_t)(rl_wrap_float(&s->rl) * (FLOAT_TYPE)n) : (fs_xs_int(&s->xs) & (n - 1u));
+}
+
 /* Gather the DOF coordinates of one slice (walker path is DOF-major). */
 static inline void gather(const FLOAT_TYPE *path, int N, int s, FLOAT_TYPE *x)
 {
@@ -497,9 +506,9 @@ FS_EXPORT int fs_oracle_run_reference(const fs_oracle_cfg *c, FLOAT_TYPE *paths,
                 for (int k = 0; k < T; k++) {
                     uint32_t start;
                     if (c->parallelizePath)
-                        start = ((st_int(&grp[k]) & (uint32_t)(P - 1)) + (uint32_t)(P * k)) % (uint32_t)N;
+                        start = (st_below(&grp[k], (uint32_t)P) + (uint32_t)(P * k)) % (uint32_t)N;
                     else
-                        start = (st_int(&grp[k]) & (uint32_t)(N - 1)) % (uint32_t)N;
+                        start = st_below(&grp[k], (uint32_t)N);
                     bisect_move(path, N, S, start, &own[k], &acc[k], sigmaN, epsv, oldPath);
                 }
             }
@@ -636,7 +645,7 @@ FS_EXPORT int fs_oracle_run_wave(const fs_oracle_cfg *c, FLOAT_TYPE *paths, uint
         FLOAT_TYPE dv[64];
         for (int i = 0; i < c->operatorRuns; i++) {
             for (int j = 0; j < c->metroSteps; j++) {
-                uint32_t r = st_int(&grp) & (uint32_t)(SEG - 1);
+                uint32_t r = st_below(&grp, (uint32_t)SEG);
                 for (int g = 0; g < NB; g++) {
                     int base = (int)r + g * B;
                     for (int p = 0; p < B; p++) gather(path, N, (base + p) % N, xo[p]);

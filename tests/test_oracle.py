@@ -192,6 +192,35 @@ def test_free_particle_always_accepts():
     assert (out["accepts"] == 15 * (32 // 8)).all()
 
 
+@pytest.mark.parametrize("mode", ["reference", "wave"])
+def test_ranlux_offsets_reach_every_slice(mode):
+    """RANLUX integers are u * 2^31 with u on the 2^-24 grid (kernel.c:206), so
+    the reference's offset draw `int & (2^S - 1)` (kernel.c:715) is always 0 and
+    segment ends never move.  The samplers draw the offset from the top bits of
+    u instead: with V = 0 every slice of a zero path must leave 0 within a few
+    sweeps, and the offsets must cover all 2^S values."""
+    N, S, W, nskip = 64, 4, 8, 76
+    lib = O.build(_Free(), N, 4.0, operators=("x1 * x1",))
+    paths = np.zeros((W, N), np.float32)
+    if mode == "reference":
+        cfg = O.make_cfg(N, S, 1, W, 4.0, 1, 6, ranlux=True, nskip=nskip, nOps=1)
+        rl = O.ranlux_init(lib, O.ranlux_seeds(31, 0, cfg.nThreads))
+        grl = O.ranlux_init(lib, O.ranlux_seeds(31, 1 << 32, cfg.nThreads))
+        O.run_reference(lib, cfg, paths, rl=rl, grl=grl)
+    else:
+        cfg = O.make_cfg(N, S, 1, W, 4.0, 1, 6, ranlux=True, nskip=nskip, nOps=1)
+        rl = O.ranlux_init(lib, O.ranlux_seeds(31, 0, W * 64))
+        grl = O.ranlux_init(lib, O.ranlux_seeds(31, 1 << 32, W))
+        O.run_wave(lib, cfg, paths, lane_rl=rl, group_rl=grl)
+    assert (paths != 0).all()
+    # the offset itself: the top 4 bits of the group stream's uniforms
+    g = O.ranlux_init(lib, O.ranlux_seeds(5, 0, 1))
+    u = O.ranlux_stream(lib, g, nskip, 64)[0]
+    offs = (u * np.float32(1 << S)).astype(np.uint32)
+    assert set(offs.tolist()) == set(range(1 << S))
+    assert ((u * np.float32(2 ** 31)).astype(np.uint64) & ((1 << S) - 1) == 0).all()
+
+
 def test_free_bridge_midpoint_law():
     """V = 0, one level: the midpoint of a segment is N(mean of ends, sigma_1^2),
     sigma_1^2 = 2^S eps / 4 (kernel.c:613-618)."""
