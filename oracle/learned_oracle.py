@@ -1,0 +1,310 @@
+"""CPU restatement of the learned bisection proposal (numpy).
+
+TEST INFRASTRUCTURE ONLY: tests and the bench's CPU baseline may use this
+module as the checker; the product path (feynsimul_amd/) never imports it.
+
+The reference has no learned proposal.  This module states the algorithm the
+engine's learned sampler implements, so that the HIP kernels can be checked
+against it.  It replaces the Levy midpoint draw of kernel.c:338-360 and
+extends the accept test of kernel.c:376-377 with the proposal density ratio:
+
+  level n = 1..S of a segment of 2^S links, node m with ends m - h, m + h,
+  h = 2^(S-n), sigma_n^2 = 2^S eps / (2 * 2^n)           (kernel.c:605-619)
+  features f = [a', b', n / S] (a, b the ends; a', b' centred on the atoms'
+      mean over both ends for translation-invariant clusters, times inScale)
+  MLP  h1 = relu(W1 f + b1), h2 = relu(W2 h1 + b2), h3 = relu(W3 h2 + b3),
+       o = W4 h3 + b4          (BF16 weights, inputs and hidden activations,
+                                FP32 accumulation)
+  mu_d = (a_d + b_d) / 2 + sigma_n o_d,   s_d = clamp(o_{D+d}, -SMAX, SMAX),
+  sigma_d = sigma_n exp(s_d),             new x_d = mu_d + sigma_d z_d
+  log q(node) = sum_d (-z_d^2 / 2 - s_d)  (constants shared by old and new dropped)
+  reverse density: the same network on the OLD path's ends at the old node
+  log alpha = -(K_new - K_old) - eps (V_new - V_old) + log q_old - log q_new
+      K = sum over the segment's 2^S links |x_{p+1} - x_p|^2 / (2 eps)
+  accept iff !(exp(log alpha) < u)
+
+With o = 0 the proposal is the Levy bridge and log alpha reduces to
+-eps dV (the reference's rule).  Training minimises the mean negative log
+likelihood sum_d (z_d^2 / 2 + s_d) of midpoints of equilibrium paths given
+their ends, with Adam (Kingma & Ba 2015).
+
+Random numbers: Philox4x32-10 (Salmon et al., SC'11), counter-based, keyed by
+the 64-bit seed, so every (walker, sweep, slice) draw is independent of the
+order in which the GPU processes rows.
+"""
+import numpy as np
+
+HIDDEN = 128
+SMAX = 3.0
+M0, M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
+W0, W1 = np.uint32(0x9E3779B9), np.uint32(0xBB67AE85)
+TAG_NORMAL, TAG_ACCEPT, TAG_OFFSET, TAG_TRAIN = 0x100, 0x200, 0x300, 0x400
+
+
+# ---------------------------------------------------------------------------
+# Philox4x32-10
+# ---------------------------------------------------------------------------
+def philox4x32(ctr, key, rounds=10):
+    """ctr: uint32 array (..., 4); key: uint32 array (..., 2). Returns (..., 4)."""
+    c = [np.asarray(ctr[..., i], dtype=np.uint32) for i in range(4)]
+    k0 = np.asarray(key[..., 0], dtype=np.uint32).copy()
+    k1 = np.asarray(key[..., 1], dtype=np.uint32).copy()
+    with np.errstate(over="ignore"):
+        for r in range(rounds):
+            if r > 0:
+                k0 = (k0 + W0).astype(np.uint32)
+                k1 = (k1 + W1).astype(np.uint32)
+            p0 = M0 * c[0].astype(np.uint64)
+            p1 = M1 * c[2].astype(np.uint64)
+            hi0, lo0 = (p0 >> np.uint64(32)).astype(np.uint32), (p0 & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+            hi1, lo1 = (p1 >> np.uint64(32)).astype(np.uint32), (p1 & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+            c = [hi1 ^ c[1] ^ k0, lo1, hi0 ^ c[3] ^ k1, lo0]
+    return np.stack(c, axis=-1)
+
+
+def seed_key(seed):
+    s = int(seed) & 0xFFFFFFFFFFFFFFFF
+    return np.array([s & 0xFFFFFFFF, s >> 32], dtype=np.uint32)
+
+
+def _ctr(a, b, c, d):
+    a, b, c, d = np.broadcast_arrays(*(np.asarray(v, dtype=np.uint64) & np.uint64(0xFFFFFFFF) for v in (a, b, c, d)))
+    return np.stack([a, b, c, d], axis=-1).astype(np.uint32)
+
+
+def u24(x, half=False):
+    """uint32 -> float32 uniform from the top 24 bits: [0,1), or (0,1) with half=True."""
+    v = (x >> np.uint32(8)).astype(np.float32)
+    if half:
+        v = v + np.float32(0.5)
+    return (v * np.float32(2.0 ** -24)).astype(np.float32)
+
+
+def normals(seed, walker, sweep, slices, dof):
+    """z[..., d] for each slice: Box-Muller on Philox block (slice, walker, sweep, TAG_NORMAL | d // 4);
+    d % 4 = 0, 1 use the pair (x0, x1) (cos, sin); d % 4 = 2, 3 use (x2, x3)."""
+    key = seed_key(seed)
+    out = np.empty(np.broadcast(walker, slices).shape + (dof,), dtype=np.float32)
+    for blk in range((dof + 3) // 4):
+        x = philox4x32(_ctr(slices, walker, sweep, TAG_NORMAL | blk), key)
+        for pair in range(2):
+            u1 = u24(x[..., 2 * pair], half=True)
+            u2 = u24(x[..., 2 * pair + 1])
+            rho = np.sqrt(np.float32(-2.0) * np.log(u1)).astype(np.float32)
+            ang = (np.float32(2.0 * np.pi) * u2).astype(np.float32)
+            for t, fn in ((0, np.cos), (1, np.sin)):
+                d = 4 * blk + 2 * pair + t
+                if d < dof:
+                    out[..., d] = rho * fn(ang).astype(np.float32)
+    return out
+
+
+def offsets(seed, walkers, sweep, seg):
+    x = philox4x32(_ctr(0, walkers, sweep, TAG_OFFSET), seed_key(seed))
+    return (x[..., 0] & np.uint32(seg - 1)).astype(np.int64)
+
+
+def accept_uniforms(seed, walkers, sweep, segs):
+    x = philox4x32(_ctr(segs, walkers, sweep, TAG_ACCEPT), seed_key(seed))
+    return u24(x[..., 0])
+
+
+def train_rows(seed, step, batch, W, N, SEG):
+    """(walker, segment start slice, node position p in [1, SEG)) of each training row."""
+    b = np.arange(batch, dtype=np.uint64)
+    x = philox4x32(_ctr(b, step, np.uint64(int(step) >> 32), TAG_TRAIN), seed_key(seed))
+    w = ((x[..., 0].astype(np.uint64) * np.uint64(W)) >> np.uint64(32)).astype(np.int64)
+    s0 = (x[..., 1] & np.uint32(N - 1)).astype(np.int64)
+    p = 1 + ((x[..., 2].astype(np.uint64) * np.uint64(SEG - 1)) >> np.uint64(32)).astype(np.int64)
+    return w, s0, p
+
+
+# ---------------------------------------------------------------------------
+# BF16 and the network
+# ---------------------------------------------------------------------------
+def bf16(x):
+    """Round float32 to bfloat16 (nearest, ties to even), returned as float32."""
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    r = (u + np.uint64(0x7FFF) + ((u >> np.uint64(16)) & np.uint64(1))) & np.uint64(0xFFFF0000)
+    return r.astype(np.uint32).view(np.float32).reshape(a.shape)
+
+
+def param_shapes(dof, hidden=HIDDEN):
+    nin, nout = 2 * dof + 1, 2 * dof
+    return [("W1", (hidden, nin)), ("b1", (hidden,)), ("W2", (hidden, hidden)), ("b2", (hidden,)),
+            ("W3", (hidden, hidden)), ("b3", (hidden,)), ("W4", (nout, hidden)), ("b4", (nout,))]
+
+
+def param_count(dof, hidden=HIDDEN):
+    return sum(int(np.prod(s)) for _, s in param_shapes(dof, hidden))
+
+
+def unflatten(theta, dof, hidden=HIDDEN):
+    out, o = {}, 0
+    for name, shape in param_shapes(dof, hidden):
+        n = int(np.prod(shape))
+        out[name] = theta[o:o + n].reshape(shape)
+        o += n
+    return out
+
+
+def features(a, b, level, S, atoms3d, in_scale):
+    """a, b: (..., D) float32 ends; returns (..., 2D+1) float32."""
+    a = a.astype(np.float32)
+    b = b.astype(np.float32)
+    if atoms3d:
+        A = a.shape[-1] // 3
+        c = (a.reshape(a.shape[:-1] + (A, 3)).sum(-2) + b.reshape(b.shape[:-1] + (A, 3)).sum(-2)) \
+            * np.float32(1.0 / (2 * A))
+        a = a - np.tile(c, A)
+        b = b - np.tile(c, A)
+    lv = np.broadcast_to(np.float32(level) / np.float32(S), a.shape[:-1])[..., None].astype(np.float32)
+    return np.concatenate([a * np.float32(in_scale), b * np.float32(in_scale), lv], axis=-1).astype(np.float32)
+
+
+def mlp_forward(p, f, emulate_bf16=True, dtype=np.float32):
+    """Returns (o, cache).  BF16 emulation rounds inputs, weights and hidden activations."""
+    r = bf16 if emulate_bf16 else (lambda v: v)
+    cast = lambda v: np.asarray(v, dtype=dtype)
+    x0 = cast(r(f) if emulate_bf16 else f)
+    Ws = [cast(r(p[k]) if emulate_bf16 else p[k]) for k in ("W1", "W2", "W3", "W4")]
+    bs = [cast(p[k]) for k in ("b1", "b2", "b3", "b4")]
+    hs = [x0]
+    h = x0
+    for L in range(3):
+        a = h @ Ws[L].T + bs[L]
+        h = np.maximum(a, 0).astype(dtype)
+        if emulate_bf16:
+            h = cast(bf16(h))
+        hs.append(h)
+    o = (h @ Ws[3].T + bs[3]).astype(dtype)
+    return o, (hs, Ws)
+
+
+def proposal_params(o, a, b, sigma_n, dof):
+    mid = (a + b) * np.float32(0.5)
+    mu = mid + np.float32(sigma_n) * o[..., :dof]
+    s = np.clip(o[..., dof:2 * dof], -SMAX, SMAX).astype(np.float32)
+    return mu.astype(np.float32), s, (np.float32(sigma_n) * np.exp(s)).astype(np.float32)
+
+
+def nll_and_grad(p, f, a, b, x, sigma_n, dof, emulate_bf16=True, dtype=np.float32):
+    """Mean over rows of sum_d (z_d^2/2 + s_d) and its gradient wrt every parameter."""
+    o, (hs, Ws) = mlp_forward(p, f, emulate_bf16, dtype)
+    B = f.shape[0]
+    sig_n = np.asarray(sigma_n, dtype=dtype).reshape(-1, 1) * np.ones((B, 1), dtype=dtype)
+    mid = (a.astype(dtype) + b.astype(dtype)) * 0.5
+    mu = mid + sig_n * o[:, :dof]
+    sraw = o[:, dof:2 * dof]
+    s = np.clip(sraw, -SMAX, SMAX)
+    z = (x.astype(dtype) - mu) / (sig_n * np.exp(s))
+    loss = float(np.mean(np.sum(0.5 * z * z + s, axis=1)))
+    do = np.zeros_like(o)
+    do[:, :dof] = -z * np.exp(-s)
+    inside = (sraw > -SMAX) & (sraw < SMAX)
+    do[:, dof:2 * dof] = np.where(inside, 1.0 - z * z, 0.0)
+    do /= B
+    grads = {}
+    g = do
+    for L in (3, 2, 1, 0):
+        # every layer's output gradient is stored in BF16 before it is used
+        if emulate_bf16:
+            g = bf16(g.astype(np.float32)).astype(dtype)
+        grads["W%d" % (L + 1)] = (g.T @ hs[L]).astype(dtype)
+        grads["b%d" % (L + 1)] = g.sum(axis=0).astype(dtype)
+        if L > 0:
+            dh = g @ Ws[L]
+            g = (dh * (hs[L] > 0)).astype(dtype)
+    return loss, grads
+
+
+def adam_update(theta, grad, m, v, step, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step (step counts from 1); arrays are float32 and updated in place."""
+    m *= np.float32(b1)
+    m += np.float32(1 - b1) * grad
+    v *= np.float32(b2)
+    v += np.float32(1 - b2) * grad * grad
+    mh = m / np.float32(1 - b1 ** step)
+    vh = v / np.float32(1 - b2 ** step)
+    theta -= np.float32(lr) * mh / (np.sqrt(vh) + np.float32(eps))
+
+
+# ---------------------------------------------------------------------------
+# One learned sweep over every walker (all segments of a walker share one offset)
+# ---------------------------------------------------------------------------
+def sigma_table(S, eps):
+    SEG = 1 << S
+    return [None] + [np.float32(np.sqrt(SEG * eps / (2.0 * (1 << n)))) for n in range(1, S + 1)]
+
+
+def learned_sweep(paths, p, V, *, S, eps, seed, sweep, atoms3d=False, in_scale=1.0, emulate_bf16=True,
+                  return_detail=False):
+    """paths: float32 (W, D, N), updated in place.  V: vectorised potential (..., D) -> (...).
+
+    Returns accepted segments per walker (and, with return_detail, the per-segment
+    log alpha, uniforms and proposals).
+    """
+    W, D, N = paths.shape
+    SEG = 1 << S
+    nseg = N // SEG
+    sig = sigma_table(S, eps)
+    wi = np.arange(W)
+    r = offsets(seed, wi, sweep, SEG)                                   # (W,)
+    j = np.arange(nseg)
+    pos = np.arange(SEG + 1)
+    slices = (r[:, None, None] + j[None, :, None] * SEG + pos[None, None, :]) % N   # (W, nseg, SEG+1)
+    old = paths[wi[:, None, None, None], np.arange(D)[None, None, None, :], slices[..., None]]
+    old = old.astype(np.float32)                                        # (W, nseg, SEG+1, D)
+    new = old.copy()
+    lq = np.zeros((W, nseg), dtype=np.float32)
+    for n in range(1, S + 1):
+        h = SEG >> n
+        m = np.arange(h, SEG, 2 * h)
+        for which in ("new", "old") if n > 1 else ("new",):
+            src = new if which == "new" else old
+            a, b = src[:, :, m - h, :], src[:, :, m + h, :]
+            f = features(a, b, n, S, atoms3d, in_scale)
+            o, _ = mlp_forward(p, f.reshape(-1, f.shape[-1]), emulate_bf16)
+            o = o.reshape(f.shape[:-1] + (o.shape[-1],))
+            mu, s, sd = proposal_params(o, a, b, sig[n], D)
+            if which == "new":
+                z = normals(seed, wi[:, None, None], sweep, slices[:, :, m], D)
+                new[:, :, m, :] = (mu + sd * z).astype(np.float32)
+                lq -= np.sum(-0.5 * z * z - s, axis=(2, 3)).astype(np.float32)
+                if n == 1:
+                    # level 1: the reverse move conditions on the same ends
+                    zo = ((old[:, :, m, :] - mu) / sd).astype(np.float32)
+                    lq += np.sum(-0.5 * zo * zo - s, axis=(2, 3)).astype(np.float32)
+            else:
+                zo = ((old[:, :, m, :] - mu) / sd).astype(np.float32)
+                lq += np.sum(-0.5 * zo * zo - s, axis=(2, 3)).astype(np.float32)
+    inv2e = np.float32(1.0 / (2.0 * eps))
+    kin = lambda x: np.sum(np.sum((x[:, :, 1:, :] - x[:, :, :-1, :]) ** 2, axis=3), axis=2) * inv2e
+    dK = (kin(new) - kin(old)).astype(np.float32)
+    dV = (np.sum(V(new[:, :, 1:SEG, :]), axis=2) - np.sum(V(old[:, :, 1:SEG, :]), axis=2)).astype(np.float32)
+    logA = (-dK - np.float32(eps) * dV + lq).astype(np.float32)
+    u = accept_uniforms(seed, wi[:, None], sweep, j[None, :])
+    acc = ~(np.exp(logA) < u)
+    for w in range(W):
+        for k in np.nonzero(acc[w])[0]:
+            paths[w][:, slices[w, k, 1:SEG]] = new[w, k, 1:SEG, :].T
+    if return_detail:
+        return acc.sum(axis=1), dict(logA=logA, u=u, accept=acc, new=new, old=old, slices=slices, lq=lq, dK=dK,
+                                     dV=dV, r=r)
+    return acc.sum(axis=1)
+
+
+def init_params(dof, seed, hidden=HIDDEN, zero_last=True, scale_last=0.0):
+    """He-normal hidden layers, zero biases; the last layer is zero (the Levy bridge) unless
+    scale_last is given."""
+    rng = np.random.default_rng(seed)
+    p = {}
+    for name, shape in param_shapes(dof, hidden):
+        if name.startswith("b"):
+            p[name] = np.zeros(shape, np.float32)
+        else:
+            p[name] = (rng.standard_normal(shape) * np.sqrt(2.0 / shape[1])).astype(np.float32)
+    if zero_last:
+        p["W4"] = (rng.standard_normal(p["W4"].shape) * scale_last).astype(np.float32)
+    return np.concatenate([p[k].reshape(-1) for k, _ in param_shapes(dof, hidden)]).astype(np.float32)
