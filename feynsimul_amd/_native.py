@@ -18,7 +18,8 @@ EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "f
            "fs_batch_create", "fs_batch_destroy", "fs_batch_path_storage", "fs_batch_build_log", "fs_batch_run",
            "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
            "fs_source", "fs_run", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
-           "fs_ranlux_draw")
+           "fs_ranlux_draw", "fs_lp_create", "fs_lp_destroy", "fs_lp_get_info", "fs_lp_build_log", "fs_lp_pack",
+           "fs_lp_sweep", "fs_lp_train", "fs_lp_eval", "fs_lp_grad")
 
 
 class FsConfig(ctypes.Structure):
@@ -39,6 +40,25 @@ class FsBuffers(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "paths", "accepts", "seeds", "seeds_snap", "rl", "grl", "old_paths", "op_means", "op_totals", "series",
         "cor_means", "bins", "pot_cache", "obs_hist")]
+
+
+class FsLpConfig(ctypes.Structure):
+    _fields_ = [("abi_version", ctypes.c_int32), ("atoms3d", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("segs_per_wave", ctypes.c_int32), ("in_scale", ctypes.c_float), ("lr", ctypes.c_float),
+                ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("adam_eps", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+class FsLpInfo(ctypes.Structure):
+    _fields_ = [("n_params", ctypes.c_int64), ("image_bytes", ctypes.c_int64), ("train_bf16", ctypes.c_int64),
+                ("train_bias", ctypes.c_int64), ("workspace_bytes", ctypes.c_int64), ("loss_parts", ctypes.c_int64),
+                ("segs_per_wave", ctypes.c_int32), ("waves_per_block", ctypes.c_int32),
+                ("lds_bytes", ctypes.c_int32), ("grid_blocks", ctypes.c_int32)]
+
+
+class FsLpBuffers(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "params", "adam_m", "adam_v", "image", "train_w", "train_b", "workspace", "loss_parts", "trace")]
 
 
 class FsError(RuntimeError):
@@ -83,6 +103,16 @@ def lib():
     L.fs_ranlux_init.argtypes = [vp, vp, i64, vp]
     L.fs_xorshift_draw.argtypes = [vp, vp, vp, vp, i64, i32, vp]
     L.fs_ranlux_draw.argtypes = [vp, vp, vp, i64, i32, i32, vp]
+    L.fs_lp_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.c_char_p, ctypes.POINTER(FsLpConfig), ctypes.c_int,
+                               ctypes.POINTER(vp)]
+    L.fs_lp_destroy.argtypes = [vp]
+    L.fs_lp_get_info.argtypes = [vp, ctypes.POINTER(FsLpInfo)]
+    L.fs_lp_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.fs_lp_pack.argtypes = [vp, ctypes.POINTER(FsLpBuffers), vp]
+    L.fs_lp_sweep.argtypes = [vp, vp, vp, ctypes.POINTER(FsLpBuffers), u64, i32, vp]
+    L.fs_lp_train.argtypes = [vp, vp, ctypes.POINTER(FsLpBuffers), u64, vp]
+    L.fs_lp_eval.argtypes = [vp, ctypes.POINTER(FsLpBuffers), vp, vp, i64, vp]
+    L.fs_lp_grad.argtypes = [vp, ctypes.POINTER(FsLpBuffers), vp, vp]
     for name in EXPORTS:
         if name not in ("fs_abi_version", "fs_last_error"):
             getattr(L, name).restype = ctypes.c_int
@@ -221,6 +251,53 @@ class Graph:
     def close(self):
         if self._h.value:
             lib().fs_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Proposal:
+    """One compiled learned-proposal module (owner of the fs_lp handle)."""
+
+    def __init__(self, cfg: FsConfig, system_src: str, lp: FsLpConfig, compile_only=False):
+        self._h = ctypes.c_void_p()
+        L = lib()
+        rc = L.fs_lp_create(ctypes.byref(cfg), system_src.encode(), ctypes.byref(lp), int(compile_only),
+                            ctypes.byref(self._h))
+        if rc != FS_OK:
+            msg = L.fs_last_error().decode(errors="replace")
+            self.build_log = _text(L.fs_lp_build_log, self._h) if self._h.value else ""
+            if self._h.value:
+                L.fs_lp_destroy(self._h)
+                self._h = ctypes.c_void_p()
+            raise FsError(rc, msg)
+        self.cfg, self.lp = cfg, lp
+        self.build_log = _text(L.fs_lp_build_log, self._h)
+        self.info = FsLpInfo()
+        check(L.fs_lp_get_info(self._h, ctypes.byref(self.info)))
+
+    def pack(self, bufs, stream):
+        check(lib().fs_lp_pack(self._h, ctypes.byref(bufs), stream))
+
+    def sweep(self, paths, accepts, bufs, sweep0, n_sweeps, stream):
+        check(lib().fs_lp_sweep(self._h, paths, accepts, ctypes.byref(bufs), sweep0, n_sweeps, stream))
+
+    def train(self, paths, bufs, step, stream):
+        check(lib().fs_lp_train(self._h, paths, ctypes.byref(bufs), step, stream))
+
+    def eval(self, bufs, X, O, rows, stream):
+        check(lib().fs_lp_eval(self._h, ctypes.byref(bufs), X, O, rows, stream))
+
+    def grad(self, bufs, out, stream):
+        check(lib().fs_lp_grad(self._h, ctypes.byref(bufs), out, stream))
+
+    def close(self):
+        if self._h.value:
+            lib().fs_lp_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

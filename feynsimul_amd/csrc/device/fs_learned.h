@@ -1,0 +1,710 @@
+// fs_learned.h -- the learned bisection proposal (gfx950, BF16 MFMA).
+//
+// No counterpart in the reference: it replaces the Levy midpoint draw of
+// FeynSimul/GPUsrc/kernel.c:338-360 by a conditional Gaussian whose mean and
+// log-width come from a small MLP, and extends the accept test of
+// kernel.c:376-377 with the exact proposal density ratio, so detailed balance
+// holds for any weights:
+//
+//   level n, node m with ends a = x(m - h), b = x(m + h), h = 2^(S-n),
+//   sigma_n^2 = 2^S eps / (2 * 2^n)                      (kernel.c:605-619)
+//   f = [a', b', n / S]     (a', b': centred on the atoms' mean for clusters,
+//                            times inScale; zero padded to 32 inputs)
+//   o = W4 relu(W3 relu(W2 relu(W1 f + b1) + b2) + b3) + b4
+//   mu_d = (a_d + b_d)/2 + sigma_n o_d,  s_d = clamp(o_{D+d}, +-SMAX),
+//   x_d = mu_d + sigma_n e^{s_d} z_d,    log q = sum_d (-z_d^2/2 - s_d)
+//   log alpha = -(K_new - K_old) - eps (V_new - V_old) + log q_old - log q_new
+//   accept iff !(exp(log alpha) < u)
+//
+// Inference (fs_lp_sweep): one launch is one sweep.  A wavefront owns SPW
+// consecutive segments of one walker and walks the S levels; the rows of a
+// level (its proposals, then the reverse rows that score the old path) go
+// through the MLP in tiles of 64 rows.  The network is computed transposed,
+// C^T[features x rows] = W[features x in] . H[in x rows], with
+// v_mfma_f32_32x32x16_bf16: weights are the A operand (read from LDS, where a
+// block keeps the whole 82 KB image), activations the B operand.  The C
+// layout of one layer is the B layout of the next once the next layer's input
+// features are permuted to match (the pack kernel does this), so hidden
+// activations never leave registers.  Proposals are staged in LDS; the old
+// path is read from HBM (cached).  Normals and uniforms come from
+// Philox4x32-10 (Salmon et al., SC'11) keyed by the seed and counted by
+// (slice or segment, walker, sweep, purpose): no per-chain state, no order
+// dependence between rows.
+//
+// Training (fs_lp_gather, fs_lp_gemm_*, fs_lp_loss, fs_lp_colsum, fs_lp_adam):
+// rows (walker, segment start, node) are drawn with Philox; the MLP forward
+// and backward are plain BF16 GEMMs on MFMA with FP32 accumulation and
+// ReLU/mask epilogues; weight gradients are split over the batch and summed
+// in a fixed order; Adam (Kingma & Ba) updates FP32 master weights, and
+// fs_lp_pack rebuilds the BF16 image.
+//
+// Needs: a trait `sys` (N, S, DOF, EPS, V), FS_LP_SPW (segments per
+// wavefront), FS_LP_WAVES (wavefronts per block), FS_LP_ATOMS3D.
+#pragma once
+
+#include "fs_lp_abi.h"
+#include "fs_rng.h"
+
+typedef __attribute__((ext_vector_type(8))) __bf16 fs_bf8;
+typedef __attribute__((ext_vector_type(16))) float fs_f16v;
+
+#define FS_LP_KIN 32
+#define FS_LP_SMAX 3.0f
+#define FS_LP_TAG_NORMAL 0x100u
+#define FS_LP_TAG_ACCEPT 0x200u
+#define FS_LP_TAG_OFFSET 0x300u
+#define FS_LP_TAG_TRAIN 0x400u
+
+// ---- packed inference image (offsets in fs_lp_abi.h) ----
+// W1p [4 ob][2 ks][64 lanes][8], W2p/W3p [4][8][64][8], W4p [8 ks][64][8] (BF16),
+// then FP32 biases b1, b2, b3 (logical order) and b4p (output slot order).
+
+// ---- Philox4x32-10 ----
+FS_DEV uint4 fs_philox(uint4 c, fs_u32 k0, fs_u32 k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const fs_u32 hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const fs_u32 hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    }
+    return c;
+}
+
+// top 24 bits as a uniform in [0, 1), or (0, 1) with the half offset
+FS_DEV float fs_lp_u24(fs_u32 x) { return (float)(x >> 8) * 0x1p-24f; }
+FS_DEV float fs_lp_u24h(fs_u32 x) { return ((float)(x >> 8) + 0.5f) * 0x1p-24f; }
+
+// Normal for coordinate d = 4 blk + 2 pair + t of a slice: Box-Muller on the
+// Philox words (2 pair, 2 pair + 1); t = 0 takes the cosine, t = 1 the sine.
+FS_DEV float fs_lp_normal(uint4 x, int pair, int t)
+{
+    const fs_u32 a = pair ? x.z : x.x, b = pair ? x.w : x.y;
+    const float rho = sqrtf(-2.0f * logf(fs_lp_u24h(a)));
+    const float v = fs_lp_u24(b);
+    return rho * (t ? fs_sin2pi(v) : fs_cos2pi(v));
+}
+
+FS_DEV fs_f16v fs_lp_mfma(fs_bf8 a, fs_bf8 b, fs_f16v c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+
+// ---- the network on a 64-row tile held by one wavefront ----
+// xin[ks][cb]: layer-1 B operand (inputs ks*16 + 8g + e of row cb*32 + lane%32).
+// out[cb][j]: output slot (g, j) of row cb*32 + lane%32; slot j of lane group g
+// holds mu-offset (j even) or s (j odd) of coordinate d = 2 (j/2) + g.
+FS_DEV void fs_lp_bias_init(const float *b, int ob, int g, fs_f16v &c0, fs_f16v &c1)
+{
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float4 v = *reinterpret_cast<const float4 *>(b + ob * 32 + 8 * q + 4 * g);
+        c0[4 * q + 0] = c1[4 * q + 0] = v.x;
+        c0[4 * q + 1] = c1[4 * q + 1] = v.y;
+        c0[4 * q + 2] = c1[4 * q + 2] = v.z;
+        c0[4 * q + 3] = c1[4 * q + 3] = v.w;
+    }
+}
+
+FS_DEV void fs_lp_relu_pack(const fs_f16v (&acc)[4][2], fs_bf8 (&h)[8][2])
+{
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+            for (int hh = 0; hh < 2; hh++)
+#pragma unroll
+                for (int e = 0; e < 8; e++) h[2 * ob + hh][cb][e] = (__bf16)fmaxf(acc[ob][cb][8 * hh + e], 0.0f);
+}
+
+FS_DEV void fs_lp_hidden(const char *img, int woff, int boff, const fs_bf8 (&hin)[8][2], fs_f16v (&acc)[4][2],
+                         int lane)
+{
+    const int g = lane >> 5;
+    const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + woff);
+    const float *b = reinterpret_cast<const float *>(img + boff);
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++) fs_lp_bias_init(b, ob, g, acc[ob][0], acc[ob][1]);
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++)
+#pragma unroll
+        for (int ob = 0; ob < 4; ob++) {
+            const fs_bf8 A = W[(ob * 8 + ks) * 64 + lane];
+            acc[ob][0] = fs_lp_mfma(A, hin[ks][0], acc[ob][0]);
+            acc[ob][1] = fs_lp_mfma(A, hin[ks][1], acc[ob][1]);
+        }
+}
+
+FS_DEV void fs_lp_mlp(const char *img, const fs_bf8 (&xin)[2][2], fs_f16v (&out)[2], int lane)
+{
+    const int g = lane >> 5;
+    fs_f16v acc[4][2];
+    {
+        const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_OFF_W1);
+        const float *b = reinterpret_cast<const float *>(img + FS_LP_OFF_B1);
+#pragma unroll
+        for (int ob = 0; ob < 4; ob++) fs_lp_bias_init(b, ob, g, acc[ob][0], acc[ob][1]);
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++)
+#pragma unroll
+            for (int ob = 0; ob < 4; ob++) {
+                const fs_bf8 A = W[(ob * 2 + ks) * 64 + lane];
+                acc[ob][0] = fs_lp_mfma(A, xin[ks][0], acc[ob][0]);
+                acc[ob][1] = fs_lp_mfma(A, xin[ks][1], acc[ob][1]);
+            }
+    }
+    // layer boundaries are scheduling barriers: without them the weight
+    // loads of later layers are hoisted and the live set outgrows the registers
+    fs_bf8 h[8][2];
+    fs_lp_relu_pack(acc, h);
+    __builtin_amdgcn_sched_barrier(0);
+    fs_lp_hidden(img, FS_LP_OFF_W2, FS_LP_OFF_B2, h, acc, lane);
+    fs_lp_relu_pack(acc, h);
+    __builtin_amdgcn_sched_barrier(0);
+    fs_lp_hidden(img, FS_LP_OFF_W3, FS_LP_OFF_B3, h, acc, lane);
+    fs_lp_relu_pack(acc, h);
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_OFF_W4);
+        const float *b = reinterpret_cast<const float *>(img + FS_LP_OFF_B4);
+        fs_lp_bias_init(b, 0, g, out[0], out[1]);
+#pragma unroll
+        for (int ks = 0; ks < 8; ks++) {
+            const fs_bf8 A = W[ks * 64 + lane];
+            out[0] = fs_lp_mfma(A, h[ks][0], out[0]);
+            out[1] = fs_lp_mfma(A, h[ks][1], out[1]);
+        }
+    }
+}
+
+// Feature vector of one row from its ends (registers, compile-time D).
+template <int D>
+FS_DEV void fs_lp_features(const float (&a)[D], const float (&b)[D], float level, float in_scale, float (&f)[32])
+{
+    float c[3] = {0.0f, 0.0f, 0.0f};
+#if FS_LP_ATOMS3D
+#pragma unroll
+    for (int d = 0; d < D; d++) c[d % 3] += a[d] + b[d];
+#pragma unroll
+    for (int k = 0; k < 3; k++) c[k] *= 1.0f / (float)(2 * (D / 3));
+#endif
+#pragma unroll
+    for (int i = 0; i < 32; i++) f[i] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        f[d] = (a[d] - c[d % 3]) * in_scale;
+        f[D + d] = (b[d] - c[d % 3]) * in_scale;
+    }
+    f[2 * D] = level;
+}
+
+// layer-1 B operand of a lane from its row's features.  Both halves are
+// converted with constant indices and the lane group selects whole vectors, so
+// the feature array never needs a runtime index (which would put it in scratch).
+FS_DEV void fs_lp_xin(const float (&f)[32], int g, fs_bf8 &k0, fs_bf8 &k1)
+{
+    fs_bf8 q[4];
+#pragma unroll
+    for (int h = 0; h < 4; h++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) q[h][e] = (__bf16)f[8 * h + e];
+    k0 = g ? q[1] : q[0];
+    k1 = g ? q[3] : q[2];
+}
+
+
+
+template <class Sys> struct fs_lp_geom {
+    static constexpr int N = Sys::N, S = Sys::S, D = Sys::DOF, SEG = 1 << S, NSEG = N / SEG;
+    static constexpr int SPW = FS_LP_SPW;
+    static constexpr int UPW = NSEG / SPW;                                // units per walker
+    static constexpr int STAGE = SPW * (SEG - 1) * D;                     // floats of staged proposals
+    static constexpr int WAVE_BYTES = ((STAGE + 2 * SPW) * 4 + 15) / 16 * 16; // + log q accumulators
+    static_assert(S >= 1 && S <= 6, "learned proposal: 1 <= S <= 6");
+    static_assert(2 * D + 1 <= 32 && D <= 15, "learned proposal: DOF <= 15");
+    static_assert(NSEG % SPW == 0, "segments per wavefront must divide N / 2^S");
+};
+
+// One wavefront, one unit: segments k0 .. k0 + SPW - 1 of walker w, one sweep.
+template <class Sys>
+FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long w, int k0, int lane)
+{
+    typedef fs_lp_geom<Sys> G;
+    constexpr int N = G::N, S = G::S, D = G::D, SEG = G::SEG, SPW = G::SPW;
+    const int g = lane >> 5, c32 = lane & 31;
+    float *path = a.paths + w * (long)(N * D);
+    // log q(old) - log q(new) per segment, in two halves: [k] collects the proposal rows,
+    // [SPW + k] the reverse rows.  Within one instruction the leader lanes of a segment's
+    // proposal rows and of its reverse rows then update different words (a shared word
+    // would lose one of the two read-modify-writes).
+    float *lq = stage + G::STAGE;
+    const fs_u32 wl = (fs_u32)w;
+    const int r = (int)(fs_philox(make_uint4(0u, wl, a.sweep, FS_LP_TAG_OFFSET), a.key0, a.key1).x & (SEG - 1));
+    if (lane < 2 * SPW) lq[lane] = 0.0f;
+    auto slice_of = [&](int k, int pos) { return (r + (k0 + k) * SEG + pos) & (N - 1); };
+    auto value = [&](int k, int pos, bool old, int d) -> float {
+        if (old || pos == 0 || pos == SEG) return path[d * N + slice_of(k, pos)];
+        return stage[(k * (SEG - 1) + pos - 1) * D + d];
+    };
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+
+#pragma unroll 1
+    for (int n = 1; n <= S; n++) {
+        const int h = SEG >> n, nn = 1 << (n - 1);
+        const int NR = SPW * nn, total = n == 1 ? NR : 2 * NR;
+        const float sig = sqrtf((float)SEG * (float)Sys::EPS / (2.0f * (float)(1 << n)));
+        const float level = (float)n / (float)S;
+#pragma unroll 1
+        for (int t0 = 0; t0 < total; t0 += 64) {
+            fs_bf8 xin[2][2];
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) {
+                const int R = t0 + cb * 32 + c32;
+                float f[32];
+                if (R < total) {
+                    const bool old = R >= NR;
+                    const int Rl = old ? R - NR : R, k = Rl >> (n - 1), m = (2 * (Rl & (nn - 1)) + 1) * h;
+                    float ea[D], eb[D];
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        ea[d] = value(k, m - h, old, d);
+                        eb[d] = value(k, m + h, old, d);
+                    }
+                    fs_lp_features<D>(ea, eb, level, a.inScale, f);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 32; i++) f[i] = 0.0f;
+                }
+                fs_lp_xin(f, g, xin[0][cb], xin[1][cb]);
+            }
+            fs_f16v out[2];
+            fs_lp_mlp(img, xin, out, lane);
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) {
+                const int R = t0 + cb * 32 + c32;
+                const bool valid = R < total, old = R >= NR;
+                const int Rl = old ? R - NR : R, k = Rl >> (n - 1), m = (2 * (Rl & (nn - 1)) + 1) * h;
+                const int s = slice_of(k, m);
+                float contrib = 0.0f;
+                if (valid) {
+                    uint4 px = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int d = 2 * q + g;
+                        if (q % 2 == 0 && !old) px = fs_philox(make_uint4((fs_u32)s, wl, a.sweep, FS_LP_TAG_NORMAL | (q / 2)), a.key0, a.key1);
+                        if (d < D) {
+                            const float mid = 0.5f * (value(k, m - h, old, d) + value(k, m + h, old, d));
+                            const float mu = mid + sig * out[cb][2 * q];
+                            const float sv = fminf(fmaxf(out[cb][2 * q + 1], -FS_LP_SMAX), FS_LP_SMAX);
+                            const float sd = sig * expf(sv);
+                            const float xo = path[d * N + s];
+                            if (!old) {
+                                const float z = fs_lp_normal(px, q % 2, g);
+                                stage[(k * (SEG - 1) + m - 1) * D + d] = mu + sd * z;
+                                contrib -= -0.5f * z * z - sv;
+                            }
+                            if (old || n == 1) {
+                                const float zo = (xo - mu) / sd;
+                                contrib += -0.5f * zo * zo - sv;
+                            }
+                        }
+                    }
+                }
+                contrib += __shfl_xor(contrib, 32, 64);
+                for (int o = 1; o < nn; o <<= 1) contrib += __shfl_xor(contrib, o, 64);
+                if (valid && g == 0 && (c32 & (nn - 1)) == 0) lq[(old ? SPW : 0) + k] += contrib;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+
+    // ---- accept or reject each segment (lanes = positions 0 .. SEG - 1) ----
+    constexpr int SPP = 64 / SEG;
+    const float inv2e = 0.5f / (float)Sys::EPS;
+    unsigned nacc = 0;
+#pragma unroll 1
+    for (int k1 = 0; k1 < SPW; k1 += SPP) {
+        const int k = k1 + lane / SEG, p = lane & (SEG - 1);
+        const bool valid = k < SPW;
+        float dk = 0.0f, dv = 0.0f;
+        if (valid) {
+            float xn[D], xo[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                const float n0 = value(k, p, false, d), n1 = value(k, p + 1, false, d);
+                const float o0 = value(k, p, true, d), o1 = value(k, p + 1, true, d);
+                dk += (n1 - n0) * (n1 - n0) - (o1 - o0) * (o1 - o0);
+                xn[d] = n0;
+                xo[d] = o0;
+            }
+            dk *= inv2e;
+            if (p > 0) dv = (float)Sys::V(xn) - (float)Sys::V(xo);
+        }
+#pragma unroll
+        for (int o = 1; o < SEG; o <<= 1) {
+            dk += __shfl_xor(dk, o, 64);
+            dv += __shfl_xor(dv, o, 64);
+        }
+        if (valid) {
+            const float lqk = lq[k] + lq[SPW + k];
+            const float logA = -dk - (float)Sys::EPS * dv + lqk;
+            if (a.trace && p == 0) {
+                float *tr = a.trace + ((long)w * G::NSEG + k0 + k) * 4;
+                tr[0] = lqk;
+                tr[1] = dk;
+                tr[2] = dv;
+                tr[3] = logA;
+            }
+            const float u = fs_lp_u24(fs_philox(make_uint4((fs_u32)(k0 + k), wl, a.sweep, FS_LP_TAG_ACCEPT), a.key0, a.key1).x);
+            if (!(expf(logA) < u)) {
+                if (p > 0) {
+#pragma unroll
+                    for (int d = 0; d < D; d++) path[d * N + slice_of(k, p)] = value(k, p, false, d);
+                } else {
+                    nacc++;
+                }
+            }
+        }
+    }
+    // every lane 0 of a segment counted its own acceptances
+    for (int o = 1; o < 64; o <<= 1) nacc += __shfl_xor(nacc, o, 64);
+    if (lane == 0 && nacc) atomicAdd(a.accepts + w, nacc);
+}
+
+// ---- pack: FP32 master parameters (logical) -> inference image and training matrices ----
+// Logical parameter order: W1 [128][2D+1], b1 [128], W2 [128][128], b2, W3 [128][128], b3,
+// W4 [2D][128], b4 [2D].  Training matrices (BF16): W1b [128][32], W2b, W3b [128][128],
+// W4b [32][128]; then FP32 biases b1..b3 [128] and b4 [32] (zero padded).
+template <int D> struct fs_lp_params {
+    static constexpr int NIN = 2 * D + 1, NOUT = 2 * D, H = FS_LP_HID;
+    static constexpr int W1 = 0, B1 = W1 + H * NIN, W2 = B1 + H, B2 = W2 + H * H, W3 = B2 + H, B3 = W3 + H * H,
+                         W4 = B3 + H, B4 = W4 + NOUT * H, COUNT = B4 + NOUT;
+    // training matrices, element offsets in the BF16 part
+    static constexpr int T_W1 = 0, T_W2 = T_W1 + H * 32, T_W3 = T_W2 + H * H, T_W4 = T_W3 + H * H,
+                         T_BF16 = T_W4 + 32 * H;
+    static constexpr int T_BIAS_FLOATS = 3 * H + 32;
+};
+
+// logical output row of output slot f (-1: unused)
+template <int D> FS_DEV int fs_lp_out_row(int f)
+{
+    const int g = (f >> 2) & 1, j = 4 * (f >> 3) + (f & 3), d = 2 * (j >> 1) + g;
+    return d < D ? ((j & 1) ? D + d : d) : -1;
+}
+
+// input feature read by element e of lane group g at k-step ks of a hidden layer
+FS_DEV int fs_lp_hidden_in(int ks, int g, int e) { return (ks >> 1) * 32 + 16 * (ks & 1) + 8 * (e >> 2) + 4 * g + (e & 3); }
+
+template <int D> FS_DEV void fs_lp_pack_body(const float *P, char *img, __bf16 *tw, float *tb, int i)
+{
+    typedef fs_lp_params<D> Q;
+    // inference image: 40960 BF16 weights, one per thread
+    if (i < 40960) {
+        __bf16 *wimg = reinterpret_cast<__bf16 *>(img);
+        float v = 0.0f;
+        if (i < 4096) {  // W1p [ob][ks][lane][e]
+            const int e = i & 7, lane = (i >> 3) & 63, ks = (i >> 9) & 1, ob = i >> 10;
+            const int o = ob * 32 + (lane & 31), in = ks * 16 + (lane >> 5) * 8 + e;
+            v = in < Q::NIN ? P[Q::W1 + o * Q::NIN + in] : 0.0f;
+        } else if (i < 36864) {  // W2p, W3p
+            const int L = i < 20480 ? 0 : 1, j = i - (L ? 20480 : 4096);
+            const int e = j & 7, lane = (j >> 3) & 63, ks = (j >> 9) & 7, ob = j >> 12;
+            const int o = ob * 32 + (lane & 31), in = fs_lp_hidden_in(ks, lane >> 5, e);
+            v = P[(L ? Q::W3 : Q::W2) + o * Q::H + in];
+        } else {  // W4p [ks][lane][e]
+            const int j = i - 36864, e = j & 7, lane = (j >> 3) & 63, ks = j >> 9;
+            const int row = fs_lp_out_row<D>(lane & 31), in = fs_lp_hidden_in(ks, lane >> 5, e);
+            v = row >= 0 ? P[Q::W4 + row * Q::H + in] : 0.0f;
+        }
+        wimg[i] = (__bf16)v;
+    }
+    if (i < 416) {
+        float *bimg = reinterpret_cast<float *>(img + FS_LP_OFF_B1);
+        float v;
+        if (i < 384) {
+            v = P[(i < 128 ? Q::B1 : i < 256 ? Q::B2 : Q::B3) + (i & 127)];
+        } else {
+            const int row = fs_lp_out_row<D>(i - 384);
+            v = row >= 0 ? P[Q::B4 + row] : 0.0f;
+        }
+        bimg[i] = v;
+        // training biases (logical, zero padded)
+        tb[i] = i < 384 ? v : ((i - 384) < Q::NOUT ? P[Q::B4 + (i - 384)] : 0.0f);
+    }
+    // training matrices
+    if (i < Q::T_BF16) {
+        float v = 0.0f;
+        if (i < Q::T_W2) {
+            const int o = i / 32, in = i % 32;
+            v = in < Q::NIN ? P[Q::W1 + o * Q::NIN + in] : 0.0f;
+        } else if (i < Q::T_W3) {
+            v = P[Q::W2 + (i - Q::T_W2)];
+        } else if (i < Q::T_W4) {
+            v = P[Q::W3 + (i - Q::T_W3)];
+        } else {
+            const int j = i - Q::T_W4, o = j / Q::H, in = j % Q::H;
+            v = o < Q::NOUT ? P[Q::W4 + o * Q::H + in] : 0.0f;
+        }
+        tw[i] = (__bf16)v;
+    }
+}
+
+// ---- training: rows, GEMM, loss, column sums, Adam ----
+
+
+template <class Sys> FS_DEV void fs_lp_gather_row(const fs_lp_train_args &a, int b)
+{
+    constexpr int N = Sys::N, S = Sys::S, D = Sys::DOF, SEG = 1 << S;
+    const uint4 x = fs_philox(make_uint4((fs_u32)b, a.stepLo, a.stepHi, FS_LP_TAG_TRAIN), a.key0, a.key1);
+    const long w = (long)(((unsigned long long)x.x * (unsigned long long)a.nWalkers) >> 32);
+    const int s0 = (int)(x.y & (N - 1));
+    const int p = 1 + (int)(((unsigned long long)x.z * (unsigned long long)(SEG - 1)) >> 32);
+    const int h = p & -p;
+    const int n = S - (31 - __builtin_clz((unsigned)h));
+    const float *path = a.paths + w * (long)(N * D);
+    float ea[D], eb[D], f[32];
+    float *T = a.T + (long)b * (3 * D + 1);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        ea[d] = path[d * N + ((s0 + p - h) & (N - 1))];
+        eb[d] = path[d * N + ((s0 + p + h) & (N - 1))];
+        T[d] = ea[d];
+        T[D + d] = eb[d];
+        T[2 * D + d] = path[d * N + ((s0 + p) & (N - 1))];
+    }
+    T[3 * D] = sqrtf((float)SEG * (float)Sys::EPS / (2.0f * (float)(1 << n)));
+    fs_lp_features<D>(ea, eb, (float)n / (float)S, a.inScale, f);
+    __bf16 *X0 = reinterpret_cast<__bf16 *>(a.X0);
+#pragma unroll
+    for (int i = 0; i < 32; i++) X0[(long)b * 32 + i] = (__bf16)f[i];
+}
+
+// C[M x N] = A[M x K] B[K x N] on one 32 x 32 tile per wavefront; K range split
+// over blockIdx.z (partials at C + z M N).  AK / BK: K is the contiguous index.
+// Epilogues: 0 FP32 (+ bias[n]), 1 BF16 relu(acc + bias[n]), 2 BF16 acc * (H[m][n] > 0).
+
+
+template <bool AK, bool BK> FS_DEV void fs_lp_gemm_body(const fs_lp_gemm_args &p)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 5, c32 = lane & 31;
+    const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32;
+    const int kc = p.K / gridDim.z, kb = blockIdx.z * kc;
+    fs_f16v acc;
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[j] = 0.0f;
+    const __bf16 *PA = reinterpret_cast<const __bf16 *>(p.A), *PB = reinterpret_cast<const __bf16 *>(p.B);
+    const __bf16 *PH = reinterpret_cast<const __bf16 *>(p.H);
+    const long am = (long)(m0 + c32) * p.lda_m, bn = (long)(n0 + c32) * p.ldb_n;
+    for (int k = kb; k < kb + kc; k += 16) {
+        const int kk = k + 8 * g;
+        fs_bf8 A, B;
+        if (AK) {
+            A = *reinterpret_cast<const fs_bf8 *>(PA + am + kk);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) A[e] = PA[am + (long)(kk + e) * p.lda_k];
+        }
+        if (BK) {
+            B = *reinterpret_cast<const fs_bf8 *>(PB + bn + kk);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) B[e] = PB[bn + (long)(kk + e) * p.ldb_k];
+        }
+        acc = fs_lp_mfma(A, B, acc);
+    }
+    const int n = n0 + c32;
+    const float bias = p.bias ? p.bias[n] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+        const int m = m0 + 8 * (j >> 2) + 4 * g + (j & 3);
+        const long ci = (long)m * p.ldc + n;
+        if (p.mode == 0) {
+            reinterpret_cast<float *>(p.C)[(long)blockIdx.z * p.M * p.N + ci] = acc[j] + bias;
+        } else if (p.mode == 1) {
+            reinterpret_cast<__bf16 *>(p.C)[ci] = (__bf16)fmaxf(acc[j] + bias, 0.0f);
+        } else {
+            const bool on = (float)PH[ci] > 0.0f;
+            reinterpret_cast<__bf16 *>(p.C)[ci] = (__bf16)(on ? acc[j] : 0.0f);
+        }
+    }
+}
+
+// Loss and output gradient of row b: O [B][32] FP32 logical outputs -> dO [B][32] BF16.
+template <int D> FS_DEV float fs_lp_loss_row(const float *O, const float *T, __bf16 *dO, int b, float invB)
+{
+    const float *o = O + (long)b * 32, *t = T + (long)b * (3 * D + 1);
+    const float sig = t[3 * D];
+    float loss = 0.0f;
+    __bf16 *g = dO + (long)b * 32;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const float mid = 0.5f * (t[d] + t[D + d]);
+        const float mu = mid + sig * o[d];
+        const float sraw = o[D + d];
+        const float sv = fminf(fmaxf(sraw, -FS_LP_SMAX), FS_LP_SMAX);
+        const float z = (t[2 * D + d] - mu) / (sig * expf(sv));
+        loss += 0.5f * z * z + sv;
+        g[d] = (__bf16)(-z * expf(-sv) * invB);
+        g[D + d] = (__bf16)((sraw > -FS_LP_SMAX && sraw < FS_LP_SMAX) ? (1.0f - z * z) * invB : 0.0f);
+    }
+#pragma unroll
+    for (int i = 2 * D; i < 32; i++) g[i] = (__bf16)0.0f;
+    return loss;
+}
+
+// Adam on one logical parameter.  grad = sum of the weight-gradient partials
+// (or the bias column sum); step counts from 1.
+
+
+template <int D> FS_DEV void fs_lp_adam_body(const fs_lp_adam_args &a, int i)
+{
+    typedef fs_lp_params<D> Q;
+    if (i >= Q::COUNT) return;
+    const float *src = nullptr;
+    long stride = 0, idx = 0;
+    float grad = 0.0f;
+    if (i < Q::B1) { src = a.dW1; stride = 128 * 32; idx = (i / Q::NIN) * 32 + i % Q::NIN; }
+    else if (i < Q::W2) { grad = a.db[i - Q::B1]; }
+    else if (i < Q::B2) { src = a.dW2; stride = 128 * 128; idx = i - Q::W2; }
+    else if (i < Q::W3) { grad = a.db[128 + i - Q::B2]; }
+    else if (i < Q::B3) { src = a.dW3; stride = 128 * 128; idx = i - Q::W3; }
+    else if (i < Q::W4) { grad = a.db[256 + i - Q::B3]; }
+    else if (i < Q::B4) { src = a.dW4; stride = 32 * 128; idx = i - Q::W4; }
+    else { grad = a.db[384 + i - Q::B4]; }
+    if (src)
+        for (int z = 0; z < a.splits; z++) grad += src[z * stride + idx];
+    if (a.gradOut) a.gradOut[i] = grad;
+    if (!a.apply) return;
+    float m = a.m[i] * a.b1 + (1.0f - a.b1) * grad;
+    float v = a.v[i] * a.b2 + (1.0f - a.b2) * grad * grad;
+    a.m[i] = m;
+    a.v[i] = v;
+    a.P[i] -= a.lr * (m / a.c1) / (sqrtf(v / a.c2) + a.aeps);
+}
+
+// ---- kernels of one module (instantiated for the trait `sys`) ----
+#ifndef FS_LP_NO_KERNELS
+extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp_args a)
+{
+    typedef fs_lp_geom<sys> G;
+    extern __shared__ __attribute__((aligned(16))) char fs_lp_smem[];
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.img);
+        uint4 *dst = reinterpret_cast<uint4 *>(fs_lp_smem);
+        for (int i = threadIdx.x; i < FS_LP_IMG_BYTES / 16; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float *stage = reinterpret_cast<float *>(fs_lp_smem + FS_LP_IMG_BYTES + wave * G::WAVE_BYTES);
+    for (long long u = (long long)blockIdx.x * FS_LP_WAVES + wave; u < a.nUnits;
+         u += (long long)gridDim.x * FS_LP_WAVES)
+        fs_lp_unit<sys>(a, fs_lp_smem, stage, (long)(u / G::UPW), (int)(u % G::UPW) * G::SPW, lane);
+}
+
+// The network alone on given feature rows (FP32 [rows][32]) -> logical outputs [rows][32].
+extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_eval(const char *img_g, const float *X,
+                                                                          float *O, int rows)
+{
+    extern __shared__ __attribute__((aligned(16))) char fs_lp_smem[];
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(img_g);
+        uint4 *dst = reinterpret_cast<uint4 *>(fs_lp_smem);
+        for (int i = threadIdx.x; i < FS_LP_IMG_BYTES / 16; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 5, c32 = lane & 31;
+#pragma unroll 1
+    for (int t0 = (blockIdx.x * FS_LP_WAVES + wave) * 64; t0 < rows; t0 += gridDim.x * FS_LP_WAVES * 64) {
+        fs_bf8 xin[2][2];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int R = t0 + cb * 32 + c32;
+            float f[32];
+#pragma unroll
+            for (int i = 0; i < 32; i++) f[i] = R < rows ? X[(long)R * 32 + i] : 0.0f;
+            fs_lp_xin(f, g, xin[0][cb], xin[1][cb]);
+        }
+        fs_f16v out[2];
+        fs_lp_mlp(fs_lp_smem, xin, out, lane);
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int R = t0 + cb * 32 + c32;
+            if (R < rows) {
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    const int row = fs_lp_out_row<sys::DOF>(8 * (j >> 2) + 4 * g + (j & 3));
+                    if (row >= 0) O[(long)R * 32 + row] = out[cb][j];
+                }
+            }
+        }
+    }
+}
+
+extern "C" __global__ __launch_bounds__(256) void fs_lp_pack(const float *P, char *img, __bf16 *tw, float *tb)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    fs_lp_pack_body<sys::DOF>(P, img, tw, tb, i);
+}
+
+extern "C" __global__ __launch_bounds__(256) void fs_lp_gather(fs_lp_train_args a)
+{
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    if (b < a.batch) fs_lp_gather_row<sys>(a, b);
+}
+
+extern "C" __global__ __launch_bounds__(64) void fs_lp_gemm_kk(fs_lp_gemm_args p) { fs_lp_gemm_body<true, true>(p); }
+extern "C" __global__ __launch_bounds__(64) void fs_lp_gemm_ks(fs_lp_gemm_args p) { fs_lp_gemm_body<true, false>(p); }
+extern "C" __global__ __launch_bounds__(64) void fs_lp_gemm_sk(fs_lp_gemm_args p) { fs_lp_gemm_body<false, true>(p); }
+extern "C" __global__ __launch_bounds__(64) void fs_lp_gemm_ss(fs_lp_gemm_args p) { fs_lp_gemm_body<false, false>(p); }
+
+// loss partial per block (256 rows) -> lossPart[blockIdx.x]
+extern "C" __global__ __launch_bounds__(256) void fs_lp_loss(const float *O, const float *T, __bf16 *dO,
+                                                            float *lossPart, int batch)
+{
+    __shared__ float red[256];
+    const int b = blockIdx.x * 256 + threadIdx.x;
+    const float l = b < batch ? fs_lp_loss_row<sys::DOF>(O, T, dO, b, 1.0f / (float)batch) : 0.0f;
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lossPart[blockIdx.x] = red[0];
+}
+
+// Column sums of the layer gradients (bias gradients): column c of
+// dO (c < 32 -> db4), dA3 (db3), dA2 (db2), dA1 (db1); one block per column,
+// written as db [b1 128][b2 128][b3 128][b4 32].
+extern "C" __global__ __launch_bounds__(256) void fs_lp_colsum(const __bf16 *dA1, const __bf16 *dA2,
+                                                              const __bf16 *dA3, const __bf16 *dO, float *db,
+                                                              int batch)
+{
+    __shared__ float red[256];
+    const int c = blockIdx.x;  // 0 .. 415
+    const __bf16 *src;
+    int ld, col;
+    if (c < 128) { src = dA1; ld = 128; col = c; }
+    else if (c < 256) { src = dA2; ld = 128; col = c - 128; }
+    else if (c < 384) { src = dA3; ld = 128; col = c - 256; }
+    else { src = dO; ld = 32; col = c - 384; }
+    float s = 0.0f;
+    for (int b = threadIdx.x; b < batch; b += 256) s += (float)src[(long)b * ld + col];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[c] = red[0];
+}
+
+extern "C" __global__ __launch_bounds__(256) void fs_lp_adam(fs_lp_adam_args a)
+{
+    fs_lp_adam_body<sys::DOF>(a, blockIdx.x * 256 + threadIdx.x);
+}
+#endif

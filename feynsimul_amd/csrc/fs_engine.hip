@@ -8,9 +8,11 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -241,6 +243,9 @@ std::vector<std::string> compile_options(const fs_config *c, const std::string &
         opts.push_back("-Xclang");
         opts.push_back("-cl-single-precision-constant");
     }
+    // FS_RESOURCE_USAGE=1: registers, spills, LDS and occupancy of every kernel in the build log
+    if (const char *ru = getenv("FS_RESOURCE_USAGE"); ru && ru[0] == '1')
+        opts.push_back("-Rpass-analysis=kernel-resource-usage");
     if (c->fast_math) {
         opts.push_back("-freciprocal-math");
         opts.push_back("-fno-signed-zeros");
@@ -897,3 +902,5 @@ int fs_ranlux_draw(void *tab, void *out_f32, void *out_f64, int64_t n, int32_t n
 }
 
 }  // extern "C"
+
+#include "fs_lp_host.inc"

@@ -391,7 +391,18 @@ class PIMCKernel:
         """Runs operatorRuns x metroStepsPerOperatorRun steps (kernel.py:459-506)."""
         if self._engine is None:
             raise Exception("this kernel is a member of a BatchedPIMC; run the batch")
-        opRuns, M = int(self._operatorRuns), int(self._metroStepsPerOperatorRun)
+        self._launch(int(self._operatorRuns), int(self._metroStepsPerOperatorRun))
+        if wait:
+            startTime = time.time()
+            try:
+                self._stream.synchronize()
+            except RuntimeError:
+                if time.time() - startTime > 5.0:
+                    raise Exception("Kernel runtime error. Over 5 seconds had passed when kernel aborted.")
+                raise
+
+    def _launch(self, opRuns, M):
+        """One engine launch of opRuns measurements, each after M sweeps (M = 0: measure only)."""
         offset = 0
         if self._series is not None:
             if self._seriesCount + opRuns > self._series.shape[0]:
@@ -402,21 +413,7 @@ class PIMCKernel:
         self._ev0.record(self._stream)
         self._engine.run(bufs, opRuns, M, offset, self._stream.cuda_stream)
         self._ev1.record(self._stream)
-        self._ranRuns += 1
-        self._corRuns += opRuns
-        self._binRuns += opRuns
-        self._obsRuns += opRuns
-        if self._series is not None:
-            self._seriesCount += opRuns
-        self._lastRunLength = (opRuns, M)
-        if wait:
-            startTime = time.time()
-            try:
-                self._stream.synchronize()
-            except RuntimeError:
-                if time.time() - startTime > 5.0:
-                    raise Exception("Kernel runtime error. Over 5 seconds had passed when kernel aborted.")
-                raise
+        self._account_run(opRuns, M)
 
     def initRanlux(self):
         """Seeds every work-item's RANLUX state (kernel.py:508-530), one distinct seed each."""

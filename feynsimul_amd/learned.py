@@ -1,0 +1,194 @@
+"""Learned bisection proposal: a conditional MLP on BF16 MFMA, MH-corrected.
+
+No counterpart in the reference.  It replaces the Levy midpoint draw of
+FeynSimul/GPUsrc/kernel.c:338-360 by a Gaussian whose mean and log-width come
+from an MLP (2 DOF + 1 -> 128 -> 128 -> 128 -> 2 DOF) fed with a node's two
+bisection ends and its level, and extends the accept test of kernel.c:376-377
+with the exact ratio of proposal densities, so any weights keep detailed
+balance:
+
+  log alpha = -(K_new - K_old) - eps (V_new - V_old) + log q(old) - log q(new).
+
+With the output layer at zero the proposal is the Levy bridge (log alpha =
+-eps dV, the reference's rule); the network starts there.  Training is online
+maximum likelihood of midpoints of the current paths given their ends
+(Adam on FP32 master weights).  ``run`` interleaves learned sweeps with the
+kernel's measurements and falls back to plain bisection sweeps for a block
+whenever the learned acceptance drops below ``fallbackAcceptance``.
+
+All arithmetic runs in libfeynsimul.so (fs_lp_* entry points); torch only
+holds the device buffers.
+"""
+import numpy as np
+
+from . import _native
+
+
+def _he_init(dof, seed, hidden=128):
+    """He-normal hidden layers, zero biases, zero output layer (the Levy bridge)."""
+    rng = np.random.default_rng(seed)
+    nin, nout = 2 * dof + 1, 2 * dof
+    parts = [rng.standard_normal((hidden, nin)) * np.sqrt(2.0 / nin), np.zeros(hidden),
+             rng.standard_normal((hidden, hidden)) * np.sqrt(2.0 / hidden), np.zeros(hidden),
+             rng.standard_normal((hidden, hidden)) * np.sqrt(2.0 / hidden), np.zeros(hidden),
+             np.zeros((nout, hidden)), np.zeros(nout)]
+    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+
+
+class LearnedProposal:
+    """Learned bisection sweeps on the paths of one PIMCKernel.
+
+    kernel:   a PIMCKernel (FP32, any sampler; its paths are updated in place)
+    batch:    training rows (segment midpoints) per Adam step, multiple of 2048
+    lr:       Adam learning rate (beta1 0.9, beta2 0.999, eps 1e-8)
+    seed:     Philox key of the proposal's random numbers (default: the kernel's seed + 1)
+    atoms3d:  centre the inputs on the atoms' mean (default: system.translationInvariant)
+    inScale:  scale of the input coordinates
+    """
+
+    def __init__(self, kernel, batch=65536, lr=1e-3, seed=None, atoms3d=None, inScale=1.0, segsPerWave=0,
+                 initSeed=0, fallbackAcceptance=0.2):
+        if kernel._enableDouble:
+            raise Exception("the learned proposal runs on FP32 paths")
+        if kernel._engine is None:
+            raise Exception("the learned proposal needs a stand-alone PIMCKernel")
+        self._k = kernel
+        t = kernel._torch
+        system = kernel._system
+        self._dof = int(system.DOF)
+        if atoms3d is None:
+            atoms3d = bool(getattr(system, "translationInvariant", False)) and self._dof % 3 == 0
+        if seed is None:
+            seed = (kernel._seed + 1) & ((1 << 64) - 1)
+        lp = _native.FsLpConfig()
+        lp.abi_version = _native.FS_ABI_VERSION
+        lp.atoms3d = int(bool(atoms3d))
+        lp.batch = int(batch)
+        lp.segs_per_wave = int(segsPerWave)
+        lp.in_scale = float(inScale)
+        lp.lr, lp.beta1, lp.beta2, lp.adam_eps = float(lr), 0.9, 0.999, 1e-8
+        lp.seed = int(seed) & ((1 << 64) - 1)
+        self._P = _native.Proposal(kernel._cfg, kernel._systemSource, lp)
+        info = self._P.info
+        z = lambda n, dt: t.zeros((int(n),), dtype=dt, device=kernel._device)
+        self._params = z(info.n_params, t.float32)
+        self._m = z(info.n_params, t.float32)
+        self._v = z(info.n_params, t.float32)
+        self._image = z(info.image_bytes, t.uint8)
+        self._tw = z(info.train_bf16, t.int16)
+        self._tb = z(info.train_bias, t.float32)
+        self._ws = z(info.workspace_bytes, t.uint8)
+        self._loss = z(info.loss_parts, t.float32)
+        self._acc = z(kernel._nbrOfWalkers, t.int32)
+        self._trace = None
+        self._batch = int(batch)
+        self._step = 0
+        self._sweepCount = 0
+        self.fallbackAcceptance = float(fallbackAcceptance)
+        self.lastAcceptance = None
+        self.fallbackBlocks = 0
+        self.setParameters(_he_init(self._dof, initSeed))
+
+    # ---- buffers and parameters ----
+    def _bufs(self):
+        b = _native.FsLpBuffers()
+        b.params, b.adam_m, b.adam_v = self._params.data_ptr(), self._m.data_ptr(), self._v.data_ptr()
+        b.image, b.train_w, b.train_b = self._image.data_ptr(), self._tw.data_ptr(), self._tb.data_ptr()
+        b.workspace, b.loss_parts = self._ws.data_ptr(), self._loss.data_ptr()
+        b.trace = self._trace.data_ptr() if self._trace is not None else None
+        return b
+
+    @property
+    def info(self):
+        return self._P.info
+
+    def parameterCount(self):
+        return int(self._P.info.n_params)
+
+    def getParameters(self):
+        return self._params.cpu().numpy()
+
+    def setParameters(self, theta, resetAdam=True):
+        """FP32 parameters in logical order W1 [128][2D+1], b1, W2, b2, W3, b3, W4 [2D][128], b4."""
+        t = self._k._torch
+        theta = np.ascontiguousarray(theta, dtype=np.float32).reshape(-1)
+        if theta.size != self.parameterCount():
+            raise ValueError("expected %d parameters, got %d" % (self.parameterCount(), theta.size))
+        self._params.copy_(t.from_numpy(theta))
+        if resetAdam:
+            self._m.zero_()
+            self._v.zero_()
+            self._step = 0
+        self._P.pack(self._bufs(), self._k._stream.cuda_stream)
+        self._k._stream.synchronize()
+
+    # ---- sampling ----
+    def sweep(self, nSweeps=1, wait=True):
+        """nSweeps learned sweeps; returns their acceptance rate (segments accepted / proposed)."""
+        k = self._k
+        self._acc.zero_()
+        self._P.sweep(k._paths.data_ptr(), self._acc.data_ptr(), self._bufs(), self._sweepCount, int(nSweeps),
+                      k._stream.cuda_stream)
+        self._sweepCount += int(nSweeps)
+        if not wait:
+            return None
+        segs = k._nbrOfWalkers * (k._N >> k._S) * int(nSweeps)
+        self.lastAcceptance = float(self._acc.sum().item()) / segs
+        return self.lastAcceptance
+
+    def run(self, operatorRuns=None, sweepsPerMeasurement=None):
+        """operatorRuns blocks of sweepsPerMeasurement learned sweeps, each followed by one
+        measurement of the kernel's operators (series, totals, correlator, histograms).  A block
+        whose learned acceptance falls below fallbackAcceptance is followed by plain bisection
+        sweeps of the same length (with the measurement)."""
+        k = self._k
+        opRuns = int(operatorRuns if operatorRuns is not None else k._operatorRuns)
+        M = int(sweepsPerMeasurement if sweepsPerMeasurement is not None else k._metroStepsPerOperatorRun)
+        for _ in range(opRuns):
+            ar = self.sweep(M)
+            if ar < self.fallbackAcceptance:
+                self.fallbackBlocks += 1
+                k._launch(1, M)
+            else:
+                k._launch(1, 0)
+        k._stream.synchronize()
+
+    # ---- training ----
+    def train(self, steps=1):
+        """Adam steps on rows drawn from the current paths; returns the mean NLL of the last step."""
+        k = self._k
+        for _ in range(int(steps)):
+            self._step += 1
+            self._P.train(k._paths.data_ptr(), self._bufs(), self._step, k._stream.cuda_stream)
+        k._stream.synchronize()
+        return float(self._loss.double().sum().item()) / self._batch
+
+    def traceSweeps(self, on=True):
+        """Keep (log q ratio, dK, dV, log alpha) of every segment of the last sweep (test support)."""
+        k = self._k
+        self._trace = (k._torch.zeros((k._nbrOfWalkers, k._N >> k._S, 4), dtype=k._torch.float32,
+                                      device=k._device) if on else None)
+
+    def getTrace(self):
+        return self._trace.cpu().numpy()
+
+    def lastGradient(self):
+        """Gradient of the mean NLL used by the last Adam step, logical order (test support)."""
+        t = self._k._torch
+        g = t.zeros_like(self._params)
+        self._P.grad(self._bufs(), g.data_ptr(), self._k._stream.cuda_stream)
+        self._k._stream.synchronize()
+        return g.cpu().numpy()
+
+    def evalNetwork(self, X):
+        """Network outputs (rows, 2 DOF) for features X (rows, 2 DOF + 1) (test support)."""
+        t = self._k._torch
+        X = np.asarray(X, dtype=np.float32)
+        rows = X.shape[0]
+        Xp = np.zeros((rows, 32), np.float32)
+        Xp[:, : X.shape[1]] = X
+        Xd = t.from_numpy(Xp).to(self._k._device)
+        Od = t.zeros((rows, 32), dtype=t.float32, device=self._k._device)
+        self._P.eval(self._bufs(), Xd.data_ptr(), Od.data_ptr(), rows, self._k._stream.cuda_stream)
+        self._k._stream.synchronize()
+        return Od.cpu().numpy()[:, : 2 * self._dof]

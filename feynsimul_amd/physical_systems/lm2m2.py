@@ -118,6 +118,7 @@ class Lm2m2Cluster:
             raise Exception("n must be >=2")
         self.nAtoms = n = int(nAtoms)
         self.DOF = 3 * n
+        self.translationInvariant = True   # V depends on relative positions only
         self.mass = mass
         self.massScale = mass_scale(mass)             # x = massScale * r
         self.rUnit = 1e-10                             # coordinates before scaling are angstrom

@@ -174,6 +174,67 @@ int fs_batch_capture(fs_batch *b, const fs_buffers *bufs, const int32_t *order, 
 int fs_graph_launch(fs_graph *g, void *stream);
 int fs_graph_destroy(fs_graph *g);
 
+/* ---- learned bisection proposal (BF16 MFMA MLP, MH-corrected) ----
+ * No reference counterpart: it replaces the Levy midpoint draw of
+ * kernel.c:338-360 with a conditional Gaussian from an MLP
+ * (2 DOF + 1 -> 128 -> 128 -> 128 -> 2 DOF) and extends the accept test of
+ * kernel.c:376-377 with the proposal density ratio, so any weights keep
+ * detailed balance.  FP32 paths only, 1 <= S <= 6, DOF <= 15.  Random numbers
+ * come from Philox4x32-10 keyed by `seed`.  The caller allocates the buffers
+ * with the sizes of fs_lp_info. */
+typedef struct fs_lp fs_lp;
+typedef struct fs_lp_config {
+    int32_t abi_version;      /* FS_ABI_VERSION */
+    int32_t atoms3d;          /* inputs centred on the atoms' mean (DOF = 3 x atoms) */
+    int32_t batch;            /* training rows per Adam step, a multiple of 2048 */
+    int32_t segs_per_wave;    /* segments per wavefront in a sweep (0 = automatic) */
+    float in_scale;           /* input scale of the coordinates */
+    float lr, beta1, beta2, adam_eps;
+    uint64_t seed;            /* Philox key */
+} fs_lp_config;
+typedef struct fs_lp_info {
+    int64_t n_params;         /* FP32 master parameters (and each Adam moment) */
+    int64_t image_bytes;      /* packed inference image */
+    int64_t train_bf16;       /* BF16 training matrices */
+    int64_t train_bias;       /* FP32 training biases */
+    int64_t workspace_bytes;  /* training workspace */
+    int64_t loss_parts;       /* FP32 partial losses of a training step (sum / batch = mean NLL) */
+    int32_t segs_per_wave;
+    int32_t waves_per_block;
+    int32_t lds_bytes;        /* dynamic LDS of a sweep block */
+    int32_t grid_blocks;      /* sweep blocks (persistent) */
+} fs_lp_info;
+typedef struct fs_lp_buffers {
+    void *params, *adam_m, *adam_v;  /* FP32 [n_params] */
+    void *image;                     /* [image_bytes] */
+    void *train_w;                   /* BF16 [train_bf16] */
+    void *train_b;                   /* FP32 [train_bias] */
+    void *workspace;                 /* [workspace_bytes] */
+    void *loss_parts;                /* FP32 [loss_parts] */
+    void *trace;                     /* optional FP32 [walkers][N / 2^S][4] of the last sweep:
+                                        log q(old) - log q(new), dK, dV, log alpha (test support) */
+} fs_lp_buffers;
+/* Compile the proposal module for one configuration (wave-sampler system
+ * source, FP32). */
+int fs_lp_create(const fs_config *cfg, const char *system_src, const fs_lp_config *lp, int compile_only,
+                 fs_lp **out);
+int fs_lp_destroy(fs_lp *p);
+int fs_lp_get_info(const fs_lp *p, fs_lp_info *info);
+int fs_lp_build_log(const fs_lp *p, char *buf, size_t len, size_t *needed);
+/* Rebuild the BF16 inference image and training matrices from params. */
+int fs_lp_pack(fs_lp *p, const fs_lp_buffers *b, void *stream);
+/* n_sweeps learned sweeps (sweep counters sweep0 ...) over paths [walkers][DOF][N];
+ * accepted segments are added to accepts[walkers]. */
+int fs_lp_sweep(fs_lp *p, void *paths, void *accepts, const fs_lp_buffers *b, uint64_t sweep0, int32_t n_sweeps,
+                void *stream);
+/* One Adam step (step counts from 1) on `batch` rows drawn from paths, then a repack. */
+int fs_lp_train(fs_lp *p, const void *paths, const fs_lp_buffers *b, uint64_t step, void *stream);
+/* The network alone: X [rows][32] FP32 features -> O [rows][32] FP32 (logical outputs 0 .. 2 DOF - 1). */
+int fs_lp_eval(fs_lp *p, const fs_lp_buffers *b, const void *X, void *O, int64_t rows, void *stream);
+/* Weight gradients of the last fs_lp_train in logical parameter order, FP32 [n_params]
+ * (a device buffer); test support. */
+int fs_lp_grad(fs_lp *p, const fs_lp_buffers *b, void *grad, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -173,3 +173,52 @@ def test_device_entry_points_need_a_gpu():
     assert _native.device_count() == 0
     rc = _native.lib().fs_seed_xorshift(8, 1, 0, 0, None)  # any non-null pointer; rejected before use
     assert rc == _native.FS_ERR_NO_DEVICE
+
+
+def _lp(**kw):
+    from feynsimul_amd import _native
+    lp = _native.FsLpConfig()
+    lp.abi_version, lp.atoms3d, lp.batch, lp.in_scale = _native.FS_ABI_VERSION, 0, 65536, 1.0
+    lp.lr, lp.beta1, lp.beta2, lp.adam_eps, lp.seed = 1e-3, 0.9, 0.999, 1e-8, 1
+    for k, v in kw.items():
+        setattr(lp, k, v)
+    return lp
+
+
+@pytest.mark.parametrize("name,N,S,spw", [("ho", 256, 4, 16), ("aho", 4096, 4, 32), ("lm2m2-3", 1024, 6, 8)])
+def test_learned_module_compiles_for_gfx950(name, N, S, spw):
+    """The learned-proposal module (MFMA sweep, training GEMMs, Adam) builds for each system;
+    segments per wavefront follow the 160 KiB LDS budget (82 KiB network image + staging)."""
+    from feynsimul_amd import _native
+    system = {"ho": HarmOsc(), "aho": AnharmOsc(0.1), "lm2m2-3": Lm2m2Cluster(3)}[name]
+    c = _cfg(system, N, S, 1, path_storage=1)
+    p = _native.Proposal(c, system_section(system, (system.energyOp,)),
+                         _lp(atoms3d=int(name.startswith("lm2m2"))), compile_only=True)
+    info = p.info
+    assert info.segs_per_wave == spw and info.waves_per_block == 4
+    assert info.lds_bytes <= 163840
+    assert info.n_params == 128 * (2 * system.DOF + 1) + 128 + 2 * (128 * 128 + 128) + 2 * system.DOF * 129
+    if name == "lm2m2-3":
+        assert info.n_params == 37906
+    p.close()
+
+
+@pytest.mark.parametrize("field,value,msg", [
+    ("batch", 1000, "multiple of 2048"),
+    ("atoms3d", 1, "atoms3d needs DOF"),
+    ("abi_version", 7, "ABI version"),
+])
+def test_learned_config_rejected(field, value, msg):
+    from feynsimul_amd import _native
+    with pytest.raises(_native.FsError, match=re.escape(msg)):
+        _native.Proposal(_cfg(HarmOsc(), 64, 3, 1), system_section(HarmOsc(), ("x1",)), _lp(**{field: value}),
+                         compile_only=True)
+
+
+def test_learned_rejects_double_and_deep_levels():
+    from feynsimul_amd import _native
+    with pytest.raises(_native.FsError, match="FP32"):
+        _native.Proposal(_cfg(HarmOsc(), 64, 3, 1, double_precision=1), system_section(HarmOsc(), ("x1",)), _lp(),
+                         compile_only=True)
+    with pytest.raises(_native.FsError, match="S <= 6"):
+        _native.Proposal(_cfg(HarmOsc(), 256, 7, 1), system_section(HarmOsc(), ("x1",)), _lp(), compile_only=True)
