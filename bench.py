@@ -7,7 +7,12 @@ distance, and a 256-bin pair-distance histogram.  One step = one launch of
 32 sweeps over every chain followed by one measurement.  value = slice-updates/s (one slice-update = one interior
 slice of a segment, all 9 coordinates, proposed by one bisection move).
 
+After the timed steps: the ESS/s of E0 of the plain sampler and, from its
+equilibrated paths, of the learned proposal (online training included), on
+equal wall-clock budgets.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--storage registers|memory]
+                       [--no-ess] [--no-learned] [--learned-train 200] [--learned-measurements 64]
        python bench.py --copy-calibration   (1 GiB device copy, to scale rocprofv3 HBM counters)
 """
 import argparse
@@ -45,6 +50,12 @@ def measured_traffic():
     except OSError:
         return None
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s, one FP32 op per lane per clock
+MFMA_BF16_PEAK = 2.5e15       # dense BF16 MFMA FLOP/s
+# learned proposal: one network evaluation = 2 (19*128 + 128*128*2 + 128*18) = 75,008 FLOP; a
+# segment evaluates 2^S - 1 proposals and 2^S - 2 reverse rows (level 1 shares its row)
+LP_FLOP_PER_ROW = 2 * (19 * 128 + 2 * 128 * 128 + 128 * 18)
+LP_ROWS_PER_SEGMENT = 2 * (SEG - 1) - 1
+LP_FLOP_PER_UPDATE = LP_FLOP_PER_ROW * LP_ROWS_PER_SEGMENT / (SEG - 1)
 
 
 def make_kernel(steps_per_launch, seed, storage=None, operator_runs=1):
@@ -71,6 +82,16 @@ def initial_paths(system, rng, walkers=W, sigma=3.0):
     return np.repeat(pos[:, :, None], N, axis=2).reshape(walkers, DOF * N)
 
 
+def chain_mean_ess(series):
+    """ESS = Var_1 / SE^2: single-measurement variance over the variance of the grand mean,
+    the latter from the spread of the independent chains' means (the reference's error
+    model, examples/lm2m2_plotter.py:100-117)."""
+    import numpy as np
+    x = np.asarray(series, dtype=np.float64)
+    se2 = x.mean(axis=0).var(ddof=1) / x.shape[1]
+    return float(x.var() / se2), float(np.sqrt(se2))
+
+
 def ess_phase(paths, storage, burn_launches, meas_launches, runs_per_launch=16):
     """ESS/s of the virial energy E0: series of per-chain slice means, one every
     32 sweeps, after burn-in; tau_int by Sokal windowing over all chains."""
@@ -81,20 +102,81 @@ def ess_phase(paths, storage, burn_launches, meas_launches, runs_per_launch=16):
     for _ in range(burn_launches):
         k.run(wait=False)
     k.setSeriesCapacity(meas_launches * runs_per_launch)
+    ess_phase.start_paths = k.getPaths()  # the learned phase starts from the same state
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(meas_launches):
         k.run(wait=False)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    r = effective_sample_size(k.getSeries()[:, :, 0])
+    series = k.getSeries()[:, :, 0]
+    r = effective_sample_size(series)
+    ess_cm, se_cm = chain_mean_ess(series)
     sweeps = meas_launches * runs_per_launch * SWEEPS_PER_STEP
     return {"metric": "ESS/s of E0 (virial energy), plain bisection sampler", "value": r["ess"] / dt,
+            "ess_chain_means": ess_cm, "ess_chain_means_per_s": ess_cm / dt, "se_chain_means_K": se_cm,
             "unit": "effective samples/s", "E0_K": r["mean"], "sigma_K": r["se"],
             "tau_int_measurements": r["tau_int"], "tau_int_sweeps": r["tau_int"] * SWEEPS_PER_STEP,
             "ess": r["ess"], "seconds": dt, "sweeps": sweeps,
             "burn_in_sweeps": burn_launches * runs_per_launch * SWEEPS_PER_STEP,
             "seconds_to_sigma_1e-3_K": dt * (r["se"] / 1e-3) ** 2}
+
+
+def learned_phase(paths, train_steps, measurements, plain_ess):
+    """The learned proposal on the same workload, started from the paths the plain
+    ESS phase starts from (after its burn-in), on a similar wall-clock budget: online training (Adam, 65,536 rows a step, each step after 2
+    plain sweeps that produce its rows), then learned sweeps with one measurement every
+    32; ESS/s counts the training time.  Also the learned sweep's throughput against
+    the BF16 MFMA peak."""
+    import numpy as np
+    import torch
+    from feynsimul_amd.learned import LearnedProposal
+    from feynsimul_amd.stats import effective_sample_size
+    _, k = make_kernel(SWEEPS_PER_STEP, seed=778, storage="memory")
+    k.setPaths(paths)
+    lp = LearnedProposal(k, batch=65536, lr=1e-3, seed=779)
+    # learned sweep throughput and the untrained (Levy) acceptance
+    lp.sweep(2)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(k._stream)
+    lp.sweep(8, wait=False)
+    ev1.record(k._stream)
+    torch.cuda.synchronize()
+    ms_sweep = ev0.elapsed_time(ev1) / 8
+    ar0 = lp.lastAcceptanceOfQueued(8)
+    # online training on paths from the plain sampler
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    first = lp.train(1)
+    last = lp.trainOnline(train_steps - 1, plainSweepsPerStep=2)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    # learned sampling with measurements
+    k.setSeriesCapacity(measurements)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    lp.run(measurements, SWEEPS_PER_STEP)
+    t_run = time.perf_counter() - t1
+    series = k.getSeries()[:, :, 0]
+    r = effective_sample_size(series)
+    ess_cm, se_cm = chain_mean_ess(series)
+    dt = t_train + t_run
+    flop = LP_FLOP_PER_UPDATE * UPDATES_PER_SWEEP
+    out = {"metric": "ESS/s of E0 (virial energy), learned proposal (training time included)",
+           "value": ess_cm / dt, "unit": "effective samples/s (chain-mean estimator)",
+           "ess_chain_means": ess_cm, "se_chain_means_K": se_cm, "ess_sokal": r["ess"],
+           "tau_int_measurements": r["tau_int"], "E0_K": r["mean"],
+           "seconds_train": t_train, "seconds_sampling": t_run, "adam_steps": train_steps,
+           "nll_first": first, "nll_last": last, "acceptance_untrained": ar0,
+           "acceptance_trained": lp.lastAcceptance, "fallback_blocks": lp.fallbackBlocks,
+           "sweeps": measurements * SWEEPS_PER_STEP, "ms_per_learned_sweep": ms_sweep,
+           "slice_updates_per_s": UPDATES_PER_SWEEP / (ms_sweep * 1e-3),
+           "mfma": {"bound": "mfma", "kernel": "fs_lp_sweep", "achieved": flop / (ms_sweep * 1e-3),
+                    "peak": MFMA_BF16_PEAK, "unit": "FLOP/s", "frac": flop / (ms_sweep * 1e-3) / MFMA_BF16_PEAK,
+                    "flop_per_update": LP_FLOP_PER_UPDATE, "updates_per_launch": UPDATES_PER_SWEEP}}
+    if plain_ess:
+        out["ess_per_s_ratio_learned_over_plain"] = (ess_cm / dt) / plain_ess["ess_chain_means_per_s"]
+    return out
 
 
 def ragged_bench(steps, warmup):
@@ -181,6 +263,9 @@ def main():
                          "2048 chains each, in one autotuned persistent launch")
     ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the ESS phase")
     ap.add_argument("--ess-launches", type=int, default=64, help="ESS launches of 16 x 32 sweeps")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal phase")
+    ap.add_argument("--learned-train", type=int, default=200, help="online Adam steps before learned sampling")
+    ap.add_argument("--learned-measurements", type=int, default=64, help="learned measurements (32 sweeps each)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (5 times), to scale HBM counters")
     ap.add_argument("--storage", choices=["registers", "memory"], default=None,
@@ -255,6 +340,9 @@ def main():
     }
     if not args.no_ess:
         line["ess"] = ess_phase(k.getPaths(), args.storage, args.ess_burn, args.ess_launches)
+        if not args.no_learned:
+            line["learned"] = learned_phase(ess_phase.start_paths, args.learned_train, args.learned_measurements,
+                                            line["ess"])
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -136,6 +136,13 @@ class LearnedProposal:
         self.lastAcceptance = float(self._acc.sum().item()) / segs
         return self.lastAcceptance
 
+    def lastAcceptanceOfQueued(self, nSweeps):
+        """Acceptance rate of the last sweep(nSweeps, wait=False) call, after it has finished."""
+        k = self._k
+        k._stream.synchronize()
+        self.lastAcceptance = float(self._acc.sum().item()) / (k._nbrOfWalkers * (k._N >> k._S) * int(nSweeps))
+        return self.lastAcceptance
+
     def run(self, operatorRuns=None, sweepsPerMeasurement=None):
         """operatorRuns blocks of sweepsPerMeasurement learned sweeps, each followed by one
         measurement of the kernel's operators (series, totals, correlator, histograms).  A block
@@ -158,6 +165,17 @@ class LearnedProposal:
         """Adam steps on rows drawn from the current paths; returns the mean NLL of the last step."""
         k = self._k
         for _ in range(int(steps)):
+            self._step += 1
+            self._P.train(k._paths.data_ptr(), self._bufs(), self._step, k._stream.cuda_stream)
+        k._stream.synchronize()
+        return float(self._loss.double().sum().item()) / self._batch
+
+    def trainOnline(self, steps, plainSweepsPerStep=2):
+        """Online training: each Adam step follows plainSweepsPerStep plain bisection sweeps of
+        the kernel, whose paths supply its rows.  Returns the mean NLL of the last step."""
+        k = self._k
+        for _ in range(int(steps)):
+            k._launch(1, int(plainSweepsPerStep))
             self._step += 1
             self._P.train(k._paths.data_ptr(), self._bufs(), self._step, k._stream.cuda_stream)
         k._stream.synchronize()
