@@ -9,7 +9,7 @@ import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfeynsimul.so")
 
-FS_ABI_VERSION = 1
+FS_ABI_VERSION = 2
 FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
 FS_STORAGE_AUTO, FS_STORAGE_MEMORY, FS_STORAGE_REGISTERS = 0, 1, 2
@@ -33,7 +33,9 @@ class FsConfig(ctypes.Structure):
         ("bin_res", ctypes.c_int32), ("waves_per_block", ctypes.c_int32), ("beta", ctypes.c_double),
         ("x_min", ctypes.c_double), ("x_max", ctypes.c_double), ("op_norm", ctypes.c_double),
         ("path_storage", ctypes.c_int32), ("n_hist_obs", ctypes.c_int32), ("hist_bins", ctypes.c_int32),
-        ("hist_lo", ctypes.c_double), ("hist_hi", ctypes.c_double)]
+        ("hist_lo", ctypes.c_double), ("hist_hi", ctypes.c_double), ("no_bisection", ctypes.c_int32),
+        ("path_shift", ctypes.c_int32), ("single_node", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("alpha", ctypes.c_double), ("ps_alpha", ctypes.c_double)]
 
 
 class FsBuffers(ctypes.Structure):

@@ -124,6 +124,51 @@ FS_DEV void fs_exact_move(fs_f *__restrict__ path, fs_f *__restrict__ oldPath, f
     }
 }
 
+#if FS_PATH_SHIFT || FS_SINGLE_NODE
+// kinEnergyEst (kernel.c:245-249): half the squared step in units of 1/eps^2.
+FS_DEV fs_f fs_kin_est(fs_f a, fs_f b)
+{
+    const fs_f delta = a - b;
+    return 0.5 * (delta * delta) * (fs_f)FS_EPS_INV2;
+}
+#endif
+
+#if FS_PATH_SHIFT
+// shiftPathEnergyDiff (kernel.c:407-473): adds `offset` to coordinate `degree`
+// of slices left..right (wrapping past N - 1 when left > right); returns the
+// change of kinetic estimator plus potential, accumulated in the reference's order.
+FS_DEV fs_f fs_shift_diff(fs_f *__restrict__ path, int degree, fs_f offset, int left, int right)
+{
+    if (left == right) return (fs_f)0.0;
+    fs_f *xd = path + degree * FS_N;
+    fs_f kinDiff = (fs_f)0.0, potDiff = (fs_f)0.0;
+    int ll, rr;
+    if (left < right) {
+        ll = left == 0 ? FS_N - 1 : left - 1;
+        rr = right == FS_N - 1 ? 0 : right + 1;
+    } else {
+        ll = left - 1;
+        rr = right + 1;
+    }
+    kinDiff -= fs_kin_est(xd[left], xd[ll]);
+    kinDiff -= fs_kin_est(xd[right], xd[rr]);
+    const int count = left < right ? right - left + 1 : FS_N - left + right + 1;
+    for (int t = 0; t < count; t++) {
+        int i = left + t;
+        if (i >= FS_N) i -= FS_N;
+        fs_f x[FS_DOF];
+        fs_gather(path, i, x);
+        potDiff -= fs_potential_v(x);
+        xd[i] = xd[i] + offset;
+        fs_gather(path, i, x);
+        potDiff += fs_potential_v(x);
+    }
+    kinDiff += fs_kin_est(xd[left], xd[ll]);
+    kinDiff += fs_kin_est(xd[right], xd[rr]);
+    return kinDiff + potDiff;
+}
+#endif
+
 extern "C" __global__ void fs_exact_sweep(fs_kargs a)
 {
 #if FS_PARALLEL
@@ -164,12 +209,54 @@ extern "C" __global__ void fs_exact_sweep(fs_kargs a)
         __syncthreads();
 #endif
         for (int j = 0; j < a.metroSteps; j++) {
+            // per step: path shift, single node, bisection (kernel.c:640-721)
+#if FS_PATH_SHIFT
+            {
+                const int degree = (int)(((fs_u32)a.metroSteps * (fs_u32)i + (fs_u32)j) % (fs_u32)FS_DOF);
+                const int left = (int)grp.below((fs_u32)FS_N);
+                const int right = (int)grp.below((fs_u32)FS_N);
+                const fs_f offset = (fs_f)FS_PSALPHA * ((fs_f)2.0 * own.uniform() - (fs_f)1.0);
+                if (exp(-eps * fs_shift_diff(path, degree, offset, left, right)) < own.uniform())
+                    fs_shift_diff(path, degree, -offset, left, right);
+                else
+                    accepts++;
+            }
+#endif
+#if FS_SINGLE_NODE
+            {
+                const fs_u32 step = (fs_u32)i * (fs_u32)a.metroSteps + (fs_u32)j;
+                const int modPoint = (int)(step % (fs_u32)FS_N);
+                const int modPathPoint = (int)(step % (fs_u32)(FS_N * FS_DOF));
+                int rightIndex = modPathPoint + 1, leftIndex = modPathPoint - 1;
+                if (modPoint == 0)
+                    leftIndex += FS_N;
+                else if (modPoint == FS_N - 1)
+                    rightIndex -= FS_N;
+                const fs_f oldX = path[modPathPoint];
+                const fs_f modX = oldX + ((fs_f)2.0 * own.uniform() - (fs_f)1.0) * (fs_f)FS_ALPHA;
+                fs_f diffE = fs_kin_est(modX, path[rightIndex]) + fs_kin_est(modX, path[leftIndex]) -
+                             fs_kin_est(oldX, path[leftIndex]) - fs_kin_est(oldX, path[rightIndex]);
+                fs_f x[FS_DOF];
+                fs_gather(path, modPoint, x);
+                diffE -= fs_potential_v(x);
+                path[modPathPoint] = modX;
+                fs_gather(path, modPoint, x);
+                diffE += fs_potential_v(x);
+                path[modPathPoint] = oldX;
+                if (exp(-eps * diffE) > own.uniform()) {
+                    path[modPathPoint] = modX;
+                    accepts++;
+                }
+            }
+#endif
+#if FS_BISECTION
 #if FS_PARALLEL
             const fs_u32 start = (grp.below((fs_u32)FS_SEG) + (fs_u32)(FS_SEG * lid0)) % (fs_u32)FS_N;
 #else
             const fs_u32 start = grp.below((fs_u32)FS_N);
 #endif
             fs_exact_move(path, oldPath, start, own, accepts);
+#endif
 #if FS_PARALLEL
             __syncthreads();
 #endif

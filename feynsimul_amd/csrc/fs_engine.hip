@@ -137,6 +137,14 @@ int validate(const fs_config *c)
     }
     if (c->bins && (c->bin_res < 1 || !(c->x_max > c->x_min)))
         return fail(FS_ERR_INVALID, "histogram needs bin_res >= 1 and x_max > x_min");
+    if (c->path_shift || c->single_node) {
+        // kernel.py:193-198: these moves exist only with one work-item per walker
+        if (c->sampler != FS_SAMPLER_EXACT || c->parallelize_path)
+            return fail(FS_ERR_INVALID, "path-shift and single-node moves need the exact sampler without "
+                                        "parallelize_path");
+    }
+    if (c->no_bisection && c->sampler == FS_SAMPLER_WAVE)
+        return fail(FS_ERR_INVALID, "the wave sampler's move is the bisection");
     return FS_OK;
 }
 
@@ -284,6 +292,12 @@ std::string build_source(const fs_config *c, const char *system_src, int wpb)
     def("FS_HLO", lit(c->hist_lo));
     def("FS_HINV", lit(c->n_hist_obs > 0 ? (double)c->hist_bins / (c->hist_hi - c->hist_lo) : 1.0));
     defi("FS_RESIDENT", resident_storage(c) ? 1 : 0);
+    defi("FS_BISECTION", c->no_bisection ? 0 : 1);
+    defi("FS_PATH_SHIFT", c->path_shift ? 1 : 0);
+    defi("FS_SINGLE_NODE", c->single_node ? 1 : 0);
+    def("FS_ALPHA", lit(c->alpha));
+    def("FS_PSALPHA", lit(c->ps_alpha));
+    def("FS_EPS_INV2", lit(((double)c->N / c->beta) * ((double)c->N / c->beta)));  // kernel.py:299
     def("eps", lit(c->beta / (double)c->N));
     s += "#pragma clang force_cuda_host_device begin\n";
     s += system_src ? system_src : "";

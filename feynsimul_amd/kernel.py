@@ -183,10 +183,15 @@ class PIMCKernel:
             raise Exception("Old Path is not used when bisection is not enabled")
         if self._operatorRuns is None or self._metroStepsPerOperatorRun is None:
             raise TypeError("%d format: a real number is required, not NoneType")
-        if self._enableSingleNodeMove or self._enablePathShift:
-            raise NotImplementedError("single-node and path-shift moves are not available in this engine yet")
-        if not self._enableBisection:
-            raise NotImplementedError("a sampling move must be enabled (enableBisection)")
+        if self._S is None:
+            # without bisection S only sizes unused tables; ParallelizePath needs it (kernel.py:200)
+            if self._enableParallelizePath:
+                raise TypeError("unsupported operand type(s) for ** or pow(): 'int' and 'NoneType'")
+            self._S = 1
+        if self._enableSingleNodeMove:
+            self._alpha = float(self._alpha)
+        if self._enablePathShift:
+            self._PSAlpha = float(self._PSAlpha)
 
         # ---- geometry (kernel.py:200-211) ----
         T = self._N // (2 ** self._S)
@@ -202,10 +207,13 @@ class PIMCKernel:
 
         # ---- sampler choice ----
         sampler = getattr(ka, "sampler", None)
+        onlyBisection = self._enableBisection and not (self._enablePathShift or self._enableSingleNodeMove)
         if sampler is None:
-            sampler = "wave" if (self._enableParallelizePath and 2 ** self._S <= 64) else "exact"
+            sampler = "wave" if (onlyBisection and self._enableParallelizePath and 2 ** self._S <= 64) else "exact"
         if sampler not in ("wave", "exact"):
             raise ValueError("sampler must be 'wave' or 'exact'")
+        if sampler == "wave" and not onlyBisection:
+            raise Exception("the wave sampler runs bisection moves only; use sampler='exact'")
         if sampler == "wave" and 2 ** self._S > 64:
             raise Exception("the wave sampler needs 2^S <= 64")
         if sampler == "exact" and self._enableParallelizePath and T * self._nbrOfWalkersPerWorkGroup > 1024:
@@ -266,6 +274,11 @@ class PIMCKernel:
         if storage not in storages:
             raise ValueError("pathStorage must be 'registers', 'memory' or None")
         cfg.path_storage = storages[storage]
+        cfg.no_bisection = int(not self._enableBisection)
+        cfg.path_shift = int(bool(self._enablePathShift))
+        cfg.single_node = int(bool(self._enableSingleNodeMove))
+        cfg.alpha = self._alpha if self._enableSingleNodeMove else 0.0
+        cfg.ps_alpha = self._PSAlpha if self._enablePathShift else 0.0
         if self._histObs:
             cfg.n_hist_obs = len(self._histObs)
             cfg.hist_bins = self._histBins

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FS_ABI_VERSION 1
+#define FS_ABI_VERSION 2
 
 enum {
     FS_OK = 0,
@@ -80,6 +80,12 @@ typedef struct fs_config {
     int32_t n_hist_obs;         /* wave sampler: scalar observables binned into one histogram */
     int32_t hist_bins;          /* bins of that histogram */
     double hist_lo, hist_hi;    /* its range; values outside are not counted */
+    int32_t no_bisection;       /* !enableBisection */
+    int32_t path_shift;         /* enablePathShift (exact sampler, one item per walker) */
+    int32_t single_node;        /* enableSingleNodeMove (exact sampler, one item per walker) */
+    int32_t reserved;
+    double alpha;               /* single-node step width */
+    double ps_alpha;            /* path-shift offset width */
 } fs_config;
 
 /* Device buffers of one launch; unused entries may be NULL.  Layouts:

@@ -285,7 +285,8 @@ def learned_sweep(paths, p, V, *, S, eps, seed, sweep, atoms3d=False, in_scale=1
     dV = (np.sum(V(new[:, :, 1:SEG, :]), axis=2) - np.sum(V(old[:, :, 1:SEG, :]), axis=2)).astype(np.float32)
     logA = (-dK - np.float32(eps) * dV + lq).astype(np.float32)
     u = accept_uniforms(seed, wi[:, None], sweep, j[None, :])
-    acc = ~(np.exp(logA) < u)
+    with np.errstate(over="ignore"):
+        acc = ~(np.exp(logA) < u)
     for w in range(W):
         for k in np.nonzero(acc[w])[0]:
             paths[w][:, slices[w, k, 1:SEG]] = new[w, k, 1:SEG, :].T

@@ -139,7 +139,9 @@ class OracleCfg(ctypes.Structure):
         "operatorRuns", "metroSteps", "enableOperator", "nOps", "enableCorrelator", "nCorr",
         "corLags", "enableBins", "binRes", "localSize", "nThreads", "nHistObs", "histBins")] + [
         (n, ctypes.c_double) for n in ("epsilon", "opNorm", "ranluxIntScale", "xMin", "xMax", "invBinSize",
-                                       "histLo", "histInv")]
+                                       "histLo", "histInv")] + [
+        (n, ctypes.c_int32) for n in ("bisection", "pathShift", "singleNode", "pad_")] + [
+        (n, ctypes.c_double) for n in ("alpha", "psAlpha", "epsInv2")]
 
 
 RANLUX_DTYPE = np.dtype([("s", "<f4", (24,)), ("carry", "<f4"), ("dummy", "<f4"),
@@ -277,7 +279,8 @@ def nskip_of_lux(lux: int) -> int:
 def make_cfg(N, S, DOF, nWalkers, beta, operatorRuns, metroSteps, *, walkersPerGroup=1,
              parallelizePath=True, ranlux=False, nskip=0, ranluxIntMax=2 ** 31 - 1, nOps=0,
              nCorr=0, corLags=None, bins=False, binRes=1, xMin=0.0, xMax=1.0, localSize=None,
-             nThreads=None, nHistObs=0, histBins=1, histRange=(0.0, 1.0)) -> OracleCfg:
+             nThreads=None, nHistObs=0, histBins=1, histRange=(0.0, 1.0), bisection=True, pathShift=False,
+             singleNode=False, alpha=0.0, psAlpha=0.0) -> OracleCfg:
     c = OracleCfg()
     c.N, c.S, c.DOF = N, S, DOF
     c.nWalkers, c.walkersPerGroup = nWalkers, walkersPerGroup
@@ -299,6 +302,9 @@ def make_cfg(N, S, DOF, nWalkers, beta, operatorRuns, metroSteps, *, walkersPerG
     c.nHistObs, c.histBins = nHistObs, histBins
     c.histLo = float(histRange[0])
     c.histInv = float(histBins) / float(histRange[1] - histRange[0])
+    c.bisection, c.pathShift, c.singleNode = int(bool(bisection)), int(bool(pathShift)), int(bool(singleNode))
+    c.alpha, c.psAlpha = float(alpha), float(psAlpha)
+    c.epsInv2 = (float(N) / beta) ** 2
     return c
 
 

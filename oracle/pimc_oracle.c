@@ -23,6 +23,8 @@
  *   fs_rl_os / fs_rl_next4 / fs_rl_init   examples/pyopencl-ranluxcl.cl:363-557, 597-725
  *   rl_wrap_float / rl_wrap_int           kernel.c:180-207
  *   bisect_move                           kernel.c:258-399 (sigmaN table 605-619)
+ *   kin_est / shift_diff / path_shift_move / single_node_move
+ *                                         kernel.c:245-249, 407-473, 640-705
  *   fs_oracle_run_reference               kernel.c:514-837, launch geometry kernel.py:192-211
  *   fs_oracle_run_wave                    the engine's slice-per-lane ordering of the same
  *                                         Levy-bisection move (no reference counterpart;
@@ -295,6 +297,9 @@ typedef struct {
     double ranluxIntScale;   /* float(ranluxIntMax + 1) */
     double xMin, xMax, invBinSize;
     double histLo, histInv;
+    int32_t bisection, pathShift, singleNode, pad_;
+    double alpha, psAlpha;   /* single-node and path-shift step widths */
+    double epsInv2;          /* (N / beta)^2, kinEnergyEst's scale (kernel.py:299) */
 } fs_oracle_cfg;
 
 typedef struct {
@@ -446,6 +451,93 @@ static fs_item item_of(const fs_oracle_cfg *c, int walker, int lid0)
     return it;
 }
 
+/* kinEnergyEst (kernel.c:245-249): half the squared step in units of 1/eps^2. */
+static inline FLOAT_TYPE kin_est(FLOAT_TYPE a, FLOAT_TYPE b, FLOAT_TYPE epsInv2)
+{
+    FLOAT_TYPE delta = a - b;
+    return 0.5 * (delta * delta) * epsInv2;
+}
+
+/* shiftPathEnergyDiff (kernel.c:407-473): adds `offset` to coordinate `degree`
+ * of slices left..right (wrapping past N - 1 when left > right) and returns the
+ * change of kinetic estimator plus potential, in the reference's order. */
+static FLOAT_TYPE shift_diff(FLOAT_TYPE *path, int N, int degree, FLOAT_TYPE offset, int left, int right,
+                             FLOAT_TYPE epsInv2)
+{
+    if (left == right) return 0.0;
+    FLOAT_TYPE *xd = path + (size_t)degree * N;
+    FLOAT_TYPE kinDiff = 0.0, potDiff = 0.0, x[FS_DOF];
+    int ll, rr;
+    if (left < right) {
+        ll = left == 0 ? N - 1 : left - 1;
+        rr = right == N - 1 ? 0 : right + 1;
+    } else {
+        ll = left - 1;
+        rr = right + 1;
+    }
+    kinDiff -= kin_est(xd[left], xd[ll], epsInv2);
+    kinDiff -= kin_est(xd[right], xd[rr], epsInv2);
+    int count = left < right ? right - left + 1 : N - left + right + 1;
+    for (int t = 0; t < count; t++) {
+        int i = left + t;
+        if (i >= N) i -= N;
+        gather(path, N, i, x);
+        potDiff -= fs_potential_v(x);
+        xd[i] = xd[i] + offset;
+        gather(path, N, i, x);
+        potDiff += fs_potential_v(x);
+    }
+    kinDiff += kin_est(xd[left], xd[ll], epsInv2);
+    kinDiff += kin_est(xd[right], xd[rr], epsInv2);
+    return kinDiff + potDiff;
+}
+
+/* Path-shift move (kernel.c:640-663): two integer draws from the group stream
+ * pick the interval, a uniform the offset in [-PSAlpha, PSAlpha], a second
+ * uniform the decision; a rejected shift is undone by shifting back. */
+static void path_shift_move(FLOAT_TYPE *path, int N, int degree, fs_stream *own, fs_stream *grp, uint32_t *acc,
+                            FLOAT_TYPE epsv, FLOAT_TYPE epsInv2, FLOAT_TYPE psAlpha)
+{
+    int left = (int)st_below(grp, (uint32_t)N);
+    int right = (int)st_below(grp, (uint32_t)N);
+    FLOAT_TYPE offset = psAlpha * (2.0 * st_float(own) - 1.0);
+    FLOAT_TYPE dE = shift_diff(path, N, degree, offset, left, right, epsInv2);
+    if (exp(-epsv * dE) < st_float(own))
+        shift_diff(path, N, degree, -offset, left, right, epsInv2);
+    else
+        (*acc)++;
+}
+
+/* Single-node move (kernel.c:665-703): node `step` mod (N DOF) of the walker's
+ * DOF-major path, uniform step in [-alpha, alpha]. */
+static void single_node_move(FLOAT_TYPE *path, int N, uint32_t step, fs_stream *own, uint32_t *acc,
+                             FLOAT_TYPE epsv, FLOAT_TYPE epsInv2, FLOAT_TYPE alpha)
+{
+    int modPoint = (int)(step % (uint32_t)N);
+    int modPathPoint = (int)(step % (uint32_t)(N * FS_DOF));
+    int rightIndex = modPathPoint + 1, leftIndex = modPathPoint - 1;
+    if (modPoint == 0)
+        leftIndex += N;
+    else if (modPoint == N - 1)
+        rightIndex -= N;
+    FLOAT_TYPE oldX = path[modPathPoint];
+    FLOAT_TYPE modX = oldX + (2.0 * st_float(own) - 1.0) * alpha;
+    FLOAT_TYPE diffE = kin_est(modX, path[rightIndex], epsInv2) + kin_est(modX, path[leftIndex], epsInv2) -
+                       kin_est(path[modPathPoint], path[leftIndex], epsInv2) -
+                       kin_est(path[modPathPoint], path[rightIndex], epsInv2);
+    FLOAT_TYPE x[FS_DOF];
+    gather(path, N, modPoint, x);
+    diffE -= fs_potential_v(x);
+    path[modPathPoint] = modX;
+    gather(path, N, modPoint, x);
+    diffE += fs_potential_v(x);
+    path[modPathPoint] = oldX;
+    if (exp(-epsv * diffE) > st_float(own)) {
+        path[modPathPoint] = modX;
+        (*acc)++;
+    }
+}
+
 FS_EXPORT int fs_oracle_run_reference(const fs_oracle_cfg *c, FLOAT_TYPE *paths, uint32_t *accepts,
                                       uint32_t *seeds, fs_ranlux *rlTab, fs_ranlux *groupRlTab,
                                       FLOAT_TYPE *opMeans, FLOAT_TYPE *corMeans, uint32_t *bins,
@@ -454,7 +546,9 @@ FS_EXPORT int fs_oracle_run_reference(const fs_oracle_cfg *c, FLOAT_TYPE *paths,
     const int N = c->N, S = c->S, P = 1 << S;
     const int T = c->parallelizePath ? (N >> S) : 1;
     const FLOAT_TYPE epsv = (FLOAT_TYPE)c->epsilon;
+    const FLOAT_TYPE epsInv2 = (FLOAT_TYPE)c->epsInv2;
     if (c->DOF != FS_DOF) return -1;
+    if ((c->pathShift || c->singleNode) && c->parallelizePath) return -3;
     uint32_t *snap = NULL;
     if (!c->ranlux) {
         size_t rows = (size_t)c->nThreads + 1;
@@ -504,6 +598,14 @@ FS_EXPORT int fs_oracle_run_reference(const fs_oracle_cfg *c, FLOAT_TYPE *paths,
         for (int i = 0; i < c->operatorRuns; i++) {
             for (int j = 0; j < c->metroSteps; j++) {
                 for (int k = 0; k < T; k++) {
+                    /* per step: path shift, single node, bisection (kernel.c:640-721) */
+                    if (c->pathShift)
+                        path_shift_move(path, N, (int)(((uint32_t)c->metroSteps * i + j) % FS_DOF), &own[k],
+                                        &grp[k], &acc[k], epsv, epsInv2, (FLOAT_TYPE)c->psAlpha);
+                    if (c->singleNode)
+                        single_node_move(path, N, (uint32_t)(i * c->metroSteps + j), &own[k], &acc[k], epsv,
+                                         epsInv2, (FLOAT_TYPE)c->alpha);
+                    if (!c->bisection) continue;
                     uint32_t start;
                     if (c->parallelizePath)
                         start = (st_below(&grp[k], (uint32_t)P) + (uint32_t)(P * k)) % (uint32_t)N;

@@ -222,3 +222,47 @@ def test_learned_rejects_double_and_deep_levels():
                          compile_only=True)
     with pytest.raises(_native.FsError, match="S <= 6"):
         _native.Proposal(_cfg(HarmOsc(), 256, 7, 1), system_section(HarmOsc(), ("x1",)), _lp(), compile_only=True)
+
+
+@pytest.mark.parametrize("name", ["ho", "lm2m2-3"])
+def test_exact_module_with_shift_and_single_node_compiles(name):
+    """Path-shift and single-node moves (kernel.c:407-473, 640-705), with and without bisection."""
+    from feynsimul_amd import _native
+    system = {"ho": HarmOsc(), "lm2m2-3": Lm2m2Cluster(3)}[name]
+    for nob in (0, 1):
+        c = _cfg(system, 128, 3, 0, parallelize_path=0, path_shift=1, single_node=1, alpha=0.5, ps_alpha=0.3,
+                 no_bisection=nob)
+        e = _native.Engine(c, system_section(system, (system.energyOp,)), compile_only=True)
+        assert "#define FS_PATH_SHIFT 1" in e.source and "#define FS_BISECTION %d" % (1 - nob) in e.source
+        e.close()
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(sampler=1, path_shift=1), "exact sampler"),
+    (dict(sampler=0, parallelize_path=1, single_node=1), "exact sampler"),
+    (dict(sampler=1, no_bisection=1), "bisection"),
+])
+def test_move_configs_rejected(kw, msg):
+    from feynsimul_amd import _native
+    c = _cfg(HarmOsc(), 64, 3, kw.pop("sampler"), **kw)
+    with pytest.raises(_native.FsError, match=msg):
+        _native.Engine(c, system_section(HarmOsc(), ("x1",)), compile_only=True)
+
+
+def test_kernel_args_moves_select_the_exact_sampler():
+    """kernel.py:193-198: the moves need enableParallelizePath = False; the engine then runs
+    one work-item per walker (the exact sampler)."""
+    from feynsimul_amd.kernel_args import KernelArgs
+    from feynsimul_amd.kernel import PIMCKernel
+    base = dict(system=HarmOsc(), nbrOfWalkers=64, N=64, beta=2.0, enableOperator=True, enableCorrelator=False,
+                enableBisection=True, enableBins=False, nbrOfWalkersPerWorkGroup=1, S=3, operators=("x1",),
+                operatorRuns=1, metroStepsPerOperatorRun=1, enableGlobalPath=False, enableGlobalOldPath=False)
+    with pytest.raises(Exception, match="Shift path not possible"):
+        PIMCKernel(KernelArgs(enablePathShift=True, enableSingleNodeMove=False, PSAlpha=0.1,
+                              enableParallelizePath=True, **base))
+    with pytest.raises(NameError, match="alpha need to be set"):
+        PIMCKernel(KernelArgs(enablePathShift=False, enableSingleNodeMove=True, enableParallelizePath=False,
+                              **base))
+    with pytest.raises(Exception, match="wave sampler runs bisection moves only"):
+        PIMCKernel(KernelArgs(enablePathShift=True, enableSingleNodeMove=False, PSAlpha=0.1,
+                              enableParallelizePath=False, sampler="wave", **base))

@@ -122,6 +122,73 @@ def test_exact_sampler_matches_reference_order(gpu, case):
     assert rel.max() < 1e-3, "operator mean error %g" % rel.max()
 
 
+CASES_MOVES = [
+    # name, moves, ranlux, W, N, beta, Wg
+    ("ho", "shift", False, 64, 32, 4.0, 1),
+    ("ho", "single", False, 64, 32, 4.0, 1),
+    ("ho", "all", False, 64, 32, 4.0, 4),
+    ("ho", "all", True, 32, 32, 4.0, 1),
+    ("lm2m2", "all", False, 16, 64, 10.0, 1),
+]
+
+
+@pytest.mark.parametrize("case", CASES_MOVES, ids=lambda c: "%s-%s-rl%d-Wg%d" % (c[0], c[1], c[2], c[6]))
+def test_exact_sampler_moves_match_reference_order(gpu, case):
+    """Path-shift and single-node moves (kernel.c:407-473, 640-705), one work-item per walker,
+    against the oracle's reference-order loop: streams bit-identical, decisions and paths as
+    in the exact-sampler test."""
+    name, moves, rl, W, N, beta, Wg = case
+    _case("exact moves " + str(case))
+    system = _system(name)
+    S = 2
+    rng = np.random.default_rng(5)
+    p0 = _initial_paths(system, W, N, rng)
+    opRuns, steps = 3, 2 * N
+    shift, single, bis = moves in ("shift", "all"), moves in ("single", "all"), moves == "all"
+    alpha, psalpha = (0.5, 0.3) if name == "ho" else (0.05, 0.05)
+    ka = KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=S, enableOperator=True,
+                    enableCorrelator=False, enableBisection=bis, enablePathShift=shift,
+                    enableSingleNodeMove=single, enableParallelizePath=False, enableGlobalPath=False,
+                    enableGlobalOldPath=False, enableBins=False, enableRanlux=rl,
+                    luxuaryFactor=1 if rl else None, ranluxIntMax=2 ** 31 - 1 if rl else None,
+                    metroStepsPerOperatorRun=steps, operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=Wg,
+                    operators=(system.energyOp,), alpha=alpha if single else None,
+                    PSAlpha=psalpha if shift else None, seed=11, sampler="exact", fastMath=False)
+    k = PIMCKernel(ka)
+    if rl:
+        k.initRanlux()
+        k.groupInitRanlux()
+    k.setPaths(p0)
+    k.run()
+    lib = O.build(system, N, beta, operators=(system.energyOp,))
+    cfg = O.make_cfg(N, S, system.DOF, W, beta, opRuns, steps, walkersPerGroup=Wg, parallelizePath=False,
+                     ranlux=rl, nskip=O.nskip_of_lux(2), nOps=1, bisection=bis, pathShift=shift,
+                     singleNode=single, alpha=alpha, psAlpha=psalpha)
+    paths = np.ascontiguousarray(p0.astype(np.float32))
+    T = cfg.nThreads
+    if rl:
+        st = O.ranlux_init(lib, O.ranlux_seeds(11, 0, T))
+        gseeds = np.array([O.splitmix64_pair((11 + _GROUP_OFFSET + w // Wg) & ((1 << 64) - 1))[0]
+                           for w in range(W)], dtype=np.uint64)
+        gst = O.ranlux_init(lib, gseeds)
+        ref = O.run_reference(lib, cfg, paths, rl=st, grl=gst)
+        np.testing.assert_array_equal(k._rl.cpu().numpy().view(np.uint8).reshape(T, 112),
+                                      st.view(np.uint8).reshape(T, 112))
+    else:
+        seeds = O.xorshift_rows(11, 0, T + 1)
+        ref = O.run_reference(lib, cfg, paths, seeds=seeds)
+        np.testing.assert_array_equal(k._seeds.cpu().numpy().view(np.uint32), seeds)
+    acc = k._accepts.cpu().numpy().view(np.uint32)
+    moves_per_step = int(shift) + int(single) + int(bis)
+    print("acceptance %.3f" % (ref["accepts"].sum() / (W * opRuns * steps * moves_per_step)))
+    assert 0.05 < ref["accepts"].sum() / (W * opRuns * steps * moves_per_step) < 0.98
+    tol = 2e-4 if name == "ho" else 2e-3
+    ok = _compare(k.getPaths(), paths, acc, ref["accepts"], np.arange(W), tol, min_match=0.9)
+    opm = k._operatorValues.cpu().numpy()
+    rel = np.abs(opm[ok] - ref["opMeans"][ok]) / (np.abs(ref["opMeans"][ok]) + 1e-3)
+    assert rel.max() < 1e-3, "operator mean error %g" % rel.max()
+
+
 CASES_WAVE = [
     # name, W, N, S, beta, ranlux, double, path storage
     ("ho", 64, 256, 4, 10.0, False, False, "registers"),

@@ -183,3 +183,30 @@ def test_aho_full_size_ranlux_launch(gpu):
     assert not torch.equal(st0, k._rl[:1024])
     del k
     torch.cuda.empty_cache()
+
+
+def test_ho_energy_with_single_node_and_path_shift_moves(gpu):
+    """No bisection: single-node (alpha 0.6) and path-shift (PSAlpha 0.3) moves alone, one
+    work-item per walker (kernel.c:640-705), must sample <x^2> of the discretised HO."""
+    _case("HO single-node + path-shift energy")
+    system = HarmOsc()
+    W, N, beta = 4096, 32, 4.0
+
+    def ka(opRuns, seed):
+        return KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=None, enableOperator=True,
+                          enableCorrelator=False, enableBisection=False, enablePathShift=True,
+                          enableSingleNodeMove=True, enableParallelizePath=False, enableGlobalPath=False,
+                          enableGlobalOldPath=False, enableBins=False, enableDouble=False, enableRanlux=False,
+                          metroStepsPerOperatorRun=N, operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=1,
+                          operators=(system.energyOp,), alpha=0.6, PSAlpha=0.3, seed=seed)
+    burn = PIMCKernel(ka(300, 31))
+    burn.setPaths(np.random.default_rng(6).uniform(-1, 1, (W, N)))
+    burn.run()
+    k = PIMCKernel(ka(300, 32))
+    k.setPaths(burn.getPaths())
+    k.run()
+    per = k.getOperators()[0]
+    E, se = per.mean(), per.std() / np.sqrt(W)
+    exact = O.ho_discrete_x2(N, beta)
+    print("E = %.6f +- %.6f, exact %.9f, AR %.3f" % (E, se, exact, k.getAcceptanceRate() / 2))
+    assert abs(E - exact) < 4 * se

@@ -351,3 +351,34 @@ def test_lm2m2_system_strings_in_oracle():
         vir = sum(lm2m2_kelvin(d) + 0.5 * d * (lm2m2_kelvin(d + h) - lm2m2_kelvin(d - h)) / (2 * h) for d in dists)
         assert abs(acc[0] - vir) < 1e-5 * max(1, abs(vir))
         assert abs(acc[1] - np.mean(dists)) < 1e-9
+
+
+@pytest.mark.parametrize("moves", ["shift", "single", "all"])
+def test_path_shift_and_single_node_keep_the_exact_law(moves):
+    """Path-shift (kernel.c:407-473, 640-663) and single-node (kernel.c:665-703) moves,
+    alone and with bisection, one work-item per walker: started in the exact Gaussian of
+    the discretised HO action on an 8-slice ring, the chain must stay there."""
+    N, S, beta, W = 8, 2, 2.0, 16384
+    Ce = _cov_exact(N, beta)
+    lib = O.build(HarmOsc(), N, beta, operators=("1.0 * x1 * x1",))
+    M = 3 * N
+    cfg = O.make_cfg(N, S, 1, W, beta, 1, M, parallelizePath=False, nOps=1, bisection=(moves == "all"),
+                     pathShift=moves in ("shift", "all"), singleNode=moves in ("single", "all"),
+                     alpha=0.8, psAlpha=0.6)
+    rng = np.random.default_rng(11)
+    paths = (rng.standard_normal((W, N)) @ np.linalg.cholesky(Ce).T).astype(np.float32)
+    seeds = O.xorshift_rows(23, 0, cfg.nThreads + 1)
+    acc = 0
+    snaps = []
+    for t in range(12):
+        out = O.run_reference(lib, cfg, paths, seeds=seeds)
+        acc += out["accepts"].sum()
+        if t >= 4:
+            snaps.append(paths.astype(np.float64).copy())
+    X = np.concatenate(snaps)
+    moves_per_step = {"shift": 1, "single": 1, "all": 3}[moves]
+    ar = acc / (12 * W * M * moves_per_step)
+    assert 0.1 < ar < 0.95, ar
+    assert np.abs(np.cov(X.T) - Ce).max() < 0.03 * Ce[0, 0]
+    assert np.abs(X.mean(axis=0)).max() < 0.02 * np.sqrt(Ce[0, 0])
+
