@@ -16,5 +16,6 @@ contract of FeynSimul/physical_systems/__init__.py:1-18.
 from .harm_osc import HarmOsc
 from .anharm_osc import AnharmOsc
 from .lm2m2 import Lm2m2Cluster
+from .lm2m2_cluster import Lm2m2_cluster
 
-__all__ = ["HarmOsc", "AnharmOsc", "Lm2m2Cluster"]
+__all__ = ["HarmOsc", "AnharmOsc", "Lm2m2Cluster", "Lm2m2_cluster"]

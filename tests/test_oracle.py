@@ -382,3 +382,49 @@ def test_path_shift_and_single_node_keep_the_exact_law(moves):
     assert np.abs(np.cov(X.T) - Ce).max() < 0.03 * Ce[0, 0]
     assert np.abs(X.mean(axis=0)).max() < 0.02 * np.sqrt(Ce[0, 0])
 
+
+
+def _eval_system_string(expr, x):
+    """Evaluates a generated system string (only fs_sqr, fs_lm2m2_r2 / virial and x1..xD) in FP64."""
+    import re
+    from feynsimul_amd.physical_systems.lm2m2 import lm2m2_kelvin
+    assert set(re.findall(r"[A-Za-z_][A-Za-z_0-9]*", expr)) <= (
+        {"fs_sqr", "fs_lm2m2_r2", "fs_lm2m2_virial_r2"} | {"x%d" % (i + 1) for i in range(len(x))})
+
+    def vir(r2):
+        r, h = math.sqrt(r2), 1e-6
+        return lm2m2_kelvin(r) + 0.5 * r * (lm2m2_kelvin(r + h) - lm2m2_kelvin(r - h)) / (2 * h)
+    env = {"fs_sqr": lambda a: a * a, "fs_lm2m2_r2": lambda r2: lm2m2_kelvin(math.sqrt(r2)),
+           "fs_lm2m2_virial_r2": vir}
+    env.update({"x%d" % (i + 1): float(v) for i, v in enumerate(x)})
+    return eval(expr, {"__builtins__": {}}, env)
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 6])
+def test_jacobi_cluster_matches_cartesian_pair_sum(n):
+    """Lm2m2_cluster (lm2m2_cluster.py:7-95, jacobi.py:3-36): unit-mass kinetic metric,
+    centre of mass removed, and the generated potential / virial / radius strings equal the
+    direct Cartesian LM2M2 pair sums at random configurations."""
+    from feynsimul_amd.physical_systems import Lm2m2_cluster
+    from feynsimul_amd.physical_systems.lm2m2_cluster import jacobi_matrix
+    from feynsimul_amd.physical_systems.lm2m2 import lm2m2_kelvin
+    sysobj = Lm2m2_cluster(n, 0.1)
+    assert sysobj.DOF == 3 * (n - 1) and abs(sysobj.beta - 10.0) < 1e-12
+    assert abs(math.sqrt(sysobj.m) - 0.287251) < 2e-6          # sqrt(m_He) in K^-1/2 / angstrom
+    J = jacobi_matrix(n, sysobj.m)
+    np.testing.assert_allclose(J @ np.diag(np.full(n, 1.0 / sysobj.m)) @ J.T, np.eye(n), atol=1e-12)
+    rng = np.random.default_rng(n)
+    for _ in range(5):
+        r = rng.normal(0, 3.0, (n, 3)) + rng.normal(0, 20.0, 3)      # angstrom, arbitrary centre
+        j = J @ r                                                      # (n, 3) Jacobi vectors
+        x = j[: n - 1].reshape(-1)                                     # x_{3i+k+1} = j_i[k]
+        pairs = [(a, b) for a in range(n) for b in range(a + 1, n)]
+        dist = [np.linalg.norm(r[a] - r[b]) for a, b in pairs]
+        V = sum(lm2m2_kelvin(d_) for d_ in dist)
+        assert abs(_eval_system_string(sysobj.potential, x) - V) < 1e-9 * max(1.0, abs(V))
+        assert abs(_eval_system_string(sysobj.meanSquaredRadiusOp, x) - sum(d_ ** 2 for d_ in dist)) < 1e-8
+        # virial V + x . grad V / 2 by central differences in the Jacobi coordinates
+        h = 1e-5
+        g = np.array([(_eval_system_string(sysobj.potential, x + h * e) -
+                       _eval_system_string(sysobj.potential, x - h * e)) / (2 * h) for e in np.eye(len(x))])
+        assert abs(_eval_system_string(sysobj.energyOp, x) - (V + 0.5 * x @ g)) < 1e-4 * max(1.0, abs(V))
