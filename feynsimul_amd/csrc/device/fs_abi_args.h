@@ -27,6 +27,24 @@ struct fs_kargs {
     int flags;            // reserved (0)
 };
 
+// Correlator staging of the wave sampler (shared by the host, which sizes the
+// dynamic LDS, and the device code).  A lane owns fs_cor_lanelags(lags)
+// consecutive lags of a lag chunk of 64 times as many.  Paths of up to
+// FS_COR_FULL_N slices stage the whole correlator row in LDS; longer ones
+// stage time chunks of FS_COR_CH slices plus the lag window that follows.
+#define FS_COR_FULL_N 1024
+#define FS_COR_CH 512
+constexpr int fs_cor_lanelags(int lags)
+{
+    int t = 1;
+    while (t < 8 && 64 * t < lags) t <<= 1;
+    return t;
+}
+constexpr int fs_cor_stage_len(int N, int lags)
+{
+    return N <= FS_COR_FULL_N ? N : 2 * FS_COR_CH + 64 * fs_cor_lanelags(lags);
+}
+
 // Persistent batched launch over several system types (one wave per walker).
 // Walkers are numbered in dispatch order: type typeOf[d] owns numbers
 // [start[d], start[d + 1]); wavefronts take numbers from *counter until

@@ -51,10 +51,46 @@ template <int G, typename T> FS_DEV T fs_group_sum(T v)
 }
 
 // Bytes of LDS one wavefront uses for a system: correlator staging, then a
-// private histogram of the scalar observables.
+// private histogram of the scalar observables; rounded up to 16 bytes.
 template <class Sys> constexpr int fs_wave_lds_bytes()
 {
-    return Sys::N * Sys::NCORR * (int)sizeof(FLOAT_TYPE) + (Sys::NHOBS > 0 ? Sys::HBINS * 4 : 0);
+    return ((Sys::NCORR > 0 ? fs_cor_stage_len(Sys::N, Sys::CORLAGS) * (int)sizeof(FLOAT_TYPE) : 0) +
+            (Sys::NHOBS > 0 ? Sys::HBINS * 4 : 0) + 15) & ~15;
+}
+
+FS_DEV void fs_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Lag sums acc[k] += sum_t c_t c_{t+tau0+k}, k < CT, for t = 0..nt-1 in
+// ascending order (each lag keeps the sequential FP32 order).  bc(t) reads
+// c_t (the same word for every lane), win(i) reads c_{i+tau0} for the lane's
+// first lag tau0.  The window of 2 CT values slides in registers, so a step
+// of CT slices costs CT broadcast reads, CT window reads and CT^2 FMAs.
+template <int CT, typename F, typename BC, typename WIN>
+FS_DEV void fs_cor_accumulate(F (&acc)[CT], int nt, BC bc, WIN win)
+{
+    F wv[2 * CT];
+#pragma unroll
+    for (int k = 0; k < CT; k++) wv[k] = win(k);
+#pragma unroll 1
+    for (int t = 0; t < nt; t += CT) {
+        F av[CT];
+#pragma unroll
+        for (int j = 0; j < CT; j++) av[j] = bc(t + j);
+#pragma unroll
+        for (int k = 0; k < CT; k++) wv[CT + k] = win(t + CT + k);
+#pragma unroll
+        for (int j = 0; j < CT; j++) {
+#pragma unroll
+            for (int k = 0; k < CT; k++) acc[k] += av[j] * wv[j + k];
+        }
+#pragma unroll
+        for (int k = 0; k < CT; k++) wv[k] = wv[CT + k];
+    }
 }
 
 template <class Sys>
@@ -95,7 +131,8 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
     for (int k = 0; k < NOPS1; k++) opS[k] = opC[k] = 0.0;
 
     F *cbuf = reinterpret_cast<F *>(lds);
-    unsigned *hbuf = reinterpret_cast<unsigned *>(lds + N * Sys::NCORR * (int)sizeof(F));
+    unsigned *hbuf = reinterpret_cast<unsigned *>(
+        lds + (Sys::NCORR > 0 ? fs_cor_stage_len(N, Sys::CORLAGS) * (int)sizeof(F) : 0));
     if constexpr (Sys::NHOBS > 0) {
         for (int b = lane; b < Sys::HBINS; b += 64) hbuf[b] = 0u;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -287,14 +324,6 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
                             if (v >= (F)0.0 && v < (F)Sys::HBINS) atomicAdd(hbuf + (int)v, 1u);
                         }
                     }
-                    if constexpr (Sys::NCORR > 0) {
-                        F prod[NCORR1];
-#pragma unroll
-                        for (int c = 0; c < NCORR1; c++) prod[c] = (F)1.0;
-                        Sys::corr(x, prod);
-#pragma unroll
-                        for (int c = 0; c < Sys::NCORR; c++) cbuf[c * N + t] = prod[c];
-                    }
                 }
             }
             if constexpr (Sys::NOPS > 0) {
@@ -310,27 +339,106 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
                 }
             }
             if constexpr (Sys::NCORR > 0) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                double *cb = reinterpret_cast<double *>(a.corMeans) + w * (long)(Sys::NCORR * Sys::CORLAGS * 2);
+                // two-time correlators C_c(tau) = sum_t c(x_t) c(x_{t+tau}), tau < CORLAGS, one
+                // FP32 sum per measurement added to FP64 Kahan bins (sum, compensation)
+                constexpr int L = Sys::CORLAGS;
+                constexpr int CT = fs_cor_lanelags(L);
+                constexpr int LSPAN = 64 * CT;
+                constexpr int NLC = (L + LSPAN - 1) / LSPAN;
+                constexpr bool FULL = N <= FS_COR_FULL_N;
+                static_assert(FULL || !Sys::RESIDENT, "register storage implies a fully staged correlator");
+                double *cb = reinterpret_cast<double *>(a.corMeans) + w * (long)(Sys::NCORR * L * 2);
+                // correlator c at slice s
+                auto cval = [&](int c, int s) -> F {
+                    F x[DOF], prod[NCORR1];
 #pragma unroll
+                    for (int d = 0; d < DOF; d++) x[d] = path[d * N + s];
+#pragma unroll
+                    for (int q = 0; q < NCORR1; q++) prod[q] = (F)1.0;
+                    Sys::corr(x, prod);
+                    F v = prod[0];
+#pragma unroll
+                    for (int q = 1; q < NCORR1; q++) v = (c == q) ? prod[q] : v;
+                    return v;
+                };
+                auto bin = [&](int c, int tau0, const F (&acc)[CT]) {
+#pragma unroll
+                    for (int k = 0; k < CT; k++) {
+                        const int tau = tau0 + k;
+                        if (tau < L) {
+                            double *bn = cb + (c * L + tau) * 2;
+                            const double sum = bn[0], comp = bn[1];
+                            const double y = (double)acc[k] - comp;
+                            const double t2 = sum + y;
+                            bn[1] = (t2 - sum) - y;
+                            bn[0] = t2;
+                        }
+                    }
+                };
+#pragma unroll 1
                 for (int c = 0; c < Sys::NCORR; c++) {
-                    const F *cv = cbuf + c * N;
-                    for (int tau = lane; tau < Sys::CORLAGS; tau += 64) {
-                        F acc = (F)0.0;
-                        for (int t = 0; t < N; t++) acc += cv[t] * cv[(t + tau) & (N - 1)];
-                        double *bin = cb + (c * Sys::CORLAGS + tau) * 2;
-                        const double sum = bin[0], comp = bin[1];
-                        const double y = (double)acc - comp;
-                        const double t2 = sum + y;
-                        bin[1] = (t2 - sum) - y;
-                        bin[0] = t2;
+                    if constexpr (FULL) {
+                        // the whole row c_0..c_{N-1}; lane j stages slices j, j + B, ...
+                        if (active) {
+                            if constexpr (Sys::RESIDENT) {
+#pragma unroll
+                                for (int k = 0; k < NB; k++) {
+                                    F x[DOF], prod[NCORR1];
+#pragma unroll
+                                    for (int d = 0; d < DOF; d++) x[d] = xs[d][k];
+#pragma unroll
+                                    for (int q = 0; q < NCORR1; q++) prod[q] = (F)1.0;
+                                    Sys::corr(x, prod);
+                                    F v = prod[0];
+#pragma unroll
+                                    for (int q = 1; q < NCORR1; q++) v = (c == q) ? prod[q] : v;
+                                    cbuf[k * B + lane] = v;
+                                }
+                            } else {
+#pragma unroll 1
+                                for (int k = 0; k < NB; k++) cbuf[k * B + lane] = cval(c, k * B + lane);
+                            }
+                        }
+                        fs_wave_sync();
+#pragma unroll 1
+                        for (int lc = 0; lc < NLC; lc++) {
+                            const int tau0 = lc * LSPAN + lane * CT;
+                            F acc[CT];
+#pragma unroll
+                            for (int k = 0; k < CT; k++) acc[k] = (F)0.0;
+                            fs_cor_accumulate<CT>(acc, N, [&](int t) { return cbuf[t]; },
+                                                  [&](int i) { return cbuf[(i + tau0) & (N - 1)]; });
+                            bin(c, tau0, acc);
+                        }
+                        fs_wave_sync();
+                    } else {
+                        // time chunks [t0, t0 + CH): bc = c_{t0..}, wn = c_{t0 + lag chunk start ..}
+                        constexpr int CH = FS_COR_CH;
+                        F *bcv = cbuf;
+                        F *wnv = cbuf + CH;
+#pragma unroll 1
+                        for (int lc = 0; lc < NLC; lc++) {
+                            const int lag0 = lc * LSPAN;
+                            const int tau0 = lag0 + lane * CT;
+                            F acc[CT];
+#pragma unroll
+                            for (int k = 0; k < CT; k++) acc[k] = (F)0.0;
+#pragma unroll 1
+                            for (int t0 = 0; t0 < N; t0 += CH) {
+#pragma unroll 1
+                                for (int i = lane; i < CH; i += 64) bcv[i] = cval(c, t0 + i);
+#pragma unroll 1
+                                for (int i = lane; i < CH + LSPAN; i += 64)
+                                    wnv[i] = cval(c, (t0 + lag0 + i) & (N - 1));
+                                fs_wave_sync();
+                                fs_cor_accumulate<CT>(acc, CH, [&](int t) { return bcv[t]; },
+                                                      [&](int i) { return wnv[i + lane * CT]; });
+                                fs_wave_sync();
+                            }
+                            bin(c, tau0, acc);
+                        }
                     }
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
         }
     }

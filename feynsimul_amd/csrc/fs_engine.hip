@@ -238,9 +238,11 @@ std::string wave_trait(const fs_config *c)
 
 size_t wave_lds_bytes(const fs_config *c)
 {
+    // the same figure as fs_wave_lds_bytes<Sys>() in device/fs_wave.h
     const size_t fsize = c->double_precision ? 8 : 4;
-    return (c->n_corr > 0 ? (size_t)c->N * c->n_corr * fsize : 0) +
-           (c->n_hist_obs > 0 ? (size_t)c->hist_bins * 4 : 0);
+    const int lags = c->cor_lags > 0 ? c->cor_lags : c->N / 2;
+    return ((c->n_corr > 0 ? (size_t)fs_cor_stage_len(c->N, lags) * fsize : 0) +
+            (c->n_hist_obs > 0 ? (size_t)c->hist_bins * 4 : 0) + 15) & ~(size_t)15;
 }
 
 std::vector<std::string> compile_options(const fs_config *c, const std::string &arch)

@@ -295,3 +295,50 @@ def test_wave_observable_histogram_matches_oracle(gpu, storage):
     # walkers whose Metropolis decisions flipped move a few slices; allow that many counts to move
     assert np.abs(counts - ref["obsHist"]).sum() <= 0.02 * counts.sum() + 6
     assert counts.sum() > 0.5 * W * N * 3 * 3
+
+
+CASES_CORR = [
+    # name, W, N, S, beta, lags, double, path storage, correlators
+    ("aho", 16, 512, 4, 10.0, 96, False, "registers", ("x1",)),
+    ("aho", 16, 512, 4, 10.0, 96, False, "memory", ("x1", "x1 * x1")),
+    ("aho", 8, 2048, 4, 20.0, 1024, False, None, ("x1",)),
+    ("aho", 8, 4096, 4, 40.0, 512, False, None, ("x1", "x1 * x1")),
+    ("ho", 16, 256, 4, 10.0, 128, True, None, ("x1",)),
+    ("ho", 32, 32, 3, 4.0, 16, False, None, ("x1",)),
+]
+
+
+@pytest.mark.parametrize("case", CASES_CORR, ids=lambda c: "%s-W%d-N%d-L%d-f64%d-%s-c%d" % (
+    c[0], c[1], c[2], c[5], c[6], c[7], len(c[8])))
+def test_wave_correlator_matches_oracle(gpu, case):
+    """Two-time correlator bins of the wave sampler (FP32 lag sums per measurement added to FP64
+    Kahan bins) against the oracle's FP64 lag sums of the same launch, for a whole staged row
+    (N <= 1024) and for time chunks with one or two lag chunks (N = 2048, 4096)."""
+    name, W, N, S, beta, lags, dbl, storage, corr = case
+    _case("wave correlator " + str(case))
+    system = _system(name)
+    rng = np.random.default_rng(17)
+    p0 = _initial_paths(system, W, N, rng)
+    opRuns, steps = 3, 2
+    ka = _ka(system, W, N, S, beta, opRuns, steps, True, False, dbl, "wave", storage=storage)
+    ka.enableCorrelator, ka.correlators, ka.correlatorLags = True, corr, lags
+    k = PIMCKernel(ka)
+    k.setPaths(p0)
+    k.run()
+    ft = np.float64 if dbl else np.float32
+    lib = O.build(system, N, beta, operators=(system.energyOp,), correlators=corr, double=dbl)
+    cfg = O.make_cfg(N, S, system.DOF, W, beta, opRuns, steps, nOps=1, nCorr=len(corr), corLags=lags)
+    paths = np.ascontiguousarray(p0.astype(ft))
+    ref = O.run_wave(lib, cfg, paths, lane_seeds=O.xorshift_rows(11, 0, W * 64),
+                     group_seeds=O.xorshift_rows(11, W * 64, W))
+    acc = k._accepts.cpu().numpy().view(np.uint32)
+    ok = _compare(k.getPaths(), paths, acc, ref["accepts"], np.arange(W), 2e-4, min_match=0.85)
+    raw = k._correlatorValues.cpu().numpy().astype(np.float64).reshape(W, len(corr), lags, 2)
+    got = raw[..., 0] - raw[..., 1]
+    exp = ref["corBins"].reshape(W, len(corr), lags)
+    scale = np.abs(exp[:, :, :1])
+    err = (np.abs(got - exp) / scale)[ok]
+    print("max error / C(0) bin: %.3g over %d walkers" % (err.max(), ok.sum()))
+    assert err.max() < (1e-7 if dbl else 2e-3)
+    # C(0) is the measurement sum of c^2, which the oracle forms the same way
+    assert np.all(exp[:, :, 0] > 0)
