@@ -223,6 +223,230 @@ def ragged_bench(steps, warmup):
             "acceptance": [round(k.getAcceptanceRate(), 4) for k in batch.kernels]}
 
 
+# ---- anharmonic oscillator: the correlator and RNG stress workload ----
+AHO_LAM, AHO_N, AHO_BETA, AHO_W, AHO_S, AHO_LAGS = 0.1, 4096, 40.0, 65536, 4, 512
+AHO_SEG = 1 << AHO_S
+AHO_UPDATES_PER_SWEEP = AHO_W * (AHO_N // AHO_SEG) * (AHO_SEG - 1)
+AHO_BYTES_PER_UPDATE = 8.0 * 1 * AHO_SEG / (AHO_SEG - 1)
+# 26 per coordinate + 2 potential evaluations (x^2/2 + lam x^4 and the eps-weighted sum: 5 ops)
+AHO_VALU_PER_UPDATE = 26 * 1 + 2 * 5
+AHO_GOLDEN = os.path.join(ROOT, "tests", "golden", "aho_c5.json")
+
+
+def make_aho_kernel(S, steps, opRuns, corr, ranlux=False, seed=5):
+    from feynsimul_amd.kernel_args import KernelArgs
+    from feynsimul_amd.kernel import PIMCKernel
+    from feynsimul_amd.physical_systems import AnharmOsc
+    system = AnharmOsc(AHO_LAM)
+    ka = KernelArgs(system=system, nbrOfWalkers=AHO_W, N=AHO_N, beta=AHO_BETA, S=S, enableOperator=True,
+                    enableCorrelator=corr, enableBisection=True, enablePathShift=False,
+                    enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                    enableGlobalOldPath=False, enableBins=False, enableDouble=False,
+                    enableRanlux=ranlux, luxuaryFactor=1 if ranlux else None,
+                    ranluxIntMax=2 ** 31 - 1 if ranlux else None,
+                    metroStepsPerOperatorRun=steps, operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=1,
+                    operators=(system.energyOp,), correlators=("x1",) if corr else None,
+                    seed=seed, sampler="wave", correlatorLags=AHO_LAGS if corr else None)
+    k = PIMCKernel(ka)
+    if ranlux:
+        k.initRanlux()
+        k.groupInitRanlux()
+    return system, k
+
+
+def timed_launches(k, steps, warmup):
+    """Mean device time (ms) of `steps` launches on the kernel's stream (HIP events) and the wall time (s)."""
+    import numpy as np
+    import torch
+    for _ in range(warmup):
+        k.run()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        ev[i][0].record(k._stream)
+        k.run(wait=False)
+        ev[i][1].record(k._stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return float(np.mean([a.elapsed_time(b) for a, b in ev])), wall
+
+
+def aho_ess(paths, S, measurements, seed):
+    """ESS/s of E0 with one measurement every 32 sweeps (no correlator), after 512 sweeps at this S."""
+    import torch
+    from feynsimul_amd.stats import effective_sample_size
+    _, k = make_aho_kernel(S, SWEEPS_PER_STEP, 16, False, seed=seed)
+    k.setPaths(paths)
+    k.run()
+    k.setSeriesCapacity(measurements)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(measurements // 16):
+        k.run(wait=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    r = effective_sample_size(k.getSeries()[:, :, 0])
+    ess_cm, se_cm = chain_mean_ess(k.getSeries()[:, :, 0])
+    return k, {"S": S, "value": ess_cm / dt, "unit": "effective samples/s (chain-mean estimator)",
+               "ess_sokal_per_s": r["ess"] / dt, "tau_int_measurements": r["tau_int"], "E0": r["mean"],
+               "sigma": se_cm, "seconds": dt, "sweeps": measurements * SWEEPS_PER_STEP,
+               "acceptance": k.getAcceptanceRate()}
+
+
+def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_train=200, cpu=True):
+    """Anharmonic oscillator x^2/2 + 0.1 x^4, N = 4096, beta = 40, 65536 chains, S = 4: one step = 32
+    sweeps + one measurement of the virial energy and the two-time correlator <x(0) x(tau)>,
+    tau = 0..511 (FP64 Kahan bins); xorshift128 and RANLUX luxury 2; estimators against the FP64
+    transfer matrix of the same discretised action; ESS/s of E0, plain and learned."""
+    import numpy as np
+    import torch
+    with open(AHO_GOLDEN) as f:
+        gold = json.load(f)["values"]
+    rng = np.random.default_rng(0)
+    p0 = rng.uniform(-1, 1, (AHO_W, AHO_N)).astype(np.float32)
+    _, kb = make_aho_kernel(AHO_S, 512, 1, False, seed=3)
+    kb.setPaths(p0)
+    for _ in range(4):
+        kb.run(wait=False)
+    paths = kb.getPaths()
+    del kb
+    torch.cuda.empty_cache()
+    # throughput, xorshift128 (the headline line of this workload)
+    _, k = make_aho_kernel(AHO_S, SWEEPS_PER_STEP, 1, True, seed=4)
+    k.setPaths(paths)
+    ms, wall = timed_launches(k, steps, warmup)
+    updates = AHO_UPDATES_PER_SWEEP * SWEEPS_PER_STEP
+    mean, se = k.getCorrelator()
+    per = k.getOperators()[0]
+    E, Ese = float(per.mean()), float(per.std() / np.sqrt(AHO_W))
+    cor = np.array(gold["correlator"][:AHO_LAGS])
+    z = np.abs(mean[0] - cor) / se[0]
+    eps = AHO_BETA / AHO_N
+    gap = float(-np.log(mean[0][300] / mean[0][100]) / (200 * eps))
+    achieved = AHO_BYTES_PER_UPDATE * updates / (ms * 1e-3) / 1e9
+    line = {"metric": "path-slice updates/s (plain bisection sampler)", "value": updates * steps / wall,
+            "unit": "slice-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic: paths uniform on [-1, 1], 2048 burn-in sweeps",
+            "config": {"workload": "anharmonic oscillator x^2/2 + 0.1 x^4, N=4096, beta=40, 65536 chains, S=4, "
+                                   "xorshift128, 32 sweeps + 1 measurement (virial energy, correlator "
+                                   "tau=0..511 in FP64 Kahan bins) per step", "sampler": "wave",
+                       "chains": AHO_W, "slices": AHO_N, "bisection_level": AHO_S, "lags": AHO_LAGS},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "fs_wave_sweep",
+                         "avg_launch_ms": ms, "bytes_per_update": AHO_BYTES_PER_UPDATE,
+                         "updates_per_launch": updates},
+            "valu": {"ops_per_update": AHO_VALU_PER_UPDATE, "achieved": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3),
+                     "peak": VALU_PEAK, "frac": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3) / VALU_PEAK},
+            "acceptance": k.getAcceptanceRate(),
+            "oracle": {"kind": "FP64 transfer matrix of the discretised action (tests/golden/aho_c5.json)",
+                       "E_virial": E, "E_virial_se": Ese, "E_virial_exact": gold["E_virial"],
+                       "E_virial_z": abs(E - gold["E_virial"]) / Ese,
+                       "correlator_max_z": float(z.max()), "correlator_lags": AHO_LAGS,
+                       "gap_from_correlator": gap, "gap_exact": gold["gap"]}}
+    del k
+    torch.cuda.empty_cache()
+    # the same with RANLUX luxury 2 streams
+    _, kr = make_aho_kernel(AHO_S, SWEEPS_PER_STEP, 1, True, ranlux=True, seed=6)
+    kr.setPaths(paths)
+    ms_rl, wall_rl = timed_launches(kr, max(2, steps // 2), 1)
+    mean_rl, se_rl = kr.getCorrelator()
+    line["ranlux"] = {"value": updates * max(2, steps // 2) / wall_rl, "avg_launch_ms": ms_rl,
+                      "luxury": 2, "correlator_max_z": float((np.abs(mean_rl[0] - cor) / se_rl[0]).max())}
+    del kr
+    torch.cuda.empty_cache()
+    # ESS/s of E0, plain sampler at S = 4 and at S = 6
+    kp, plain = aho_ess(paths, AHO_S, 256, seed=7)
+    plain["E0_exact"] = gold["E_virial"]
+    plain["z"] = abs(plain["E0"] - gold["E_virial"]) / plain["sigma"]
+    start = kp.getPaths()
+    del kp
+    torch.cuda.empty_cache()
+    k6, plain6 = aho_ess(paths, 6, 256, seed=8)
+    plain6["z"] = abs(plain6["E0"] - gold["E_virial"]) / plain6["sigma"]
+    del k6
+    torch.cuda.empty_cache()
+    line["ess"] = plain
+    line["ess_S6"] = plain6
+    if learned:
+        from feynsimul_amd.learned import LearnedProposal
+        from feynsimul_amd.stats import effective_sample_size
+        _, kl = make_aho_kernel(AHO_S, SWEEPS_PER_STEP, 1, False, seed=9)
+        kl.setPaths(start)
+        lp = LearnedProposal(kl, batch=65536, lr=1e-3, seed=10)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        lp.sweep(1)
+        ev0.record(kl._stream)
+        lp.sweep(4, wait=False)
+        ev1.record(kl._stream)
+        torch.cuda.synchronize()
+        ms_sweep = ev0.elapsed_time(ev1) / 4
+        ar0 = lp.lastAcceptanceOfQueued(4)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        first = lp.train(1)
+        last = lp.trainOnline(learned_train - 1, plainSweepsPerStep=2)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        kl.setSeriesCapacity(learned_measurements)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        lp.run(learned_measurements, SWEEPS_PER_STEP)
+        t_run = time.perf_counter() - t1
+        ser = kl.getSeries()[:, :, 0]
+        r = effective_sample_size(ser)
+        ess_cm, se_cm = chain_mean_ess(ser)
+        dt = t_train + t_run
+        flop_row = 2 * (3 * 128 + 2 * 128 * 128 + 128 * 2)
+        flop_upd = flop_row * (2 * (AHO_SEG - 1) - 1) / (AHO_SEG - 1)
+        line["learned"] = {"value": ess_cm / dt, "unit": "effective samples/s (chain-mean estimator, training "
+                                                           "time included)",
+                           "E0": r["mean"], "sigma": se_cm, "z": abs(r["mean"] - gold["E_virial"]) / se_cm,
+                           "tau_int_measurements": r["tau_int"], "seconds_train": t_train,
+                           "seconds_sampling": t_run, "adam_steps": learned_train, "nll_first": first,
+                           "nll_last": last, "acceptance_untrained": ar0, "acceptance_trained": lp.lastAcceptance,
+                           "ms_per_learned_sweep": ms_sweep,
+                           "ms_per_plain_sweep": plain["seconds"] / plain["sweeps"] * 1e3,
+                           "mfma": {"kernel": "fs_lp_sweep", "achieved": flop_upd * AHO_UPDATES_PER_SWEEP /
+                                    (ms_sweep * 1e-3), "peak": MFMA_BF16_PEAK, "unit": "FLOP/s",
+                                    "frac": flop_upd * AHO_UPDATES_PER_SWEEP / (ms_sweep * 1e-3) / MFMA_BF16_PEAK},
+                           "ess_per_s_ratio_learned_over_plain": (ess_cm / dt) / plain["value"],
+                           "ess_per_s_ratio_learned_over_plain_S6": (ess_cm / dt) / plain6["value"]}
+        del kl, lp
+        torch.cuda.empty_cache()
+    if cpu and int(os.environ.get("RANK", "0")) == 0:
+        line["cpu_baseline"] = cpu_baseline_aho()
+    return line
+
+
+def cpu_baseline_aho(target_s=12.0):
+    """Reference-order CPU restatement (oracle) of the anharmonic workload, bounded sample."""
+    import numpy as np
+    from oracle import oracle as O
+    from feynsimul_amd.physical_systems import AnharmOsc
+    system = AnharmOsc(AHO_LAM)
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    lib = O.build(system, AHO_N, AHO_BETA, operators=(system.energyOp,), native=True)
+    rng = np.random.default_rng(1)
+    walkers = max(cores * 4, 16)
+    sweeps = 2
+    while True:
+        paths = np.ascontiguousarray(rng.uniform(-1, 1, (walkers, AHO_N)).astype(np.float32))
+        cfg = O.make_cfg(AHO_N, AHO_S, 1, walkers, AHO_BETA, 1, sweeps, nOps=1)
+        seeds = O.xorshift_rows(3, 0, cfg.nThreads + 1)
+        t0 = time.perf_counter()
+        O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=cores)
+        dt = time.perf_counter() - t0
+        if dt > target_s / 2 or sweeps >= 1 << 14:
+            break
+        sweeps = max(sweeps * 2, int(sweeps * target_s / max(dt, 1e-3)))
+    upd = walkers * (AHO_N // AHO_SEG) * (AHO_SEG - 1) * sweeps
+    return {"value": upd / dt, "unit": "slice-updates/s", "cores": cores, "kind": "port",
+            "sample": "%d chains x %d sweeps of the same workload (no correlator), reference work-item order, "
+                      "FP32, xorshift128, gcc -O3 -march=native -fopenmp, %.1f s" % (walkers, sweeps, dt)}
+
+
 def cpu_baseline(target_s=12.0):
     """Reference-order CPU restatement (oracle) on this host's cores, bounded sample."""
     import numpy as np
@@ -258,9 +482,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the ESS/s phase")
-    ap.add_argument("--workload", choices=["he3", "ragged"], default="he3",
+    ap.add_argument("--workload", choices=["he3", "ragged", "aho"], default="he3",
                     help="he3: the headline workload; ragged: LM2M2 n in {3,4,6,8} x N in {256..2048}, "
-                         "2048 chains each, in one autotuned persistent launch")
+                         "2048 chains each, in one autotuned persistent launch; aho: anharmonic oscillator "
+                         "N=4096, beta=40, 65536 chains, 512 correlator lags, xorshift128 and RANLUX")
     ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the ESS phase")
     ap.add_argument("--ess-launches", type=int, default=64, help="ESS launches of 16 x 32 sweeps")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal phase")
@@ -288,6 +513,10 @@ def main():
         return
     if args.workload == "ragged":
         print(json.dumps(ragged_bench(args.steps, args.warmup)), flush=True)
+        return
+    if args.workload == "aho":
+        print(json.dumps(aho_bench(args.steps, args.warmup, learned=not args.no_learned,
+                                   cpu=not args.no_cpu)), flush=True)
         return
     system, k = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
     rng = np.random.default_rng(0)

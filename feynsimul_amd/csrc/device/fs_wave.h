@@ -41,6 +41,29 @@ FS_DEV double fs_readlane(double v, int l)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Element i of an array whose base is wave-uniform, addressed with a 32-bit
+// unsigned byte offset: the load or store then takes the base in SGPRs and the
+// offset in one VGPR, with no 64-bit address arithmetic per lane.
+template <typename T> FS_DEV T &fs_at(T *base, int i)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (unsigned)i * (unsigned)sizeof(T));
+}
+template <typename T>
+FS_DEV __attribute__((address_space(1))) T &fs_at(__attribute__((address_space(1))) T *base, int i)
+{
+    return *(__attribute__((address_space(1))) T *)((__attribute__((address_space(1))) char *)base +
+                                                      (unsigned)i * (unsigned)sizeof(T));
+}
+
+// A wave-uniform value moved to scalar registers (the compiler cannot prove
+// that threadIdx.x / 64 or a value broadcast by a shuffle is uniform).
+FS_DEV long fs_uniform(long v)
+{
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)v);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)((unsigned long long)v >> 32));
+    return (long)(((unsigned long long)hi << 32) | lo);
+}
+
 // Sum over aligned groups of G lanes (xor butterfly); every lane of a group
 // ends with the same bits.
 template <int G, typename T> FS_DEV T fs_group_sum(T v)
@@ -94,8 +117,9 @@ FS_DEV void fs_cor_accumulate(F (&acc)[CT], int nt, BC bc, WIN win)
 }
 
 template <class Sys>
-FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char *lds)
+FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, char *lds)
 {
+    const long w = fs_uniform(wIn);
     typedef FLOAT_TYPE F;
     constexpr int N = Sys::N, S = Sys::S, DOF = Sys::DOF;
     constexpr int SEG = 1 << S;
@@ -109,6 +133,12 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
     static_assert(!Sys::RESIDENT || (N >= 64 && NB <= 16), "register storage needs 64 <= N <= 1024");
 
     F *path = reinterpret_cast<F *>(a.paths) + w * (long)(N * DOF);
+    // one scalar base per coordinate row (opaque to the compiler, so a row access
+    // stays base + 32-bit lane offset instead of 64-bit per-lane arithmetic)
+    __attribute__((address_space(1))) F *row[DOF];
+#pragma unroll
+    for (int d = 0; d < DOF; d++)
+        row[d] = (__attribute__((address_space(1))) F *)fs_uniform((long)(path + d * N));
     const bool active = lane < B;
     F sigtab[S + 1];
 #pragma unroll
@@ -148,7 +178,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
         for (int k = 0; k < NB; k++) {
             F x[DOF];
 #pragma unroll
-            for (int d = 0; d < DOF; d++) x[d] = path[d * N + k * 64 + lane];
+            for (int d = 0; d < DOF; d++) x[d] = fs_at(path, d * N + k * 64 + lane);
 #pragma unroll
             for (int d = 0; d < DOF; d++) xs[d][k] = x[d];
             vc[k] = Sys::V(x);
@@ -159,8 +189,8 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
         for (int k = 0; k < NB; k++) {
             F x[DOF];
 #pragma unroll
-            for (int d = 0; d < DOF; d++) x[d] = path[d * N + k * B + lane];
-            pot[k * B + lane] = Sys::V(x);
+            for (int d = 0; d < DOF; d++) x[d] = fs_at(path, d * N + k * B + lane);
+            fs_at(pot, k * B + lane) = Sys::V(x);
         }
     }
 
@@ -202,7 +232,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
             int snext = (lane + (wrap ? B : 0)) & (N - 1);
             if constexpr (!Sys::RESIDENT) {
 #pragma unroll
-                for (int d = 0; d < DOF; d++) xnext[d] = path[d * N + snext];
+                for (int d = 0; d < DOF; d++) xnext[d] = fs_at(row[d], snext);
             }
 #pragma unroll 1
             for (int g = 0; g < NB; g++) {
@@ -225,10 +255,10 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
                     for (int d = 0; d < DOF; d++) xo[d] = xnext[d];
                     snext = ((g + 1) * B + lane + (wrap ? B : 0)) & (N - 1);
 #pragma unroll
-                    for (int d = 0; d < DOF; d++) xnext[d] = path[d * N + snext];
+                    for (int d = 0; d < DOF; d++) xnext[d] = fs_at(row[d], snext);
 #pragma unroll
                     for (int d = 0; d < DOF; d++) xr[d] = fs_readlane(xnext[d], r);
-                    vOld = pot[s];
+                    vOld = fs_at(pot, s);
                 }
                 // every lane draws the same count so buffered streams stay in step
                 F u[KDRAW];
@@ -279,8 +309,8 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
                     }
                 } else if (put) {
 #pragma unroll
-                    for (int d = 0; d < DOF; d++) path[d * N + s] = xn[d];
-                    pot[s] = vNew;
+                    for (int d = 0; d < DOF; d++) fs_at(row[d], s) = xn[d];
+                    fs_at(pot, s) = vNew;
                 }
             }
         }
@@ -300,7 +330,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
                         for (int d = 0; d < DOF; d++) x[d] = xs[d][k];
                     } else {
 #pragma unroll
-                        for (int d = 0; d < DOF; d++) x[d] = path[d * N + t];
+                        for (int d = 0; d < DOF; d++) x[d] = fs_at(path, d * N + t);
                     }
                     if constexpr (Sys::NOPS > 0) Sys::ops(x, part);
                     if constexpr (Sys::BINS) {
@@ -352,7 +382,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
                 auto cval = [&](int c, int s) -> F {
                     F x[DOF], prod[NCORR1];
 #pragma unroll
-                    for (int d = 0; d < DOF; d++) x[d] = path[d * N + s];
+                    for (int d = 0; d < DOF; d++) x[d] = fs_at(path, d * N + s);
 #pragma unroll
                     for (int q = 0; q < NCORR1; q++) prod[q] = (F)1.0;
                     Sys::corr(x, prod);
@@ -457,7 +487,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long w, const int lane, char
 #pragma unroll 1
         for (int k = 0; k < NB; k++) {
 #pragma unroll
-            for (int d = 0; d < DOF; d++) path[d * N + k * 64 + lane] = xs[d][k];
+            for (int d = 0; d < DOF; d++) fs_at(path, d * N + k * 64 + lane) = xs[d][k];
         }
     }
     const fs_u32 tot = fs_group_sum<64>(nacc);

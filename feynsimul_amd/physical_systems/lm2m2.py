@@ -23,6 +23,9 @@ _HBAR = 1.054571817e-34   # J s
 _KB = 1.380649e-23        # J / K
 _AMU = 1.66053906660e-27  # kg
 
+_LOG2E = 1.0 / math.log(2.0)
+_SQRT_LOG2E = math.sqrt(_LOG2E)
+
 # LM2M2 parameters (Aziz & Slaman 1991, table V)
 LM2M2 = dict(A=1.89635353e5, alpha=10.70203539, c6=1.34687065, c8=0.41308398, c10=0.17060159,
              D=1.4088, beta=-1.90740649, eps_k=10.97, rm=2.9695, Aa=0.0026,
@@ -66,16 +69,21 @@ def _user_code(qscale):
  * and the add-on A_a (sin(B (q - x_a1) - pi/2) + 1) = A_a (1 - cos(2 pi w)),
  * w = (q - x_a1) / (x_a2 - x_a1), is selected on [x_a1, x_a2]. */
 FLOAT_TYPE fs_sqr(FLOAT_TYPE a) { return a * a; }
-FLOAT_TYPE fs_lm2m2_q(FLOAT_TYPE q)
+/* The pair energy at squared distance r2: one reciprocal square root gives
+ * both 1/q and q, the exponentials are base 2 with log2(e) folded into their
+ * coefficients, and eps/k is folded into every term (same function). */
+FLOAT_TYPE fs_lm2m2_pair(FLOAT_TYPE r2)
 {
-    const FLOAT_TYPE iq = 1.0 / q;
+    const FLOAT_TYPE s = 1.0 / sqrt(r2);
+    const FLOAT_TYPE q = r2 * s * %(qscale)r;
+    const FLOAT_TYPE iq = s * %(iqscale)r;
     const FLOAT_TYPE iq2 = iq * iq;
-    const FLOAT_TYPE disp = iq2 * iq2 * iq2 * (%(c6)r + iq2 * (%(c8)r + iq2 * %(c10)r));
-    const FLOAT_TYPE t = fmax(%(D)r * iq - 1.0, 0.0);
-    const FLOAT_TYPE damp = q > %(qmin)r ? exp(-t * t) : 0.0;
+    const FLOAT_TYPE disp = iq2 * iq2 * iq2 * (%(ec6)r + iq2 * (%(ec8)r + iq2 * %(ec10)r));
+    const FLOAT_TYPE t = fmax(%(Dl)r * iq - %(sl)r, 0.0);
+    const FLOAT_TYPE damp = q > %(qmin)r ? exp2(-t * t) : 0.0;
     const FLOAT_TYPE w = (q - %(xa1)r) * %(iwidth)r;
-    const FLOAT_TYPE va = (w >= 0.0 && w <= 1.0) ? %(Aa)r * (1.0 - fs_cos2pi(w)) : 0.0;
-    return %(eps_k)r * (%(A)r * exp(q * (%(beta)r * q - %(alpha)r)) - damp * disp + va);
+    const FLOAT_TYPE va = (w >= 0.0 && w <= 1.0) ? %(eAa)r * (1.0 - fs_cos2pi(w)) : 0.0;
+    return %(eA)r * exp2(q * (%(betal)r * q - %(alphal)r)) - damp * disp + va;
 }
 /* V(q) + r dV/dr / 2 in K, the pair term of the virial energy estimator */
 FLOAT_TYPE fs_lm2m2_virial_q(FLOAT_TYPE q)
@@ -97,9 +105,12 @@ FLOAT_TYPE fs_lm2m2_virial_q(FLOAT_TYPE q)
     const FLOAT_TYPE qdv = rep * q * (2.0 * %(beta)r * q - %(alpha)r) - (qdamp * disp + damp * qdisp) + qva;
     return %(eps_k)r * (v + 0.5 * qdv);
 }
-FLOAT_TYPE fs_lm2m2_r2(FLOAT_TYPE r2) { return fs_lm2m2_q(sqrt(r2) * %(qscale)r); }
+FLOAT_TYPE fs_lm2m2_r2(FLOAT_TYPE r2) { return fs_lm2m2_pair(r2); }
 FLOAT_TYPE fs_lm2m2_virial_r2(FLOAT_TYPE r2) { return fs_lm2m2_virial_q(sqrt(r2) * %(qscale)r); }
-""" % dict(p, B=B, iwidth=1.0 / (p["xa2"] - p["xa1"]), qmin=0.0005 / p["rm"], qscale=qscale)
+""" % dict(p, B=B, iwidth=1.0 / (p["xa2"] - p["xa1"]), qmin=0.0005 / p["rm"], qscale=qscale,
+                  iqscale=1.0 / qscale, ec6=p["eps_k"] * p["c6"], ec8=p["eps_k"] * p["c8"],
+                  ec10=p["eps_k"] * p["c10"], Dl=p["D"] * _SQRT_LOG2E, sl=_SQRT_LOG2E, eAa=p["eps_k"] * p["Aa"],
+                  eA=p["eps_k"] * p["A"], betal=p["beta"] * _LOG2E, alphal=p["alpha"] * _LOG2E)
 
 
 class Lm2m2Cluster:

@@ -376,6 +376,21 @@ def transfer_matrix_gap(V, N: int, beta: float, L: float = 8.0, M: int = 1200) -
     return float(np.log(w[0] / w[1]) / eps)
 
 
+def transfer_matrix_density(V, N: int, beta: float, L: float = 8.0, M: int = 1200):
+    """(x, p): grid of [-L, L] and the one-slice density of the discretised action,
+    diag(K^N) / tr(K^N) with the transfer matrix K of transfer_matrix_1d, so that the
+    mean of any slice function f is sum(p * f(x))."""
+    eps = beta / N
+    x = np.linspace(-L, L, M)
+    h = x[1] - x[0]
+    v = V(x)
+    K = np.exp(-(x[:, None] - x[None, :]) ** 2 / (2 * eps) - eps * (v[:, None] + v[None, :]) / 2) * h
+    w, U = np.linalg.eigh(K)
+    lam = (np.clip(w, 0.0, None) / w.max()) ** N
+    diag = (U ** 2) @ lam
+    return x, diag / lam.sum()
+
+
 def transfer_matrix_1d(V, N: int, beta: float, L: float = 8.0, M: int = 1200):
     """FP64 transfer-matrix quadrature of the primitive action for a 1D potential.
 

@@ -23,6 +23,7 @@ What fixes each oracle function, so that a plausible mistake fails a check:
 Parity unpinned: none of the functions above.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -324,6 +325,35 @@ def test_transfer_matrix_reproduces_ho():
     x2, _, cor = O.transfer_matrix_1d(lambda x: 0.5 * x * x, 32, 6.0)
     assert abs(x2 - O.ho_discrete_x2(32, 6.0)) < 1e-9
     np.testing.assert_allclose(cor, O.ho_discrete_correlator(32, 6.0)[:17], atol=1e-9)
+
+
+def test_transfer_matrix_density_reproduces_ho():
+    """The slice density of the HO action is the Gaussian of variance <x^2>_N; its x^4
+    moment is 3 <x^2>_N^2."""
+    x, p = O.transfer_matrix_density(lambda x: 0.5 * x * x, 64, 8.0, L=7.0, M=900)
+    v = O.ho_discrete_x2(64, 8.0)
+    assert abs(p.sum() - 1.0) < 1e-12
+    assert abs(np.sum(p * x * x) - v) < 1e-9
+    assert abs(np.sum(p * x ** 4) - 3 * v * v) < 1e-8
+
+
+def test_aho_c5_golden_fixture():
+    """tests/golden/aho_c5.json (N = 4096, beta = 40): grid-converged (1000 vs 1400 points),
+    consistent (C(0) = <x^2>), and within O(eps^2) of the continuum ground state of
+    x^2/2 + 0.1 x^4 (E0 = 0.559146, E1 - E0 = 1.210348; beta = 40 suppresses excited
+    states by exp(-48))."""
+    import json
+    with open(os.path.join(os.path.dirname(__file__), "golden", "aho_c5.json")) as f:
+        g = json.load(f)
+    a, b = g["values"], g["values_M1000"]
+    assert abs(a["E_virial"] - b["E_virial"]) < 1e-10 and abs(a["gap"] - b["gap"]) < 1e-8
+    assert abs(a["correlator"][0] - a["x2"]) < 1e-12
+    assert abs(a["E_virial"] - 0.559146) < 5e-5
+    assert abs(a["gap"] - 1.210348) < 1e-3
+    # the correlator decays with the gap between lags 100 and 300
+    c = np.array(a["correlator"])
+    rate = -np.log(c[300] / c[100]) / (200 * g["eps"])
+    assert abs(rate - a["gap"]) < 1e-3 * a["gap"]
 
 
 def test_lm2m2_literature_values():
