@@ -82,6 +82,36 @@ def initial_paths(system, rng, walkers=W, sigma=3.0):
     return np.repeat(pos[:, :, None], N, axis=2).reshape(walkers, DOF * N)
 
 
+HE3_GOLDEN = os.path.join(ROOT, "tests", "golden", "he3_ensemble_fp64.json")
+HE3_GOLDEN_WALKERS = os.path.join(ROOT, "tests", "golden", "he3_ensemble_fp64_walkers.npz")
+
+
+def he3_oracle(per_walker, sweeps):
+    """Error of the ensemble means after `sweeps` sweeps from the benchmark's initial paths against the
+    FP64 CPU reference-order run of the same 16384 initial paths (tests/golden), when it holds that
+    sweep count: combined standard error, and walker by walker where its per-walker values are kept."""
+    import numpy as np
+    with open(HE3_GOLDEN) as f:
+        g = json.load(f)
+    row = next((r for r in g["rows"] if r["sweeps"] == sweeps), None)
+    if row is None:
+        return {"sweeps": sweeps, "note": "no FP64 reference at this sweep count (warmup + steps = 23 has one)"}
+    out = {"kind": "FP64 CPU run of the reference-order sampler from the same initial paths (tests/golden)",
+           "sweeps": sweeps}
+    for col, key in ((0, "E_K"), (1, "pair_distance_A")):
+        v = np.asarray(per_walker[col], dtype=np.float64)
+        m, se = v.mean(), v.std(ddof=1) / np.sqrt(len(v))
+        ref, ref_se = row[key], row[key.replace("_K", "_se_K").replace("_A", "_se_A")]
+        d = {"gpu": m, "gpu_se": se, "fp64": ref, "fp64_se": ref_se, "z": abs(m - ref) / np.hypot(se, ref_se)}
+        if sweeps in g.get("paired_sweeps", []) and len(v) == g["walkers"]:
+            dw = v - np.load(HE3_GOLDEN_WALKERS)["%s_T%d" % (key, sweeps)]
+            d["paired_difference"] = float(dw.mean())
+            d["paired_se"] = float(dw.std(ddof=1) / np.sqrt(len(dw)))
+            d["paired_z"] = abs(d["paired_difference"]) / d["paired_se"]
+        out[key] = d
+    return out
+
+
 def chain_mean_ess(series):
     """ESS = Var_1 / SE^2: single-measurement variance over the variance of the grand mean,
     the latter from the spread of the independent chains' means (the reference's error
@@ -447,6 +477,68 @@ def cpu_baseline_aho(target_s=12.0):
                       "FP32, xorshift128, gcc -O3 -march=native -fopenmp, %.1f s" % (walkers, sweeps, dt)}
 
 
+# ---- random streams: the reference's RNG benchmark ----
+# examples/benchmark_xorshift_vs_ranlux.py:12-26: seconds for 1e7 uniforms per thread on
+# 448 * 32 * 4 = 57,344 threads (single precision; RANLUX also double), hardware unstated
+REF_RNG_THREADS, REF_RNG_DRAWS = 448 * 32 * 4, 10_000_000
+REF_RNG_SECONDS = {("xorshift", None, "f32"): 7.748644,
+                   ("ranlux", 0, "f32"): 12.359268, ("ranlux", 0, "f64"): 23.163764,
+                   ("ranlux", 1, "f32"): 18.334329, ("ranlux", 1, "f64"): 34.851969,
+                   ("ranlux", 2, "f32"): 36.360428, ("ranlux", 2, "f64"): 71.408121,
+                   ("ranlux", 3, "f32"): 67.230933, ("ranlux", 3, "f64"): 133.259846,
+                   ("ranlux", 4, "f32"): 112.292905, ("ranlux", 4, "f64"): 224.170527}
+
+
+def rng_bench(streams=REF_RNG_THREADS, draws=None):
+    """Uniforms/s of every generator the reference times, through the samplers' wrappers
+    (fs_rng_bench), on the reference's 57,344 streams and on a device-filling 1,048,576."""
+    import numpy as np
+    import torch
+    from feynsimul_amd import _native
+    L = _native.lib()
+    dev = torch.device("cuda", 0)
+    rows_out = []
+
+    def run(kind, lux, dbl, n, count):
+        if kind == "xorshift":
+            st = torch.zeros((n, 4), dtype=torch.int32, device=dev)
+            _native.check(L.fs_seed_xorshift(st.data_ptr(), n, 2026, 0, None))
+        else:
+            seeds = torch.arange(1, n + 1, dtype=torch.int64, device=dev) * 0x9E3779B97F4A7C15 % (1 << 62)
+            st = torch.zeros((n, 28), dtype=torch.int32, device=dev)
+            _native.check(L.fs_ranlux_init(st.data_ptr(), seeds.data_ptr(), n, None))
+        chk = torch.zeros(n, dtype=torch.int32, device=dev)
+        k = _native.FS_RNG_XORSHIFT if kind == "xorshift" else _native.FS_RNG_RANLUX
+        _native.check(L.fs_rng_bench(k, lux or 0, int(dbl), st.data_ptr(), chk.data_ptr(), n, 1024, None))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        _native.check(L.fs_rng_bench(k, lux or 0, int(dbl), st.data_ptr(), chk.data_ptr(), n, count, None))
+        e1.record()
+        torch.cuda.synchronize()
+        return n * count / (e0.elapsed_time(e1) * 1e-3)
+
+    for (kind, lux, prec), ref_s in REF_RNG_SECONDS.items():
+        ref = REF_RNG_THREADS * REF_RNG_DRAWS / ref_s
+        # draws per stream sized to ~0.2-2 s per launch
+        count = draws or (REF_RNG_DRAWS if kind == "xorshift" else REF_RNG_DRAWS // 10)
+        v_ref_shape = run(kind, lux, prec == "f64", streams, count)
+        v_full = run(kind, lux, prec == "f64", 1 << 20, max(count // 16, 1024))
+        rows_out.append({"generator": kind if lux is None else "ranlux luxury %d" % lux, "precision": prec,
+                         "uniforms_per_s": v_ref_shape, "uniforms_per_s_1M_streams": v_full,
+                         "reference_uniforms_per_s": ref, "vs_reference": v_ref_shape / ref,
+                         "vs_reference_1M_streams": v_full / ref})
+    head = rows_out[0]
+    return {"metric": "uniforms/s, xorshift128 FP32 through the sampler's wrapper (the reference's RNG benchmark)",
+            "value": head["uniforms_per_s"], "unit": "uniforms/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": REF_RNG_THREADS * REF_RNG_DRAWS / head["uniforms_per_s"] * 1e3,
+            "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": head["vs_reference"], "dtype": "u32/f32", "data": "synthetic: splitmix64 / ranluxcl_init seeds",
+            "config": {"workload": "57,344 streams x 1e7 uniforms (RANLUX: 1e6), XOR checksum per stream, "
+                                   "as examples/benchmark_xorshift_vs_ranlux.py times it",
+                       "reference_hardware": "unstated in the reference"},
+            "generators": rows_out}
+
+
 def cpu_baseline(target_s=12.0):
     """Reference-order CPU restatement (oracle) on this host's cores, bounded sample."""
     import numpy as np
@@ -482,10 +574,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the ESS/s phase")
-    ap.add_argument("--workload", choices=["he3", "ragged", "aho"], default="he3",
+    ap.add_argument("--workload", choices=["he3", "ragged", "aho", "rng"], default="he3",
                     help="he3: the headline workload; ragged: LM2M2 n in {3,4,6,8} x N in {256..2048}, "
                          "2048 chains each, in one autotuned persistent launch; aho: anharmonic oscillator "
-                         "N=4096, beta=40, 65536 chains, 512 correlator lags, xorshift128 and RANLUX")
+                         "N=4096, beta=40, 65536 chains, 512 correlator lags, xorshift128 and RANLUX; "
+                         "rng: stream throughput of every generator, as the reference's RNG benchmark")
     ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the ESS phase")
     ap.add_argument("--ess-launches", type=int, default=64, help="ESS launches of 16 x 32 sweeps")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal phase")
@@ -514,6 +607,9 @@ def main():
     if args.workload == "ragged":
         print(json.dumps(ragged_bench(args.steps, args.warmup)), flush=True)
         return
+    if args.workload == "rng":
+        print(json.dumps(rng_bench()), flush=True)
+        return
     if args.workload == "aho":
         print(json.dumps(aho_bench(args.steps, args.warmup, learned=not args.no_learned,
                                    cpu=not args.no_cpu)), flush=True)
@@ -538,7 +634,8 @@ def main():
     updates = UPDATES_PER_SWEEP * SWEEPS_PER_STEP
     value = updates * args.steps / wall
     achieved = BYTES_PER_UPDATE * updates / kern_s / 1e9
-    E = float(k.getOperators()[0].mean())
+    per_walker = k.getOperators()
+    E = float(per_walker[0].mean())
     line = {
         "metric": "path-slice updates/s (plain bisection sampler)",
         "value": value,
@@ -566,6 +663,7 @@ def main():
                  "peak": VALU_PEAK, "frac": VALU_PER_UPDATE * updates / kern_s / VALU_PEAK},
         "acceptance": k.getAcceptanceRate(),
         "energy_K": E,
+        "oracle": he3_oracle(per_walker, (args.warmup + args.steps) * SWEEPS_PER_STEP),
     }
     if not args.no_ess:
         line["ess"] = ess_phase(k.getPaths(), args.storage, args.ess_burn, args.ess_launches)

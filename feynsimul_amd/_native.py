@@ -12,13 +12,14 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfeynsimu
 FS_ABI_VERSION = 2
 FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
+FS_RNG_XORSHIFT, FS_RNG_RANLUX = 0, 1
 FS_STORAGE_AUTO, FS_STORAGE_MEMORY, FS_STORAGE_REGISTERS = 0, 1, 2
 
 EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_path_storage",
            "fs_batch_create", "fs_batch_destroy", "fs_batch_path_storage", "fs_batch_build_log", "fs_batch_run",
            "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
            "fs_source", "fs_run", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
-           "fs_ranlux_draw", "fs_lp_create", "fs_lp_destroy", "fs_lp_get_info", "fs_lp_build_log", "fs_lp_pack",
+           "fs_ranlux_draw", "fs_rng_bench", "fs_lp_create", "fs_lp_destroy", "fs_lp_get_info", "fs_lp_build_log", "fs_lp_pack",
            "fs_lp_sweep", "fs_lp_train", "fs_lp_eval", "fs_lp_grad")
 
 
@@ -105,6 +106,7 @@ def lib():
     L.fs_ranlux_init.argtypes = [vp, vp, i64, vp]
     L.fs_xorshift_draw.argtypes = [vp, vp, vp, vp, i64, i32, vp]
     L.fs_ranlux_draw.argtypes = [vp, vp, vp, i64, i32, i32, vp]
+    L.fs_rng_bench.argtypes = [i32, i32, i32, vp, vp, i64, i64, vp]
     L.fs_lp_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.c_char_p, ctypes.POINTER(FsLpConfig), ctypes.c_int,
                                ctypes.POINTER(vp)]
     L.fs_lp_destroy.argtypes = [vp]

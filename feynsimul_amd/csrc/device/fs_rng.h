@@ -255,8 +255,9 @@ FS_DEV void fs_rl_store(fs_ranlux *__restrict__ tab, long idx, const fs_ranlux &
 // Per-launch buffered draw (kernel.c:180-207): refill four values whenever the
 // launch-local counter is a multiple of four.
 template <typename F, int NSKIP> struct fs_rl_wrap {
+    typedef F vec4 __attribute__((ext_vector_type(4)));
     fs_ranlux st;
-    F buf[4];
+    vec4 buf;
     fs_u32 cnt;
 
     FS_DEV F next_float()
@@ -265,19 +266,23 @@ template <typename F, int NSKIP> struct fs_rl_wrap {
             if constexpr (sizeof(F) == 8) {
                 double v[4];
                 fs_rl_next4_double<NSKIP>(st, v);
-                buf[0] = v[0]; buf[1] = v[1]; buf[2] = v[2]; buf[3] = v[3];
+                buf = vec4{v[0], v[1], v[2], v[3]};
             } else {
                 float v[4];
                 fs_rl_next4<NSKIP>(st, v);
-                buf[0] = v[0]; buf[1] = v[1]; buf[2] = v[2]; buf[3] = v[3];
+                buf = vec4{v[0], v[1], v[2], v[3]};
             }
         }
         const fs_u32 k = (cnt++) & 3u;
-        // select rather than index so the buffer stays in registers
-        F v = buf[0];
-        v = (k == 1u) ? buf[1] : v;
-        v = (k == 2u) ? buf[2] : v;
-        v = (k == 3u) ? buf[3] : v;
+        // selects on the buffer's value: an index into the buffer in memory
+        // (or a select between its addresses, which the optimiser may form
+        // from selects between loads) would keep the whole stream state in
+        // scratch memory
+        const vec4 b = buf;
+        F v = b.x;
+        v = (k == 1u) ? b.y : v;
+        v = (k == 2u) ? b.z : v;
+        v = (k == 3u) ? b.w : v;
         return v;
     }
 

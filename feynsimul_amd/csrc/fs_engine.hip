@@ -917,6 +917,29 @@ int fs_ranlux_draw(void *tab, void *out_f32, void *out_f64, int64_t n, int32_t n
     return FS_OK;
 }
 
+// Stream throughput: `count` uniforms per stream through the sampler's
+// wrappers, XOR of their bits per stream in check[n]; states advance.
+int fs_rng_bench(int32_t kind, int32_t lux, int32_t double_precision, void *states, void *check, int64_t n,
+                 int64_t count, void *stream)
+{
+    if (!states || !check || n < 0 || count < 0 || (kind != FS_RNG_XORSHIFT && kind != FS_RNG_RANLUX) ||
+        (kind == FS_RNG_RANLUX && (lux < 0 || lux > 4)))
+        return fail(FS_ERR_INVALID, "bad stream benchmark request");
+    if (int rc = need_device()) return rc;
+    if (n == 0) return FS_OK;
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (kind == FS_RNG_XORSHIFT)
+        hipLaunchKernelGGL(fs_k_xs_bench, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           static_cast<uint4 *>(states), static_cast<unsigned *>(check), (long long)n,
+                           (long long)count, (int)double_precision);
+    else
+        hipLaunchKernelGGL(fs_k_rl_bench, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           static_cast<fs_ranlux *>(states), static_cast<unsigned *>(check), (long long)n,
+                           (long long)count, (int)lux, (int)double_precision);
+    FS_HIP(hipGetLastError());
+    return FS_OK;
+}
+
 }  // extern "C"
 
 #include "fs_lp_host.inc"

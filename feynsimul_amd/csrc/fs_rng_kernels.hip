@@ -81,3 +81,72 @@ extern "C" __global__ void fs_k_rl_draw(fs_ranlux *__restrict__ tab, float *__re
     default: fs_rl_draw_one<fs_ranlux_nskip(4)>(tab, out32, out64, i, nquad); break;
     }
 }
+
+// ---- stream throughput (the reference's xorshift_test_kernel and
+// ranlux_test_kernel, examples/ranlux_test_kernel.c:90-200) ----
+// Every stream draws `count` uniforms through the sampler's wrapper (RANLUX:
+// buffered float4, counter reset at launch start, kernel.c:180-207) and keeps
+// the XOR of their bit patterns instead of storing them; the stream state
+// advances.  One stream per lane.
+template <typename F> __device__ __forceinline__ fs_u32 fs_bits_of(F v)
+{
+    if constexpr (sizeof(F) == 8) {
+        const fs_u64 b = (fs_u64)__double_as_longlong(v);
+        return (fs_u32)b ^ (fs_u32)(b >> 32);
+    } else {
+        return __float_as_uint(v);
+    }
+}
+
+template <typename F>
+__device__ void fs_xs_bench_one(uint4 *rows, fs_u32 *check, long long i, long long count)
+{
+    const uint4 v = rows[i];
+    fs_xs128 s{v.x, v.y, v.z, v.w};
+    fs_u32 acc = 0u;
+    for (long long k = 0; k < count; k++) acc ^= fs_bits_of(fs_xs_float<F>(s));
+    rows[i] = make_uint4(s.x, s.y, s.z, s.w);
+    check[i] = acc;
+}
+
+template <typename F, int NSKIP>
+__device__ void fs_rl_bench_one(fs_ranlux *tab, fs_u32 *check, long long i, long long count)
+{
+    fs_rl_wrap<F, NSKIP> g;
+    fs_rl_load(tab, i, g.st);
+    g.cnt = 0u;
+    fs_u32 acc = 0u;
+    for (long long k = 0; k < count; k++) acc ^= fs_bits_of(g.next_float());
+    fs_rl_store(tab, i, g.st);
+    check[i] = acc;
+}
+
+template <typename F>
+__device__ void fs_rl_bench_lux(fs_ranlux *tab, fs_u32 *check, long long i, long long count, int lux)
+{
+    switch (lux) {
+    case 0: fs_rl_bench_one<F, fs_ranlux_nskip(0)>(tab, check, i, count); break;
+    case 1: fs_rl_bench_one<F, fs_ranlux_nskip(1)>(tab, check, i, count); break;
+    case 2: fs_rl_bench_one<F, fs_ranlux_nskip(2)>(tab, check, i, count); break;
+    case 3: fs_rl_bench_one<F, fs_ranlux_nskip(3)>(tab, check, i, count); break;
+    default: fs_rl_bench_one<F, fs_ranlux_nskip(4)>(tab, check, i, count); break;
+    }
+}
+
+extern "C" __global__ void fs_k_xs_bench(uint4 *__restrict__ rows, fs_u32 *__restrict__ check, long long n,
+                                         long long count, int dbl)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (dbl) fs_xs_bench_one<double>(rows, check, i, count);
+    else fs_xs_bench_one<float>(rows, check, i, count);
+}
+
+extern "C" __global__ void fs_k_rl_bench(fs_ranlux *__restrict__ tab, fs_u32 *__restrict__ check, long long n,
+                                         long long count, int lux, int dbl)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (dbl) fs_rl_bench_lux<double>(tab, check, i, count, lux);
+    else fs_rl_bench_lux<float>(tab, check, i, count, lux);
+}

@@ -18,7 +18,7 @@
  *   fs_seed_xorshift              <- seed table fill, kernel.py:421-426
  *   fs_ranlux_init                <- initRanlux / groupInitRanlux,
  *                                    kernel.py:508-554, kernel.c:152-166
- *   fs_xorshift_draw, fs_ranlux_draw, fs_eval_system
+ *   fs_xorshift_draw, fs_ranlux_draw, fs_rng_bench, fs_eval_system
  *                                 <- stream and expression dumps of
  *                                    examples/ranlux_test_kernel.c:50-212
  *                                    (test support)
@@ -154,6 +154,18 @@ int fs_xorshift_draw(const void *rows, void *out_u32, void *out_f32, void *out_f
                      void *stream);
 /* nquad four-value RANLUX draws per state (advancing the states); lux 0..4. */
 int fs_ranlux_draw(void *tab, void *out_f32, void *out_f64, int64_t n, int32_t nquad, int32_t lux, void *stream);
+/* Stream throughput, as the reference's RNG benchmark kernels
+ * (examples/ranlux_test_kernel.c:90-200, timings in
+ * examples/benchmark_xorshift_vs_ranlux.py:12-26): each of n streams draws
+ * `count` uniforms through the samplers' wrappers (xorshift128 randFloat,
+ * kernel.c:233-237; RANLUX ranluxWrapper with its per-launch float4 buffer,
+ * kernel.c:180-207), FP32 or FP64, and writes the XOR of their bit patterns
+ * to check[i] (u32) instead of the values; the states advance.  kind is
+ * FS_RNG_XORSHIFT (states: uint32 rows [n][4]) or FS_RNG_RANLUX (states:
+ * 112-byte RANLUX states [n], lux 0..4). */
+enum { FS_RNG_XORSHIFT = 0, FS_RNG_RANLUX = 1 };
+int fs_rng_bench(int32_t kind, int32_t lux, int32_t double_precision, void *states, void *check, int64_t n,
+                 int64_t count, void *stream);
 
 /* ---- batched systems: one persistent launch over several configurations ----
  * No reference counterpart as a call: the reference compiles one program per

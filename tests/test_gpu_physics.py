@@ -246,3 +246,55 @@ def test_jacobi_cluster_compiles_evaluates_and_samples(gpu):
     E = k.getOperators()[0].mean()
     print("AR %.3f, E %.4f K" % (ar, E))
     assert 0.05 < ar < 0.99 and np.isfinite(E)
+
+
+def test_he3_ensemble_vs_fp64_oracle(gpu):
+    """He3 (LM2M2, N = 1024, beta = 10/K, S = 6): the wave sampler in FP32 over 16384 chains against
+    the FP64 CPU run of the reference-order sampler over the same 16384 initial paths
+    (tests/golden/he3_ensemble_fp64.json).  The trimer dissociates slowly at this temperature, so the
+    comparison is of the ensemble means after T sweeps, T = 32 .. 736: virial energy and mean pair
+    distance, each within 4 combined standard errors, and walker by walker at T = 128 and 736 (both
+    sides start from the same 16384 initial paths)."""
+    import json
+    import os
+    from feynsimul_amd.physical_systems import Lm2m2Cluster
+    _case("He3 ensemble vs FP64 oracle")
+    with open(os.path.join(os.path.dirname(__file__), "golden", "he3_ensemble_fp64.json")) as f:
+        g = json.load(f)
+    system = Lm2m2Cluster(3)
+    W, N, S, beta = 16384, g["N"], g["S"], g["beta_per_K"]
+    pos = system.initialPositions(W, np.random.default_rng(0), side=3.5, sigma=3.0)
+    assert np.array_equal(pos[:4], system.initialPositions(4, np.random.default_rng(0), side=3.5, sigma=3.0))
+    ka = KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=S, enableOperator=True,
+                    enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                    enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                    enableGlobalOldPath=False, enableBins=False, metroStepsPerOperatorRun=32, operatorRuns=1,
+                    nbrOfWalkersPerWorkGroup=1, operators=(system.energyOp, system.meanPairDistanceOp),
+                    seed=99, sampler="wave")
+    k = PIMCKernel(ka)
+    k.setPaths(np.repeat(pos[:, :, None], N, axis=2).reshape(W, 9 * N))
+    last = g["rows"][-1]["sweeps"]
+    k.setSeriesCapacity(last // 32)
+    for _ in range(last // 32):
+        k.run(wait=False)
+    ser = k.getSeries()
+    # the same 16384 initial paths on both sides: walker-by-walker differences remove the spread of the
+    # initial ensemble from the error
+    walkers = np.load(os.path.join(os.path.dirname(__file__), "golden", "he3_ensemble_fp64_walkers.npz"))
+    worst = 0.0
+    for T in g["paired_sweeps"]:
+        v = ser[T // 32 - 1].astype(np.float64)
+        for col, key in ((0, "E_K"), (1, "pair_distance_A")):
+            d = v[:, col] - walkers["%s_T%d" % (key, T)]
+            z = abs(d.mean()) / (d.std(ddof=1) / np.sqrt(W))
+            print("T=%d %s paired: mean difference %.5f +- %.5f, z %.2f" % (T, key, d.mean(),
+                                                                          d.std(ddof=1) / np.sqrt(W), z))
+            worst = max(worst, z)
+    for row in g["rows"]:
+        v = ser[row["sweeps"] // 32 - 1].astype(np.float64)
+        for col, key in ((0, "E_K"), (1, "pair_distance_A")):
+            m, se = v[:, col].mean(), v[:, col].std(ddof=1) / np.sqrt(W)
+            z = abs(m - row[key]) / np.hypot(se, row[key.replace("_K", "_se_K").replace("_A", "_se_A")])
+            print("T=%d %s: GPU %.5f +- %.5f, FP64 oracle %.5f, z %.2f" % (row["sweeps"], key, m, se, row[key], z))
+            worst = max(worst, z)
+    assert worst < 4.0

@@ -101,3 +101,57 @@ def test_ranlux_double_bit_exact(gpu, olib):
     st = O.ranlux_init(olib, seeds)
     ref = O.ranlux_stream(olib, st, O.nskip_of_lux(2), 40, double=True)
     np.testing.assert_array_equal(out.cpu().numpy(), ref)
+
+
+def _xor_bits(vals):
+    v = np.ascontiguousarray(vals)
+    if v.dtype == np.float64:
+        b = v.view(np.uint64)
+        w = (b & np.uint64(0xFFFFFFFF)).astype(np.uint32) ^ (b >> np.uint64(32)).astype(np.uint32)
+    else:
+        w = v.view(np.uint32)
+    return np.bitwise_xor.reduce(w, axis=1)
+
+
+@pytest.mark.parametrize("dbl", [False, True])
+def test_stream_benchmark_xorshift_checksums(gpu, olib, olib64, dbl):
+    """fs_rng_bench (the reference's RNG timing kernel): the XOR of the bits of every stream's
+    `count` uniforms and the advanced rows equal the oracle's stream."""
+    import torch
+    from feynsimul_amd import _native
+    _case("stream benchmark xorshift f64=%d" % dbl)
+    n, count = 320, 1001
+    rows = O.xorshift_rows(8, 0, n)
+    d_rows = torch.from_numpy(rows.view(np.int32).copy()).to(gpu)
+    check = torch.zeros(n, dtype=torch.int32, device=gpu)
+    _native.check(_native.lib().fs_rng_bench(_native.FS_RNG_XORSHIFT, 0, int(dbl), d_rows.data_ptr(),
+                                             check.data_ptr(), n, count, None))
+    torch.cuda.synchronize()
+    ints, flt = O.xs_stream(olib64 if dbl else olib, rows, count)
+    np.testing.assert_array_equal(check.cpu().numpy().view(np.uint32), _xor_bits(flt))
+    # the rows advanced by `count` steps: their next word is the oracle's draw count + 1
+    nxt, _ = O.xs_stream(olib, rows, count + 1)
+    got_next, _ = O.xs_stream(olib, d_rows.cpu().numpy().view(np.uint32), 1)
+    np.testing.assert_array_equal(got_next[:, 0], nxt[:, count])
+
+
+@pytest.mark.parametrize("lux,dbl,count", [(0, False, 97), (2, False, 1000), (4, False, 250), (2, True, 130)])
+def test_stream_benchmark_ranlux_checksums(gpu, olib, lux, dbl, count):
+    """fs_rng_bench with RANLUX: the buffered wrapper draws ceil(count / 4) float4s (values left in the
+    buffer at the end are dropped, kernel.c:180-207); checksum of the first `count` values and the
+    advanced states equal the oracle's."""
+    import torch
+    from feynsimul_amd import _native
+    _case("stream benchmark ranlux lux %d f64=%d count %d" % (lux, dbl, count))
+    n = 192
+    seeds = O.ranlux_seeds(31, 0, n)
+    st = O.ranlux_init(olib, seeds)
+    tab = torch.from_numpy(st.view(np.int32).reshape(n, 28).copy()).to(gpu)
+    check = torch.zeros(n, dtype=torch.int32, device=gpu)
+    _native.check(_native.lib().fs_rng_bench(_native.FS_RNG_RANLUX, lux, int(dbl), tab.data_ptr(),
+                                             check.data_ptr(), n, count, None))
+    torch.cuda.synchronize()
+    nquad = (count + 3) // 4
+    ref = O.ranlux_stream(olib, st, O.nskip_of_lux(lux), nquad, double=dbl)
+    np.testing.assert_array_equal(check.cpu().numpy().view(np.uint32), _xor_bits(ref[:, :count]))
+    np.testing.assert_array_equal(tab.cpu().numpy().view(np.uint8).reshape(n, 112), st.view(np.uint8).reshape(n, 112))
