@@ -324,7 +324,7 @@ def aho_ess(paths, S, measurements, seed):
                "acceptance": k.getAcceptanceRate()}
 
 
-def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_train=200, cpu=True):
+def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_train=200, cpu=True, ess=True):
     """Anharmonic oscillator x^2/2 + 0.1 x^4, N = 4096, beta = 40, 65536 chains, S = 4: one step = 32
     sweeps + one measurement of the virial energy and the two-time correlator <x(0) x(tau)>,
     tau = 0..511 (FP64 Kahan bins); xorshift128 and RANLUX luxury 2; estimators against the FP64
@@ -386,6 +386,10 @@ def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_trai
                       "luxury": 2, "correlator_max_z": float((np.abs(mean_rl[0] - cor) / se_rl[0]).max())}
     del kr
     torch.cuda.empty_cache()
+    if not ess:
+        if cpu and int(os.environ.get("RANK", "0")) == 0:
+            line["cpu_baseline"] = cpu_baseline_aho()
+        return line
     # ESS/s of E0, plain sampler at S = 4 and at S = 6
     kp, plain = aho_ess(paths, AHO_S, 256, seed=7)
     plain["E0_exact"] = gold["E_virial"]
@@ -468,13 +472,118 @@ def cpu_baseline_aho(target_s=12.0):
         t0 = time.perf_counter()
         O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=cores)
         dt = time.perf_counter() - t0
-        if dt > target_s / 2 or sweeps >= 1 << 14:
+        if dt > target_s / 2 or sweeps >= 1 << 17:
             break
         sweeps = max(sweeps * 2, int(sweeps * target_s / max(dt, 1e-3)))
     upd = walkers * (AHO_N // AHO_SEG) * (AHO_SEG - 1) * sweeps
     return {"value": upd / dt, "unit": "slice-updates/s", "cores": cores, "kind": "port",
             "sample": "%d chains x %d sweeps of the same workload (no correlator), reference work-item order, "
                       "FP32, xorshift128, gcc -O3 -march=native -fopenmp, %.1f s" % (walkers, sweeps, dt)}
+
+
+# ---- harmonic oscillator: the small plain-method workload ----
+HO_N, HO_BETA, HO_W, HO_S = 256, 10.0, 4096, 4
+HO_SEG = 1 << HO_S
+HO_UPDATES_PER_SWEEP = HO_W * (HO_N // HO_SEG) * (HO_SEG - 1)
+
+
+def ho_exact_x2(N, beta):
+    """<x^2> (= the virial energy) of the discretised periodic HO action with m = omega = 1: the path's
+    normal modes a_k = (2 - 2 cos(2 pi k / N)) / eps + eps give <x^2> = (1/N) sum_k 1/a_k."""
+    import numpy as np
+    eps = beta / N
+    a = (2.0 - 2.0 * np.cos(2.0 * np.pi * np.arange(N) / N)) / eps + eps
+    return float(np.mean(1.0 / a))
+
+
+def ho_bench(steps, warmup, cpu=True):
+    """1D harmonic oscillator, N = 256, beta = 10, 4096 chains, S = 4, xorshift128, paths uniform on [-1, 1],
+    2000 burn-in sweeps, then steps of 32 sweeps + one measurement of the virial energy; ESS/s of E0 over
+    256 measurements against the exact discretised value 0.499950082044."""
+    import numpy as np
+    import torch
+    from feynsimul_amd.kernel_args import KernelArgs
+    from feynsimul_amd.kernel import PIMCKernel
+    from feynsimul_amd.physical_systems import HarmOsc
+    from feynsimul_amd.stats import effective_sample_size
+    system = HarmOsc()
+    exact = ho_exact_x2(HO_N, HO_BETA)
+
+    def kernel(steps_per_run, runs, seed):
+        return PIMCKernel(KernelArgs(system=system, nbrOfWalkers=HO_W, N=HO_N, beta=HO_BETA, S=HO_S,
+                                     enableOperator=True, enableCorrelator=False, enableBisection=True,
+                                     enablePathShift=False, enableSingleNodeMove=False, enableParallelizePath=True,
+                                     enableGlobalPath=False, enableGlobalOldPath=False, enableBins=False,
+                                     metroStepsPerOperatorRun=steps_per_run, operatorRuns=runs,
+                                     nbrOfWalkersPerWorkGroup=1, operators=(system.energyOp,), seed=seed,
+                                     sampler="wave"))
+    kb = kernel(500, 4, 1)
+    kb.setPaths(np.random.default_rng(0).uniform(-1, 1, (HO_W, HO_N)).astype(np.float32))
+    kb.run()
+    paths = kb.getPaths()
+    k = kernel(SWEEPS_PER_STEP, 1, 2)
+    k.setPaths(paths)
+    ms, wall = timed_launches(k, steps, warmup)
+    updates = HO_UPDATES_PER_SWEEP * SWEEPS_PER_STEP
+    bytes_upd = 8.0 * HO_SEG / (HO_SEG - 1)
+    ke = kernel(SWEEPS_PER_STEP, 64, 3)
+    ke.setPaths(k.getPaths())
+    ke.setSeriesCapacity(256)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(4):
+        ke.run(wait=False)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ser = ke.getSeries()[:, :, 0]
+    r = effective_sample_size(ser)
+    ess_cm, se_cm = chain_mean_ess(ser)
+    line = {"metric": "path-slice updates/s (plain bisection sampler)", "value": updates * steps / wall,
+            "unit": "slice-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic: paths uniform on [-1, 1], 2000 burn-in sweeps",
+            "config": {"workload": "harmonic oscillator, N=256, beta=10, 4096 chains, S=4, xorshift128, 32 sweeps "
+                                   "+ 1 measurement (virial energy) per step", "sampler": "wave"},
+            "roofline": {"bound": "hbm", "achieved": bytes_upd * updates / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": bytes_upd * updates / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": None, "kernel": "fs_wave_sweep", "avg_launch_ms": ms,
+                         "bytes_per_update": bytes_upd, "updates_per_launch": updates,
+                         "note": "the 4 MiB of paths stay in the caches; the VALU bound applies"},
+            "acceptance": k.getAcceptanceRate(),
+            "ess": {"value": ess_cm / dt, "unit": "effective samples/s (chain-mean estimator)",
+                    "ess_sokal_per_s": r["ess"] / dt, "tau_int_measurements": r["tau_int"], "E0": r["mean"],
+                    "sigma": se_cm, "E0_exact": exact, "z": abs(r["mean"] - exact) / se_cm, "seconds": dt,
+                    "sweeps": 256 * SWEEPS_PER_STEP, "seconds_to_sigma_1e-4": dt * (se_cm / 1e-4) ** 2}}
+    if cpu and int(os.environ.get("RANK", "0")) == 0:
+        line["cpu_baseline"] = cpu_baseline_ho()
+    return line
+
+
+def cpu_baseline_ho(target_s=12.0):
+    """Reference-order CPU restatement (oracle) of the harmonic-oscillator workload, bounded sample."""
+    import numpy as np
+    from oracle import oracle as O
+    from feynsimul_amd.physical_systems import HarmOsc
+    system = HarmOsc()
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    lib = O.build(system, HO_N, HO_BETA, operators=(system.energyOp,), native=True)
+    rng = np.random.default_rng(1)
+    walkers = max(cores * 16, 64)
+    sweeps = 16
+    while True:
+        paths = np.ascontiguousarray(rng.uniform(-1, 1, (walkers, HO_N)).astype(np.float32))
+        cfg = O.make_cfg(HO_N, HO_S, 1, walkers, HO_BETA, 1, sweeps, nOps=1)
+        seeds = O.xorshift_rows(3, 0, cfg.nThreads + 1)
+        t0 = time.perf_counter()
+        O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=cores)
+        dt = time.perf_counter() - t0
+        if dt > target_s / 2 or sweeps >= 1 << 18:
+            break
+        sweeps = max(sweeps * 2, int(sweeps * target_s / max(dt, 1e-3)))
+    upd = walkers * (HO_N // HO_SEG) * (HO_SEG - 1) * sweeps
+    return {"value": upd / dt, "unit": "slice-updates/s", "cores": cores, "kind": "port",
+            "sample": "%d chains x %d sweeps of the same workload, reference work-item order, FP32, xorshift128, "
+                      "gcc -O3 -march=native -fopenmp, %.1f s" % (walkers, sweeps, dt)}
 
 
 # ---- random streams: the reference's RNG benchmark ----
@@ -574,11 +683,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the ESS/s phase")
-    ap.add_argument("--workload", choices=["he3", "ragged", "aho", "rng"], default="he3",
+    ap.add_argument("--workload", choices=["he3", "ragged", "aho", "rng", "ho"], default="he3",
                     help="he3: the headline workload; ragged: LM2M2 n in {3,4,6,8} x N in {256..2048}, "
                          "2048 chains each, in one autotuned persistent launch; aho: anharmonic oscillator "
                          "N=4096, beta=40, 65536 chains, 512 correlator lags, xorshift128 and RANLUX; "
-                         "rng: stream throughput of every generator, as the reference's RNG benchmark")
+                         "rng: stream throughput of every generator, as the reference's RNG benchmark; "
+                         "ho: harmonic oscillator N=256, beta=10, 4096 chains")
     ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the ESS phase")
     ap.add_argument("--ess-launches", type=int, default=64, help="ESS launches of 16 x 32 sweeps")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal phase")
@@ -607,12 +717,15 @@ def main():
     if args.workload == "ragged":
         print(json.dumps(ragged_bench(args.steps, args.warmup)), flush=True)
         return
+    if args.workload == "ho":
+        print(json.dumps(ho_bench(args.steps, args.warmup, cpu=not args.no_cpu)), flush=True)
+        return
     if args.workload == "rng":
         print(json.dumps(rng_bench()), flush=True)
         return
     if args.workload == "aho":
         print(json.dumps(aho_bench(args.steps, args.warmup, learned=not args.no_learned,
-                                   cpu=not args.no_cpu)), flush=True)
+                                   cpu=not args.no_cpu, ess=not args.no_ess)), flush=True)
         return
     system, k = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
     rng = np.random.default_rng(0)

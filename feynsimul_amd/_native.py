@@ -9,16 +9,17 @@ import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfeynsimul.so")
 
-FS_ABI_VERSION = 2
+FS_ABI_VERSION = 3
 FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
 FS_RNG_XORSHIFT, FS_RNG_RANLUX = 0, 1
 FS_STORAGE_AUTO, FS_STORAGE_MEMORY, FS_STORAGE_REGISTERS = 0, 1, 2
+FS_POTCACHE_AUTO, FS_POTCACHE_ON, FS_POTCACHE_OFF = 0, 1, 2
 
 EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_path_storage",
            "fs_batch_create", "fs_batch_destroy", "fs_batch_path_storage", "fs_batch_build_log", "fs_batch_run",
            "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
-           "fs_source", "fs_run", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
+           "fs_source", "fs_run", "fs_set_waves_per_simd", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
            "fs_ranlux_draw", "fs_rng_bench", "fs_lp_create", "fs_lp_destroy", "fs_lp_get_info", "fs_lp_build_log", "fs_lp_pack",
            "fs_lp_sweep", "fs_lp_train", "fs_lp_eval", "fs_lp_grad")
 
@@ -35,8 +36,8 @@ class FsConfig(ctypes.Structure):
         ("x_min", ctypes.c_double), ("x_max", ctypes.c_double), ("op_norm", ctypes.c_double),
         ("path_storage", ctypes.c_int32), ("n_hist_obs", ctypes.c_int32), ("hist_bins", ctypes.c_int32),
         ("hist_lo", ctypes.c_double), ("hist_hi", ctypes.c_double), ("no_bisection", ctypes.c_int32),
-        ("path_shift", ctypes.c_int32), ("single_node", ctypes.c_int32), ("reserved", ctypes.c_int32),
-        ("alpha", ctypes.c_double), ("ps_alpha", ctypes.c_double)]
+        ("path_shift", ctypes.c_int32), ("single_node", ctypes.c_int32), ("max_waves_per_simd", ctypes.c_int32),
+        ("alpha", ctypes.c_double), ("ps_alpha", ctypes.c_double), ("potential_cache", ctypes.c_int32)]
 
 
 class FsBuffers(ctypes.Structure):
@@ -101,6 +102,7 @@ def lib():
     L.fs_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_source.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_run.argtypes = [vp, ctypes.POINTER(FsBuffers), i32, i32, i32, vp]
+    L.fs_set_waves_per_simd.argtypes = [vp, i32]
     L.fs_eval_system.argtypes = [vp, vp, vp, vp, i64, vp]
     L.fs_seed_xorshift.argtypes = [vp, i64, u64, i64, vp]
     L.fs_ranlux_init.argtypes = [vp, vp, i64, vp]
@@ -171,6 +173,9 @@ class Engine:
     @property
     def source(self):
         return _text(lib().fs_source, self._h)
+
+    def set_waves_per_simd(self, cap):
+        check(lib().fs_set_waves_per_simd(self._h, int(cap)))
 
     def run(self, bufs: FsBuffers, operator_runs, metro_steps, series_offset, stream):
         check(lib().fs_run(self._h, ctypes.byref(bufs), operator_runs, metro_steps, series_offset, stream))

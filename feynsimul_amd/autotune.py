@@ -1,11 +1,13 @@
-"""Launch-shape autotuner for batched runs.
+"""Launch-shape autotuner.
 
-Measures, on the device, every candidate (wavefronts per workgroup) x
-(dispatch order) for a list of KernelArgs and keeps the fastest.  Waves per
-workgroup fixes the LDS layout and the launch bounds, so each candidate value
-is a separate compile; the dispatch order is a run-time choice.  Results are
-cached per configuration and device in a JSON file
-($FEYNSIMUL_AMD_CACHE, default ~/.cache/feynsimul_amd/autotune.json).
+autotune_batch: every candidate (wavefronts per workgroup) x (dispatch order)
+for a list of KernelArgs in one persistent launch.  autotune_kernel: every
+(wavefronts per workgroup) x (wavefronts per SIMD cap) for one wave-sampler
+kernel.  Waves per workgroup fixes the LDS layout and the launch bounds, so
+each candidate value is a separate compile; the dispatch order and the
+occupancy cap are run-time choices.  The fastest is kept; results are cached
+per configuration and device in a JSON file ($FEYNSIMUL_AMD_CACHE, default
+~/.cache/feynsimul_amd/autotune.json).
 """
 import hashlib
 import json
@@ -31,14 +33,7 @@ def config_key(kernelArgsList, device_name):
     return h.hexdigest()[:20]
 
 
-def autotune_batch(kernelArgsList, wavesPerBlock=(1, 2, 4), orders=("cost", "given"), launches=3,
-                   initialPaths=None, use_cache=True, verbose=False):
-    """Returns (best, trials): best = {"wavesPerBlock", "order", "ms_per_launch"}; every trial is timed
-    over `launches` launches after one warm-up launch, starting from `initialPaths` (a list with one
-    array per member, or None for the members' default paths)."""
-    import torch
-    dev = torch.cuda.get_device_properties(torch.cuda.current_device()).name
-    key = config_key(kernelArgsList, dev)
+def _cache_load(key, use_cache):
     path = _cache_path()
     cache = {}
     if use_cache and os.path.exists(path):
@@ -47,8 +42,71 @@ def autotune_batch(kernelArgsList, wavesPerBlock=(1, 2, 4), orders=("cost", "giv
                 cache = json.load(f)
         except (OSError, ValueError):
             cache = {}
-        if key in cache:
-            return cache[key]["best"], cache[key]["trials"]
+    return cache, cache.get(key) if use_cache else None
+
+
+def _cache_store(cache, key, best, trials):
+    path = _cache_path()
+    cache[key] = {"best": best, "trials": trials}
+    try:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        with open(path + ".tmp", "w") as f:
+            json.dump(cache, f, indent=1)
+        os.replace(path + ".tmp", path)
+    except OSError:
+        pass
+
+
+def autotune_kernel(kernelArgs, initialPaths=None, wavesPerBlock=(2, 4, 8), wavesPerSimd=(0, 8, 6, 5, 4, 3),
+                    launches=3, use_cache=True, verbose=False):
+    """Returns (best, trials) for one wave-sampler KernelArgs: best = {"wavesPerBlock", "wavesPerSimd",
+    "ms_per_launch"}.  Each trial is one warm-up launch and `launches` timed launches from
+    `initialPaths` (or the kernel's default paths); the occupancy cap does not change results."""
+    import copy
+    import torch
+    from .kernel import PIMCKernel
+    dev = torch.cuda.get_device_properties(torch.cuda.current_device()).name
+    key = "kernel:" + config_key([kernelArgs], dev)
+    cache, hit = _cache_load(key, use_cache)
+    if hit is not None:
+        return hit["best"], hit["trials"]
+    trials = []
+    for wpb in wavesPerBlock:
+        ka = copy.copy(kernelArgs)
+        ka.wavesPerBlock, ka.wavesPerSimd = wpb, None
+        k = PIMCKernel(ka)
+        for cap in wavesPerSimd:
+            if initialPaths is not None:
+                k.setPaths(initialPaths)
+            k.setWavesPerSimd(cap)
+            k.run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(launches):
+                k.run(wait=False)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / launches * 1e3
+            trials.append({"wavesPerBlock": wpb, "wavesPerSimd": cap, "ms_per_launch": ms})
+            if verbose:
+                print("autotune: waves/workgroup %d, waves/SIMD cap %d: %.3f ms" % (wpb, cap, ms), flush=True)
+        del k
+    best = min(trials, key=lambda t: t["ms_per_launch"])
+    if use_cache:
+        _cache_store(cache, key, best, trials)
+    return best, trials
+
+
+def autotune_batch(kernelArgsList, wavesPerBlock=(1, 2, 4), orders=("cost", "given"), launches=3,
+                   initialPaths=None, use_cache=True, verbose=False):
+    """Returns (best, trials): best = {"wavesPerBlock", "order", "ms_per_launch"}; every trial is timed
+    over `launches` launches after one warm-up launch, starting from `initialPaths` (a list with one
+    array per member, or None for the members' default paths)."""
+    import torch
+    dev = torch.cuda.get_device_properties(torch.cuda.current_device()).name
+    key = config_key(kernelArgsList, dev)
+    cache, hit = _cache_load(key, use_cache)
+    if hit is not None:
+        return hit["best"], hit["trials"]
     trials = []
     for wpb in wavesPerBlock:
         b = BatchedPIMC(kernelArgsList, wavesPerBlock=wpb)
@@ -70,12 +128,5 @@ def autotune_batch(kernelArgsList, wavesPerBlock=(1, 2, 4), orders=("cost", "giv
         del b
     best = min(trials, key=lambda t: t["ms_per_launch"])
     if use_cache:
-        cache[key] = {"best": best, "trials": trials}
-        try:
-            os.makedirs(os.path.dirname(path), exist_ok=True)
-            with open(path + ".tmp", "w") as f:
-                json.dump(cache, f, indent=1)
-            os.replace(path + ".tmp", path)
-        except OSError:
-            pass
+        _cache_store(cache, key, best, trials)
     return best, trials

@@ -37,7 +37,7 @@ struct fs_kargs {
 constexpr int fs_cor_lanelags(int lags)
 {
     int t = 1;
-    while (t < 8 && 64 * t < lags) t <<= 1;
+    while (t < 4 && 64 * t < lags) t <<= 1;
     return t;
 }
 constexpr int fs_cor_stage_len(int N, int lags)

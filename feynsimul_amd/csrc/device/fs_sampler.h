@@ -11,7 +11,7 @@
 // FLOAT_TYPE, FS_DOUBLE, FS_RANLUX, FS_NSKIP, FS_ISCALE, FS_BINS, FS_BINRES,
 // FS_XMIN, FS_XMAX, FS_INVBIN, FS_OPNORM, FS_PARALLEL, FS_W, FS_LOCAL,
 // FS_WPB (waves per block of the wave sampler), FS_SAMPLER (0 exact, 1 wave),
-// FS_RESIDENT (wave path storage), FS_NHOBS, FS_HBINS, FS_HLO, FS_HINV
+// FS_RESIDENT (wave path storage), FS_POTCACHE (wave potential cache), FS_NHOBS, FS_HBINS, FS_HLO, FS_HINV
 // (observable histogram of the wave sampler).
 //
 // Two samplers share the move:
@@ -322,7 +322,7 @@ extern "C" __global__ void fs_exact_sweep(fs_kargs a)
 struct fs_sys_main {
     static constexpr int N = FS_N, S = FS_S, DOF = FS_DOF, NOPS = FS_NOPS, NCORR = FS_NCORR,
                          CORLAGS = FS_CORLAGS, NHOBS = FS_NHOBS, HBINS = FS_HBINS, RESIDENT = FS_RESIDENT,
-                         BINS = FS_BINS, BINRES = FS_BINRES;
+                         POTCACHE = FS_POTCACHE || FS_RESIDENT, BINS = FS_BINS, BINRES = FS_BINRES;
     static constexpr FLOAT_TYPE EPS = eps, HLO = FS_HLO, HINV = FS_HINV, XMIN = FS_XMIN, XMAX = FS_XMAX,
                                 INVBIN = FS_INVBIN;
     static FS_DEV FLOAT_TYPE V(const FLOAT_TYPE *x) { return fs_potential_v(x); }

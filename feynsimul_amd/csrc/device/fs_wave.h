@@ -12,7 +12,7 @@
 // numbers differs.
 //
 // Everything system-specific comes from a trait class Sys:
-//   constants N, S, DOF, NOPS, NCORR, CORLAGS, NHOBS, HBINS, RESIDENT, BINS,
+//   constants N, S, DOF, NOPS, NCORR, CORLAGS, NHOBS, HBINS, RESIDENT, POTCACHE, BINS,
 //   BINRES and EPS, HLO, HINV, XMIN, XMAX, INVBIN;
 //   V(x), ops(x, acc), corr(x, prod), hobs(x, out) over one slice x[DOF].
 // A module for one system instantiates fs_wave_walker once; the batched
@@ -20,10 +20,12 @@
 //
 // Path storage: with RESIDENT, lane j keeps slices j, j + 64, ... and their
 // potential in registers for the whole launch (HBM touched on entry and
-// exit); otherwise blocks stream through the cache hierarchy, the next block
-// is prefetched, and the potential at every slice is cached in potCache
-// ([walkers][N], refreshed at the start of each launch), so a move evaluates
-// the potential only at its proposed slices in both storages.
+// exit); otherwise blocks stream through the cache hierarchy and the next
+// block is prefetched.  With POTCACHE the potential at every slice is cached
+// (registers, or potCache [walkers][N] refreshed at the start of each launch),
+// so a move evaluates the potential only at its proposed slices; without it
+// (memory storage, cheap potentials) the old slice's potential is evaluated
+// again, which costs less than reading and writing the cache.
 #pragma once
 
 #include "fs_abi_args.h"
@@ -183,7 +185,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
             for (int d = 0; d < DOF; d++) xs[d][k] = x[d];
             vc[k] = Sys::V(x);
         }
-    } else if (active) {
+    } else if (Sys::POTCACHE && active) {
         // lane j only ever touches the cache entries of slices congruent to j mod B
 #pragma unroll 1
         for (int k = 0; k < NB; k++) {
@@ -258,7 +260,8 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                     for (int d = 0; d < DOF; d++) xnext[d] = fs_at(row[d], snext);
 #pragma unroll
                     for (int d = 0; d < DOF; d++) xr[d] = fs_readlane(xnext[d], r);
-                    vOld = fs_at(pot, s);
+                    if constexpr (Sys::POTCACHE) vOld = fs_at(pot, s);
+                    else vOld = Sys::V(xo);
                 }
                 // every lane draws the same count so buffered streams stay in step
                 F u[KDRAW];
@@ -310,7 +313,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                 } else if (put) {
 #pragma unroll
                     for (int d = 0; d < DOF; d++) fs_at(row[d], s) = xn[d];
-                    fs_at(pot, s) = vNew;
+                    if constexpr (Sys::POTCACHE) fs_at(pot, s) = vNew;
                 }
             }
         }

@@ -90,6 +90,8 @@ struct fs_engine {
     int wpb = 4;
     size_t wave_smem = 0;
     int device = -1;
+    int lds_per_cu = 0;        // bytes of LDS per CU (device attribute)
+    int waves_per_simd = 0;    // occupancy cap of wave launches (0 = none)
 };
 
 namespace {
@@ -128,6 +130,10 @@ int validate(const fs_config *c)
         if (c->n_hist_obs < 0 || (c->n_hist_obs > 0 && (c->hist_bins < 1 || c->hist_bins > 8192 ||
                                                       !(c->hist_hi > c->hist_lo))))
             return fail(FS_ERR_INVALID, "observable histogram needs 1 <= hist_bins <= 8192 and hist_hi > hist_lo");
+        if (c->potential_cache < FS_POTCACHE_AUTO || c->potential_cache > FS_POTCACHE_OFF)
+            return fail(FS_ERR_INVALID, "unknown potential_cache %d", c->potential_cache);
+        if (c->max_waves_per_simd < 0 || c->max_waves_per_simd > 16)
+            return fail(FS_ERR_INVALID, "max_waves_per_simd must be in [0, 16]");
         if (c->path_storage < FS_STORAGE_AUTO || c->path_storage > FS_STORAGE_REGISTERS)
             return fail(FS_ERR_INVALID, "unknown path_storage %d", c->path_storage);
         if (c->path_storage == FS_STORAGE_REGISTERS && (c->N < 64 || c->N / 64 > 16))
@@ -159,6 +165,13 @@ bool resident_storage(const fs_config *c)
     if (c->path_storage == FS_STORAGE_MEMORY) return false;
     const int words = c->double_precision ? 2 : 1;
     return (c->N / 64) * (c->DOF + 1) * words <= 64;
+}
+
+// Memory storage with the per-slice potential cache (see FS_POTCACHE_*).
+bool potential_cached(const fs_config *c)
+{
+    if (resident_storage(c)) return false;
+    return c->potential_cache == FS_POTCACHE_ON || (c->potential_cache == FS_POTCACHE_AUTO && c->DOF > 1);
 }
 
 int nskip_of(const fs_config *c)
@@ -223,7 +236,9 @@ std::string wave_trait(const fs_config *c)
                     ", CORLAGS = " + std::to_string(c->cor_lags > 0 ? c->cor_lags : c->N / 2) +
                     ", NHOBS = " + std::to_string(c->n_hist_obs) +
                     ", HBINS = " + std::to_string(c->hist_bins > 0 ? c->hist_bins : 1) +
-                    ", RESIDENT = " + std::to_string(res ? 1 : 0) + ", BINS = " + std::to_string(c->bins ? 1 : 0) +
+                    ", RESIDENT = " + std::to_string(res ? 1 : 0) +
+                    ", POTCACHE = " + std::to_string(potential_cached(c) || res ? 1 : 0) +
+                    ", BINS = " + std::to_string(c->bins ? 1 : 0) +
                     ", BINRES = " + std::to_string(c->bin_res > 0 ? c->bin_res : 1) + ";\n";
     s += "    static constexpr FLOAT_TYPE EPS = " + lit(c->beta / (double)c->N) + ", HLO = " + lit(c->hist_lo) +
          ", HINV = " + lit(c->n_hist_obs > 0 ? (double)c->hist_bins / (c->hist_hi - c->hist_lo) : 1.0) +
@@ -294,6 +309,7 @@ std::string build_source(const fs_config *c, const char *system_src, int wpb)
     def("FS_HLO", lit(c->hist_lo));
     def("FS_HINV", lit(c->n_hist_obs > 0 ? (double)c->hist_bins / (c->hist_hi - c->hist_lo) : 1.0));
     defi("FS_RESIDENT", resident_storage(c) ? 1 : 0);
+    defi("FS_POTCACHE", potential_cached(c) ? 1 : 0);
     defi("FS_BISECTION", c->no_bisection ? 0 : 1);
     defi("FS_PATH_SHIFT", c->path_shift ? 1 : 0);
     defi("FS_SINGLE_NODE", c->single_node ? 1 : 0);
@@ -409,6 +425,10 @@ int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs
             return fail(FS_ERR_HIP, "module has no %s: %s", kname, hipGetErrorString(fe));
         }
         (cfg->sampler == FS_SAMPLER_WAVE ? e->fn_wave : e->fn_exact) = fn;
+        if (hipDeviceGetAttribute(&e->lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, e->device) !=
+            hipSuccess)
+            e->lds_per_cu = 0;
+        e->waves_per_simd = cfg->max_waves_per_simd;
         (void)hipModuleGetFunction(&e->fn_eval, e->module, "fs_eval_system");
         (void)hipGetLastError();
     }
@@ -503,9 +523,23 @@ int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t met
     } else {
         if (!e->fn_wave) return fail(FS_ERR_INVALID, "module has no wave sampler");
         const unsigned grid = (unsigned)((c.n_walkers + e->wpb - 1) / e->wpb);
-        FS_HIP(hipModuleLaunchKernel(e->fn_wave, grid, 1, 1, 64 * e->wpb, 1, 1, (unsigned)e->wave_smem, st, args,
-                                     nullptr));
+        size_t smem = e->wave_smem;
+        if (e->waves_per_simd > 0 && e->lds_per_cu > 0) {
+            // at most B blocks of wpb wavefronts per CU (4 SIMDs): request more than 1/(B+1) of its LDS
+            const int B = std::max(2, e->waves_per_simd * 4 / e->wpb);
+            const size_t pad = ((size_t)e->lds_per_cu / (size_t)(B + 1) + 16) & ~(size_t)15;
+            if (pad <= 65536) smem = std::max(smem, pad);
+        }
+        FS_HIP(hipModuleLaunchKernel(e->fn_wave, grid, 1, 1, 64 * e->wpb, 1, 1, (unsigned)smem, st, args, nullptr));
     }
+    return FS_OK;
+}
+
+int fs_set_waves_per_simd(fs_engine *e, int32_t cap)
+{
+    if (!e) return fail(FS_ERR_INVALID, "engine is NULL");
+    if (cap < 0 || cap > 16) return fail(FS_ERR_INVALID, "waves per SIMD must be in [0, 16]");
+    e->waves_per_simd = cap;
     return FS_OK;
 }
 

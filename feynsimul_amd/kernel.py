@@ -264,6 +264,10 @@ class PIMCKernel:
         cfg.bins = int(bool(self._enableBins))
         cfg.bin_res = int(self._binResolutionPerDOF) if self._enableBins else 1
         cfg.waves_per_block = int(getattr(ka, "wavesPerBlock", None) or 0)
+        cfg.max_waves_per_simd = int(getattr(ka, "wavesPerSimd", None) or 0)
+        pc = getattr(ka, "potentialCache", None)
+        cfg.potential_cache = (_native.FS_POTCACHE_AUTO if pc is None else
+                               _native.FS_POTCACHE_ON if pc else _native.FS_POTCACHE_OFF)
         cfg.beta = float(self._beta)
         cfg.x_min = float(self._xMin) if self._enableBins else 0.0
         cfg.x_max = float(self._xMax) if self._enableBins else 1.0
@@ -486,6 +490,12 @@ class PIMCKernel:
             raise Exception("operator totals are kept by the wave sampler with operators enabled")
         t = self._opTotals.cpu().numpy()
         return (t[..., 0] - t[..., 1]).T.copy()
+
+    def setWavesPerSimd(self, cap):
+        """Occupancy cap of the wave sampler's launches (0 = none); results do not depend on it."""
+        if self._engine is None:
+            raise Exception("this kernel is a member of a BatchedPIMC; the batch sets its launch shape")
+        self._engine.set_waves_per_simd(int(cap or 0))
 
     def setSeriesCapacity(self, measurements):
         """Keeps every measurement's per-walker slice mean (for autocorrelation / ESS); wave sampler."""

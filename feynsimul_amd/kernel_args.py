@@ -20,6 +20,13 @@ arguments at the end, all optional:
     (default True).
 ``wavesPerBlock``
     launch shape of the wave sampler (default automatic).
+``potentialCache``
+    wave sampler with paths in memory: keep the potential of every slice in
+    a cache buffer (True), evaluate it again for the old slice of each move
+    (False), or decide by the system (None: cache when DOF > 1).
+``wavesPerSimd``
+    occupancy cap of the wave sampler: at most this many wavefronts per SIMD
+    (default None: as registers and LDS allow; ``autotune_kernel`` picks it).
 ``pathStorage``
     where the wave sampler keeps each walker's path during a launch:
     ``"registers"``, ``"memory"`` or ``None`` (automatic: registers when the
@@ -71,7 +78,9 @@ class KernelArgs:
                  pathStorage=None,
                  histogramObservables=None,
                  histogramRange=None,
-                 histogramBins=256):
+                 histogramBins=256,
+                 wavesPerSimd=None,
+                 potentialCache=None):
         self.system = system
         self.nbrOfWalkers = nbrOfWalkers
         self.N = N
@@ -109,3 +118,5 @@ class KernelArgs:
         self.histogramObservables = histogramObservables
         self.histogramRange = histogramRange
         self.histogramBins = histogramBins
+        self.wavesPerSimd = wavesPerSimd
+        self.potentialCache = potentialCache

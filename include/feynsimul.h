@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FS_ABI_VERSION 2
+#define FS_ABI_VERSION 3
 
 enum {
     FS_OK = 0,
@@ -49,6 +49,12 @@ enum { FS_SAMPLER_EXACT = 0, FS_SAMPLER_WAVE = 1 };
  * registers when 64 <= N <= 1024 and the path plus its potential cache take
  * at most 64 VGPRs per lane, else memory (then pot_cache is required). */
 enum { FS_STORAGE_AUTO = 0, FS_STORAGE_MEMORY = 1, FS_STORAGE_REGISTERS = 2 };
+
+/* Memory storage keeps the potential of every slice in pot_cache (read for
+ * the old slice of a move, written on acceptance) instead of evaluating it
+ * again; that pays when the potential is dearer than 8 bytes of HBM traffic
+ * per slice-update.  AUTO: cache when DOF > 1. */
+enum { FS_POTCACHE_AUTO = 0, FS_POTCACHE_ON = 1, FS_POTCACHE_OFF = 2 };
 
 /* One configuration = one compiled module (the reference rebuilds its
  * program for every change of these, kernel.py:213-382). */
@@ -83,9 +89,11 @@ typedef struct fs_config {
     int32_t no_bisection;       /* !enableBisection */
     int32_t path_shift;         /* enablePathShift (exact sampler, one item per walker) */
     int32_t single_node;        /* enableSingleNodeMove (exact sampler, one item per walker) */
-    int32_t reserved;
+    int32_t max_waves_per_simd; /* wave sampler: occupancy cap (0 = as registers and LDS allow), see
+                                  * fs_set_waves_per_simd */
     double alpha;               /* single-node step width */
     double ps_alpha;            /* path-shift offset width */
+    int32_t potential_cache;    /* wave sampler, memory storage: FS_POTCACHE_* */
 } fs_config;
 
 /* Device buffers of one launch; unused entries may be NULL.  Layouts:
@@ -136,6 +144,13 @@ int fs_path_storage(const fs_engine *e, int *storage);
 int fs_build_log(const fs_engine *e, char *buf, size_t len, size_t *needed);
 /* Copies the generated module source (for verbose dumps). */
 int fs_source(const fs_engine *e, char *buf, size_t len, size_t *needed);
+
+/* Occupancy cap of the wave sampler's launches: at most `cap` wavefronts per
+ * SIMD (0 = no cap), enforced by padding each block's LDS request; caps that
+ * would need fewer than two blocks per CU give two.  No reference counterpart
+ * (the reference fixes its work-group shape per build, kernel.py:192-211); the
+ * autotuner uses it. */
+int fs_set_waves_per_simd(fs_engine *e, int32_t cap);
 
 /* One launch of operator_runs x metro_steps sweeps (the reference `run`). */
 int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t metro_steps,

@@ -266,3 +266,18 @@ def test_kernel_args_moves_select_the_exact_sampler():
     with pytest.raises(Exception, match="wave sampler runs bisection moves only"):
         PIMCKernel(KernelArgs(enablePathShift=True, enableSingleNodeMove=False, PSAlpha=0.1,
                               enableParallelizePath=False, sampler="wave", **base))
+
+
+def test_occupancy_cap_range():
+    """fs_set_waves_per_simd and max_waves_per_simd accept 0..16 (0 = no cap)."""
+    from feynsimul_amd import _native
+    e = _native.Engine(_cfg(HarmOsc(), 64, 3, 1), system_section(HarmOsc(), ("x1",)), compile_only=True)
+    e.set_waves_per_simd(5)
+    e.set_waves_per_simd(0)
+    for bad in (-1, 17):
+        with pytest.raises(_native.FsError, match="waves per SIMD"):
+            e.set_waves_per_simd(bad)
+    e.close()
+    with pytest.raises(_native.FsError, match="max_waves_per_simd"):
+        _native.Engine(_cfg(HarmOsc(), 64, 3, 1, max_waves_per_simd=20), system_section(HarmOsc(), ("x1",)),
+                       compile_only=True)

@@ -123,3 +123,27 @@ def test_ragged_lm2m2_batch_full_size(gpu):
         assert 0.0 < ar < 1.0
         assert np.isfinite(k.getOperators()).all()
     torch.cuda.empty_cache()
+
+
+def test_occupancy_cap_keeps_results_and_autotune_kernel(gpu):
+    """The occupancy cap only pads the launch's LDS request: paths, operators and streams are bit-identical
+    with and without it; autotune_kernel times every (waves per workgroup, cap) pair."""
+    from feynsimul_amd.autotune import autotune_kernel
+    _case("occupancy cap + autotune_kernel")
+    system = AnharmOsc(0.1)
+    W, N, S = 512, 1024, 4
+    p0 = np.random.default_rng(12).uniform(-1, 1, (W, N))
+    out = []
+    for cap in (0, 3):
+        k = PIMCKernel(_ka(system, W, N, S, 20.0, 2, 8, seed=5))
+        k.setPaths(p0)
+        k.setWavesPerSimd(cap)
+        k.run()
+        out.append((k.getPaths(), k.getOperators(), k._seeds.cpu().numpy()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    np.testing.assert_array_equal(out[0][2], out[1][2])
+    best, trials = autotune_kernel(_ka(system, W, N, S, 20.0, 1, 8, seed=5), initialPaths=p0, wavesPerBlock=(2, 4),
+                                   wavesPerSimd=(0, 4), launches=1, use_cache=False)
+    assert len(trials) == 4 and best in trials
+    assert all(t["ms_per_launch"] > 0 for t in trials)
