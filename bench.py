@@ -41,10 +41,10 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fs_wave_sweep_lm2m2_he3_n1024.md")
 
 
-def measured_traffic():
+def measured_traffic(profile=TRAFFIC_PROFILE):
     import re
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(profile) as f:
             m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", f.read(), re.M)
         return int(m.group(1)) if m else None
     except OSError:
@@ -261,6 +261,7 @@ AHO_BYTES_PER_UPDATE = 8.0 * 1 * AHO_SEG / (AHO_SEG - 1)
 # 26 per coordinate + 2 potential evaluations (x^2/2 + lam x^4 and the eps-weighted sum: 5 ops)
 AHO_VALU_PER_UPDATE = 26 * 1 + 2 * 5
 AHO_GOLDEN = os.path.join(ROOT, "tests", "golden", "aho_c5.json")
+AHO_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fs_wave_sweep_aho_n4096.md")
 
 
 def make_aho_kernel(S, steps, opRuns, corr, ranlux=False, seed=5):
@@ -364,8 +365,8 @@ def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_trai
                                    "tau=0..511 in FP64 Kahan bins) per step", "sampler": "wave",
                        "chains": AHO_W, "slices": AHO_N, "bisection_level": AHO_S, "lags": AHO_LAGS},
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "fs_wave_sweep",
-                         "avg_launch_ms": ms, "bytes_per_update": AHO_BYTES_PER_UPDATE,
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(AHO_TRAFFIC_PROFILE),
+                         "kernel": "fs_wave_sweep", "avg_launch_ms": ms, "bytes_per_update": AHO_BYTES_PER_UPDATE,
                          "updates_per_launch": updates},
             "valu": {"ops_per_update": AHO_VALU_PER_UPDATE, "achieved": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3),
                      "peak": VALU_PEAK, "frac": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3) / VALU_PEAK},
