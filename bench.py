@@ -49,6 +49,18 @@ def measured_traffic(profile=TRAFFIC_PROFILE):
         return int(m.group(1)) if m else None
     except OSError:
         return None
+
+
+def measured_valu_issue(profile=TRAFFIC_PROFILE):
+    """Fraction of VALU issue capacity the kernel used, from the SQ counters in its profile summary."""
+    import re
+    try:
+        with open(profile) as f:
+            m = re.search(r"VALU issue (?:is )?[0-9.e]+ (?:wave-instructions )?x 4 (?:cycles )?/ \(1024[^)]*\) = ([0-9.]+)",
+                          f.read().replace("×", "x"))
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s, one FP32 op per lane per clock
 MFMA_BF16_PEAK = 2.5e15       # dense BF16 MFMA FLOP/s
 # learned proposal: one network evaluation = 2 (19*128 + 128*128*2 + 128*18) = 75,008 FLOP; a
@@ -369,7 +381,8 @@ def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_trai
                          "kernel": "fs_wave_sweep", "avg_launch_ms": ms, "bytes_per_update": AHO_BYTES_PER_UPDATE,
                          "updates_per_launch": updates},
             "valu": {"ops_per_update": AHO_VALU_PER_UPDATE, "achieved": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3),
-                     "peak": VALU_PEAK, "frac": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3) / VALU_PEAK},
+                     "peak": VALU_PEAK, "frac": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3) / VALU_PEAK,
+                     "issue_measured": measured_valu_issue(AHO_TRAFFIC_PROFILE)},
             "acceptance": k.getAcceptanceRate(),
             "oracle": {"kind": "FP64 transfer matrix of the discretised action (tests/golden/aho_c5.json)",
                        "E_virial": E, "E_virial_se": Ese, "E_virial_exact": gold["E_virial"],
@@ -774,7 +787,9 @@ def main():
                      "kernel": "fs_wave_sweep", "avg_launch_ms": kern_s * 1e3,
                      "bytes_per_update": BYTES_PER_UPDATE, "updates_per_launch": updates},
         "valu": {"ops_per_update": VALU_PER_UPDATE, "achieved": VALU_PER_UPDATE * updates / kern_s,
-                 "peak": VALU_PEAK, "frac": VALU_PER_UPDATE * updates / kern_s / VALU_PEAK},
+                 "peak": VALU_PEAK, "frac": VALU_PER_UPDATE * updates / kern_s / VALU_PEAK,
+                 "issue_measured": measured_valu_issue(),
+                 "note": "the kernel is VALU-issue bound (SQ counters in profiles/); HBM is the second bound"},
         "acceptance": k.getAcceptanceRate(),
         "energy_K": E,
         "oracle": he3_oracle(per_walker, (args.warmup + args.steps) * SWEEPS_PER_STEP),

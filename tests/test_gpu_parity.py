@@ -342,3 +342,23 @@ def test_wave_correlator_matches_oracle(gpu, case):
     assert err.max() < (1e-7 if dbl else 2e-3)
     # C(0) is the measurement sum of c^2, which the oracle forms the same way
     assert np.all(exp[:, :, 0] > 0)
+
+
+@pytest.mark.parametrize("name,N,S", [("aho", 2048, 4), ("lm2m2", 1024, 6)])
+def test_potential_cache_choice_keeps_results(gpu, name, N, S):
+    """Memory storage with and without the per-slice potential cache: the cached value is the same FP32
+    evaluation of the same coordinates, so paths, operators and streams are bit-identical."""
+    _case("potential cache on/off " + name)
+    system = _system(name)
+    W = 16
+    p0 = _initial_paths(system, W, N, np.random.default_rng(23))
+    out = []
+    for pc in (True, False):
+        ka = _ka(system, W, N, S, 10.0, 2, 3, True, False, False, "wave", storage="memory")
+        ka.potentialCache = pc
+        k = PIMCKernel(ka)
+        k.setPaths(p0)
+        k.run()
+        out.append((k.getPaths(), k.getOperators(), k._seeds.cpu().numpy(), k.getAcceptanceRate()))
+    for a, b in zip(out[0], out[1]):
+        np.testing.assert_array_equal(a, b)
