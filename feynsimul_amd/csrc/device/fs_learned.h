@@ -39,7 +39,10 @@
 // fs_lp_pack rebuilds the BF16 image.
 //
 // Needs: a trait `sys` (N, S, DOF, EPS, V), FS_LP_SPW (segments per
-// wavefront), FS_LP_WAVES (wavefronts per block), FS_LP_ATOMS3D.
+// wavefront), FS_LP_WAVES (wavefronts per block), FS_LP_ATOMS3D, and
+// FS_LP_CB (row tile = 32 FS_LP_CB rows: 2 reuses each weight operand for two
+// MFMAs, 1 halves the accumulators and activations so two wavefronts fit a
+// SIMD).
 #pragma once
 
 #include "fs_lp_abi.h"
@@ -95,69 +98,73 @@ FS_DEV fs_f16v fs_lp_mfma(fs_bf8 a, fs_bf8 b, fs_f16v c) { return __builtin_amdg
 // xin[ks][cb]: layer-1 B operand (inputs ks*16 + 8g + e of row cb*32 + lane%32).
 // out[cb][j]: output slot (g, j) of row cb*32 + lane%32; slot j of lane group g
 // holds mu-offset (j even) or s (j odd) of coordinate d = 2 (j/2) + g.
-FS_DEV void fs_lp_bias_init(const float *b, int ob, int g, fs_f16v &c0, fs_f16v &c1)
+FS_DEV void fs_lp_bias_init(const float *b, int ob, int g, fs_f16v (&c)[FS_LP_CB])
 {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const float4 v = *reinterpret_cast<const float4 *>(b + ob * 32 + 8 * q + 4 * g);
-        c0[4 * q + 0] = c1[4 * q + 0] = v.x;
-        c0[4 * q + 1] = c1[4 * q + 1] = v.y;
-        c0[4 * q + 2] = c1[4 * q + 2] = v.z;
-        c0[4 * q + 3] = c1[4 * q + 3] = v.w;
+#pragma unroll
+        for (int cb = 0; cb < FS_LP_CB; cb++) {
+            c[cb][4 * q + 0] = v.x;
+            c[cb][4 * q + 1] = v.y;
+            c[cb][4 * q + 2] = v.z;
+            c[cb][4 * q + 3] = v.w;
+        }
     }
 }
 
-FS_DEV void fs_lp_relu_pack(const fs_f16v (&acc)[4][2], fs_bf8 (&h)[8][2])
+FS_DEV void fs_lp_relu_pack(const fs_f16v (&acc)[4][FS_LP_CB], fs_bf8 (&h)[8][FS_LP_CB])
 {
 #pragma unroll
     for (int ob = 0; ob < 4; ob++)
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++)
+        for (int cb = 0; cb < FS_LP_CB; cb++)
 #pragma unroll
             for (int hh = 0; hh < 2; hh++)
 #pragma unroll
                 for (int e = 0; e < 8; e++) h[2 * ob + hh][cb][e] = (__bf16)fmaxf(acc[ob][cb][8 * hh + e], 0.0f);
 }
 
-FS_DEV void fs_lp_hidden(const char *img, int woff, int boff, const fs_bf8 (&hin)[8][2], fs_f16v (&acc)[4][2],
+FS_DEV void fs_lp_hidden(const char *img, int woff, int boff, const fs_bf8 (&hin)[8][FS_LP_CB],
+                         fs_f16v (&acc)[4][FS_LP_CB],
                          int lane)
 {
     const int g = lane >> 5;
     const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + woff);
     const float *b = reinterpret_cast<const float *>(img + boff);
 #pragma unroll
-    for (int ob = 0; ob < 4; ob++) fs_lp_bias_init(b, ob, g, acc[ob][0], acc[ob][1]);
+    for (int ob = 0; ob < 4; ob++) fs_lp_bias_init(b, ob, g, acc[ob]);
 #pragma unroll
     for (int ks = 0; ks < 8; ks++)
 #pragma unroll
         for (int ob = 0; ob < 4; ob++) {
             const fs_bf8 A = W[(ob * 8 + ks) * 64 + lane];
-            acc[ob][0] = fs_lp_mfma(A, hin[ks][0], acc[ob][0]);
-            acc[ob][1] = fs_lp_mfma(A, hin[ks][1], acc[ob][1]);
+#pragma unroll
+            for (int cb = 0; cb < FS_LP_CB; cb++) acc[ob][cb] = fs_lp_mfma(A, hin[ks][cb], acc[ob][cb]);
         }
 }
 
-FS_DEV void fs_lp_mlp(const char *img, const fs_bf8 (&xin)[2][2], fs_f16v (&out)[2], int lane)
+FS_DEV void fs_lp_mlp(const char *img, const fs_bf8 (&xin)[2][FS_LP_CB], fs_f16v (&out)[FS_LP_CB], int lane)
 {
     const int g = lane >> 5;
-    fs_f16v acc[4][2];
+    fs_f16v acc[4][FS_LP_CB];
     {
         const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_OFF_W1);
         const float *b = reinterpret_cast<const float *>(img + FS_LP_OFF_B1);
 #pragma unroll
-        for (int ob = 0; ob < 4; ob++) fs_lp_bias_init(b, ob, g, acc[ob][0], acc[ob][1]);
+        for (int ob = 0; ob < 4; ob++) fs_lp_bias_init(b, ob, g, acc[ob]);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++)
 #pragma unroll
             for (int ob = 0; ob < 4; ob++) {
                 const fs_bf8 A = W[(ob * 2 + ks) * 64 + lane];
-                acc[ob][0] = fs_lp_mfma(A, xin[ks][0], acc[ob][0]);
-                acc[ob][1] = fs_lp_mfma(A, xin[ks][1], acc[ob][1]);
+#pragma unroll
+                for (int cb = 0; cb < FS_LP_CB; cb++) acc[ob][cb] = fs_lp_mfma(A, xin[ks][cb], acc[ob][cb]);
             }
     }
     // layer boundaries are scheduling barriers: without them the weight
     // loads of later layers are hoisted and the live set outgrows the registers
-    fs_bf8 h[8][2];
+    fs_bf8 h[8][FS_LP_CB];
     fs_lp_relu_pack(acc, h);
     __builtin_amdgcn_sched_barrier(0);
     fs_lp_hidden(img, FS_LP_OFF_W2, FS_LP_OFF_B2, h, acc, lane);
@@ -169,12 +176,12 @@ FS_DEV void fs_lp_mlp(const char *img, const fs_bf8 (&xin)[2][2], fs_f16v (&out)
     {
         const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_OFF_W4);
         const float *b = reinterpret_cast<const float *>(img + FS_LP_OFF_B4);
-        fs_lp_bias_init(b, 0, g, out[0], out[1]);
+        fs_lp_bias_init(b, 0, g, out);
 #pragma unroll
         for (int ks = 0; ks < 8; ks++) {
             const fs_bf8 A = W[ks * 64 + lane];
-            out[0] = fs_lp_mfma(A, h[ks][0], out[0]);
-            out[1] = fs_lp_mfma(A, h[ks][1], out[1]);
+#pragma unroll
+            for (int cb = 0; cb < FS_LP_CB; cb++) out[cb] = fs_lp_mfma(A, h[ks][cb], out[cb]);
         }
     }
 }
@@ -258,10 +265,10 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
         const float sig = sqrtf((float)SEG * (float)Sys::EPS / (2.0f * (float)(1 << n)));
         const float level = (float)n / (float)S;
 #pragma unroll 1
-        for (int t0 = 0; t0 < total; t0 += 64) {
-            fs_bf8 xin[2][2];
+        for (int t0 = 0; t0 < total; t0 += 32 * FS_LP_CB) {
+            fs_bf8 xin[2][FS_LP_CB];
 #pragma unroll
-            for (int cb = 0; cb < 2; cb++) {
+            for (int cb = 0; cb < FS_LP_CB; cb++) {
                 const int R = t0 + cb * 32 + c32;
                 float f[32];
                 if (R < total) {
@@ -280,10 +287,10 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
                 }
                 fs_lp_xin(f, g, xin[0][cb], xin[1][cb]);
             }
-            fs_f16v out[2];
+            fs_f16v out[FS_LP_CB];
             fs_lp_mlp(img, xin, out, lane);
 #pragma unroll
-            for (int cb = 0; cb < 2; cb++) {
+            for (int cb = 0; cb < FS_LP_CB; cb++) {
                 const int R = t0 + cb * 32 + c32;
                 const bool valid = R < total, old = R >= NR;
                 const int Rl = old ? R - NR : R, k = Rl >> (n - 1), m = (2 * (Rl & (nn - 1)) + 1) * h;
@@ -618,20 +625,21 @@ extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_eval(const 
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 5, c32 = lane & 31;
 #pragma unroll 1
-    for (int t0 = (blockIdx.x * FS_LP_WAVES + wave) * 64; t0 < rows; t0 += gridDim.x * FS_LP_WAVES * 64) {
-        fs_bf8 xin[2][2];
+    for (int t0 = (blockIdx.x * FS_LP_WAVES + wave) * 32 * FS_LP_CB; t0 < rows;
+         t0 += gridDim.x * FS_LP_WAVES * 32 * FS_LP_CB) {
+        fs_bf8 xin[2][FS_LP_CB];
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
+        for (int cb = 0; cb < FS_LP_CB; cb++) {
             const int R = t0 + cb * 32 + c32;
             float f[32];
 #pragma unroll
             for (int i = 0; i < 32; i++) f[i] = R < rows ? X[(long)R * 32 + i] : 0.0f;
             fs_lp_xin(f, g, xin[0][cb], xin[1][cb]);
         }
-        fs_f16v out[2];
+        fs_f16v out[FS_LP_CB];
         fs_lp_mlp(fs_lp_smem, xin, out, lane);
 #pragma unroll
-        for (int cb = 0; cb < 2; cb++) {
+        for (int cb = 0; cb < FS_LP_CB; cb++) {
             const int R = t0 + cb * 32 + c32;
             if (R < rows) {
 #pragma unroll
