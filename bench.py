@@ -228,7 +228,7 @@ def ragged_bench(steps, warmup):
     from feynsimul_amd.kernel_args import KernelArgs
     from feynsimul_amd.physical_systems import Lm2m2Cluster
     from feynsimul_amd.batch import BatchedPIMC
-    from feynsimul_amd.autotune import autotune_batch
+    from feynsimul_amd.autotune import autotune_batch, autotune_levels
     rng = np.random.default_rng(5)
     kas, starts = [], []
     for n in (3, 4, 6, 8):
@@ -242,6 +242,10 @@ def ragged_bench(steps, warmup):
                                   seed=n * 4096 + NN, sampler="wave"))
             pos = system.initialPositions(2048, rng, side=3.5, sigma=3.0)
             starts.append(np.repeat(pos[:, :, None], NN, axis=2).reshape(2048, system.DOF * NN))
+    # bisection level per type from measured acceptance (the reference's 20-50 % window), then launch shape
+    levels, table = autotune_levels(kas, initialPaths=starts, warmupSweeps=512)
+    for ka, S in zip(kas, levels):
+        ka.S = S
     best, trials = autotune_batch(kas, initialPaths=starts, use_cache=False)
     batch = BatchedPIMC(kas, wavesPerBlock=best["wavesPerBlock"], order=best["order"])
     for k, p in zip(batch.kernels, starts):
@@ -255,14 +259,20 @@ def ragged_bench(steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     upd = batch.sliceUpdatesPerLaunch() * steps
+    acc = [k.getAcceptanceRate() for k in batch.kernels]
+    per_type = [k._nbrOfWalkers * (k._N >> k._S) * ((1 << k._S) - 1) * SWEEPS_PER_STEP for k in batch.kernels]
+    accepted = sum(u * a for u, a in zip(per_type, acc)) * steps
     return {"metric": "path-slice updates/s, ragged batch in one persistent launch", "value": upd / wall,
             "unit": "slice-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
             "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic: seeded Gaussian blobs (sigma 3 A)",
             "config": {"workload": "LM2M2 clusters n in {3,4,6,8} x N in {256,512,1024,2048}, 2048 chains each "
-                                   "(32768 chains), S=6, beta=10/K, 32 sweeps per step, hipGraph replay",
-                       "autotune": best, "trials": trials},
-            "acceptance": [round(k.getAcceptanceRate(), 4) for k in batch.kernels]}
+                                   "(32768 chains), beta=10/K, S per type from measured acceptance, 32 sweeps per "
+                                   "step, hipGraph replay",
+                       "levels": levels, "autotune": best, "trials": trials,
+                       "acceptance_by_level": [{str(S): round(a, 4) for S, a in sorted(t.items())} for t in table]},
+            "accepted_slice_updates_per_s": accepted / wall,
+            "acceptance": [round(a, 4) for a in acc]}
 
 
 # ---- anharmonic oscillator: the correlator and RNG stress workload ----

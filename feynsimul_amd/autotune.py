@@ -1,7 +1,10 @@
 """Launch-shape autotuner.
 
 autotune_batch: every candidate (wavefronts per workgroup) x (dispatch order)
-for a list of KernelArgs in one persistent launch.  autotune_kernel: every
+for a list of KernelArgs in one persistent launch.  autotune_levels: the
+bisection level S of every member of a ragged batch, from acceptance rates
+measured on the device (the reference's run loop adapts S to keep the
+acceptance between 20 and 50 %, FeynSimul/pimc_utils.py:270-279).  autotune_kernel: every
 (wavefronts per workgroup) x (wavefronts per SIMD cap) for one wave-sampler
 kernel.  Waves per workgroup fixes the LDS layout and the launch bounds, so
 each candidate value is a separate compile; the dispatch order and the
@@ -130,3 +133,49 @@ def autotune_batch(kernelArgsList, wavesPerBlock=(1, 2, 4), orders=("cost", "giv
     if use_cache:
         _cache_store(cache, key, best, trials)
     return best, trials
+
+
+def autotune_levels(kernelArgsList, initialPaths=None, levels=(1, 2, 3, 4, 5, 6), warmupSweeps=64, sweeps=32,
+                    minAcceptance=0.2, verbose=False):
+    """Bisection level per member: the largest S whose measured acceptance is at least `minAcceptance`
+    (the lower end of the reference's 20-50 % window), else the S with the highest acceptance.
+
+    Every candidate S is one batch of all members (levels a member cannot take, 2^S > min(N, 64), are
+    skipped for it); each starts from the same paths, which `warmupSweeps` sweeps at the members' own S
+    first bring away from `initialPaths`.  Returns (levels, table): levels[t] is the chosen S of member t,
+    table[t] maps S to the acceptance measured with it.
+    """
+    import copy
+    n = len(kernelArgsList)
+
+    def with_steps(ka, S, steps):
+        k = copy.copy(ka)
+        k.S, k.metroStepsPerOperatorRun, k.operatorRuns = S, steps, 1
+        return k
+
+    warm = BatchedPIMC([with_steps(ka, ka.S, warmupSweeps) for ka in kernelArgsList])
+    if initialPaths is not None:
+        for k, p in zip(warm.kernels, initialPaths):
+            k.setPaths(p)
+    warm.run()
+    start = [k.getPaths() for k in warm.kernels]
+    del warm
+    table = [dict() for _ in range(n)]
+    for S in levels:
+        members = [t for t, ka in enumerate(kernelArgsList) if (1 << S) <= min(int(ka.N), 64)]
+        if not members:
+            continue
+        b = BatchedPIMC([with_steps(kernelArgsList[t], S, sweeps) for t in members])
+        for k, t in zip(b.kernels, members):
+            k.setPaths(start[t])
+        b.run()
+        for k, t in zip(b.kernels, members):
+            table[t][S] = k.getAcceptanceRate()
+        if verbose:
+            print("autotune levels: S=%d acceptance %s" % (S, [round(table[t][S], 3) for t in members]), flush=True)
+        del b
+    chosen = []
+    for t in range(n):
+        ok = [S for S, a in table[t].items() if a >= minAcceptance]
+        chosen.append(max(ok) if ok else max(table[t], key=table[t].get))
+    return chosen, table

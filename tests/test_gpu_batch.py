@@ -147,3 +147,22 @@ def test_occupancy_cap_keeps_results_and_autotune_kernel(gpu):
                                    wavesPerSimd=(0, 4), launches=1, use_cache=False)
     assert len(trials) == 4 and best in trials
     assert all(t["ms_per_launch"] > 0 for t in trials)
+
+
+def test_autotune_levels_follows_the_acceptance_window(gpu):
+    """autotune_levels measures every feasible S per member and keeps the largest whose acceptance is at
+    least 20 % (else the best): a stiff oscillator gets a smaller S than a soft one."""
+    from feynsimul_amd.autotune import autotune_levels
+    _case("autotune_levels")
+    kas = [_ka(HarmOsc(), 64, 256, 6, 10.0, 1, 8, seed=1), _ka(HarmOsc(k=400.0), 64, 256, 6, 10.0, 1, 8, seed=2),
+           _ka(Lm2m2Cluster(3), 32, 128, 5, 10.0, 1, 8, seed=3)]
+    rng = np.random.default_rng(4)
+    starts = [_paths(ka.system, ka.nbrOfWalkers, ka.N, rng) for ka in kas]
+    levels, table = autotune_levels(kas, initialPaths=starts, levels=(2, 3, 4, 5, 6), warmupSweeps=16, sweeps=8)
+    print(levels, table)
+    assert sorted(table[2]) == [2, 3, 4, 5, 6]  # N = 128 allows S <= 6 (64 lanes)
+    for t, S in enumerate(levels):
+        assert S in table[t]
+        ok = [s for s, a in table[t].items() if a >= 0.2]
+        assert S == (max(ok) if ok else max(table[t], key=table[t].get))
+    assert levels[1] < levels[0]
