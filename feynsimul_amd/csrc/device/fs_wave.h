@@ -311,9 +311,14 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                         vc[g1] = (put && wrap) ? vNew : v1;
                     }
                 } else if (put) {
+                    // a fresh copy of the slice index inside the branch: the stores then form their
+                    // 32-bit offset here and take the scalar-base form, instead of reusing the 64-bit
+                    // address the loads above left in registers
+                    int sl = s;
+                    __asm__ volatile("" : "+v"(sl));
 #pragma unroll
-                    for (int d = 0; d < DOF; d++) fs_at(row[d], s) = xn[d];
-                    if constexpr (Sys::POTCACHE) fs_at(pot, s) = vNew;
+                    for (int d = 0; d < DOF; d++) fs_at(row[d], sl) = xn[d];
+                    if constexpr (Sys::POTCACHE) fs_at(pot, sl) = vNew;
                 }
             }
         }
