@@ -91,30 +91,51 @@ FS_DEV void fs_wave_sync()
 }
 
 // Lag sums acc[k] += sum_t c_t c_{t+tau0+k}, k < CT, for t = 0..nt-1 in
-// ascending order (each lag keeps the sequential FP32 order).  bc(t) reads
-// c_t (the same word for every lane), win(i) reads c_{i+tau0} for the lane's
-// first lag tau0.  The window of 2 CT values slides in registers, so a step
-// of CT slices costs CT broadcast reads, CT window reads and CT^2 FMAs.
+// ascending order (each lag keeps the sequential FP32 order).  bc(t) points
+// at c_t .. c_{t+CT-1} (the same words for every lane), win(i) at
+// c_{i+tau0} .. c_{i+tau0+CT-1}; both are CT-aligned runs in LDS, read as one
+// vector each.  The window of 2 CT values alternates between two register
+// halves (the loop is unrolled by two), so a step of CT slices costs two
+// vector LDS reads and CT^2 FMAs.  nt is a multiple of 2 CT.
+template <int CT, typename F> struct fs_run_of {
+    typedef F type __attribute__((ext_vector_type(CT)));
+};
+template <typename F> struct fs_run_of<1, F> {
+    typedef F type;
+};
+template <int CT, typename F> FS_DEV void fs_load_run(const F *p, F (&v)[CT])
+{
+    typedef typename fs_run_of<CT, F>::type R;
+    const R r = *reinterpret_cast<const R *>(p);
+    if constexpr (CT == 1) {
+        v[0] = r;
+    } else {
+#pragma unroll
+        for (int k = 0; k < CT; k++) v[k] = r[k];
+    }
+}
+template <int CT, typename F>
+FS_DEV void fs_cor_step(F (&acc)[CT], const F (&av)[CT], const F (&lo)[CT], const F (&hi)[CT])
+{
+#pragma unroll
+    for (int j = 0; j < CT; j++) {
+#pragma unroll
+        for (int k = 0; k < CT; k++) acc[k] += av[j] * (j + k < CT ? lo[j + k] : hi[j + k - CT]);
+    }
+}
 template <int CT, typename F, typename BC, typename WIN>
 FS_DEV void fs_cor_accumulate(F (&acc)[CT], int nt, BC bc, WIN win)
 {
-    F wv[2 * CT];
-#pragma unroll
-    for (int k = 0; k < CT; k++) wv[k] = win(k);
+    F w0[CT], w1[CT], av[CT];
+    fs_load_run<CT>(win(0), w0);
 #pragma unroll 1
-    for (int t = 0; t < nt; t += CT) {
-        F av[CT];
-#pragma unroll
-        for (int j = 0; j < CT; j++) av[j] = bc(t + j);
-#pragma unroll
-        for (int k = 0; k < CT; k++) wv[CT + k] = win(t + CT + k);
-#pragma unroll
-        for (int j = 0; j < CT; j++) {
-#pragma unroll
-            for (int k = 0; k < CT; k++) acc[k] += av[j] * wv[j + k];
-        }
-#pragma unroll
-        for (int k = 0; k < CT; k++) wv[k] = wv[CT + k];
+    for (int t = 0; t < nt; t += 2 * CT) {
+        fs_load_run<CT>(bc(t), av);
+        fs_load_run<CT>(win(t + CT), w1);
+        fs_cor_step<CT>(acc, av, w0, w1);
+        fs_load_run<CT>(bc(t + CT), av);
+        fs_load_run<CT>(win(t + 2 * CT), w0);
+        fs_cor_step<CT>(acc, av, w1, w0);
     }
 }
 
@@ -444,8 +465,8 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                             F acc[CT];
 #pragma unroll
                             for (int k = 0; k < CT; k++) acc[k] = (F)0.0;
-                            fs_cor_accumulate<CT>(acc, N, [&](int t) { return cbuf[t]; },
-                                                  [&](int i) { return cbuf[(i + tau0) & (N - 1)]; });
+                            fs_cor_accumulate<CT>(acc, N, [&](int t) { return cbuf + t; },
+                                                  [&](int i) { return cbuf + ((i + tau0) & (N - 1)); });
                             bin(c, tau0, acc);
                         }
                         fs_wave_sync();
@@ -469,8 +490,8 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                                 for (int i = lane; i < CH + LSPAN; i += 64)
                                     wnv[i] = cval(c, (t0 + lag0 + i) & (N - 1));
                                 fs_wave_sync();
-                                fs_cor_accumulate<CT>(acc, CH, [&](int t) { return bcv[t]; },
-                                                      [&](int i) { return wnv[i + lane * CT]; });
+                                fs_cor_accumulate<CT>(acc, CH, [&](int t) { return bcv + t; },
+                                                      [&](int i) { return wnv + i + lane * CT; });
                                 fs_wave_sync();
                             }
                             bin(c, tau0, acc);
