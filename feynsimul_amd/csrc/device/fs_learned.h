@@ -71,9 +71,10 @@ FS_DEV uint4 fs_philox(uint4 c, fs_u32 k0, fs_u32 k1)
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        const fs_u32 hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
-        const fs_u32 hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
-        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        // one 32 x 32 -> 64-bit product per multiplier (v_mad_u64_u32) gives both halves
+        const fs_u64 p0 = (fs_u64)0xD2511F53u * c.x;
+        const fs_u64 p1 = (fs_u64)0xCD9E8D57u * c.z;
+        c = make_uint4((fs_u32)(p1 >> 32) ^ c.y ^ k0, (fs_u32)p1, (fs_u32)(p0 >> 32) ^ c.w ^ k1, (fs_u32)p0);
     }
     return c;
 }

@@ -71,7 +71,9 @@ def _user_code(qscale):
 FLOAT_TYPE fs_sqr(FLOAT_TYPE a) { return a * a; }
 /* The pair energy at squared distance r2: one reciprocal square root gives
  * both 1/q and q, the exponentials are base 2 with log2(e) folded into their
- * coefficients, and eps/k is folded into every term (same function). */
+ * coefficients, eps/k is folded into every term (into the repulsive
+ * exponent as log2(eps/k A)), and the add-on window w in [0, 1] is one
+ * comparison |w - 1/2| <= 1/2 (same function). */
 FLOAT_TYPE fs_lm2m2_pair(FLOAT_TYPE r2)
 {
     const FLOAT_TYPE s = 1.0 / sqrt(r2);
@@ -82,8 +84,8 @@ FLOAT_TYPE fs_lm2m2_pair(FLOAT_TYPE r2)
     const FLOAT_TYPE t = fmax(%(Dl)r * iq - %(sl)r, 0.0);
     const FLOAT_TYPE damp = q > %(qmin)r ? exp2(-t * t) : 0.0;
     const FLOAT_TYPE w = (q - %(xa1)r) * %(iwidth)r;
-    const FLOAT_TYPE va = (w >= 0.0 && w <= 1.0) ? %(eAa)r * (1.0 - fs_cos2pi(w)) : 0.0;
-    return %(eA)r * exp2(q * (%(betal)r * q - %(alphal)r)) - damp * disp + va;
+    const FLOAT_TYPE va = fabs(w - 0.5) <= 0.5 ? %(eAa)r * (1.0 - fs_cos2pi(w)) : 0.0;
+    return exp2(q * (%(betal)r * q - %(alphal)r) + %(log2eA)r) - damp * disp + va;
 }
 /* V(q) + r dV/dr / 2 in K, the pair term of the virial energy estimator */
 FLOAT_TYPE fs_lm2m2_virial_q(FLOAT_TYPE q)
@@ -110,7 +112,7 @@ FLOAT_TYPE fs_lm2m2_virial_r2(FLOAT_TYPE r2) { return fs_lm2m2_virial_q(sqrt(r2)
 """ % dict(p, B=B, iwidth=1.0 / (p["xa2"] - p["xa1"]), qmin=0.0005 / p["rm"], qscale=qscale,
                   iqscale=1.0 / qscale, ec6=p["eps_k"] * p["c6"], ec8=p["eps_k"] * p["c8"],
                   ec10=p["eps_k"] * p["c10"], Dl=p["D"] * _SQRT_LOG2E, sl=_SQRT_LOG2E, eAa=p["eps_k"] * p["Aa"],
-                  eA=p["eps_k"] * p["A"], betal=p["beta"] * _LOG2E, alphal=p["alpha"] * _LOG2E)
+                  log2eA=math.log2(p["eps_k"] * p["A"]), betal=p["beta"] * _LOG2E, alphal=p["alpha"] * _LOG2E)
 
 
 class Lm2m2Cluster:
