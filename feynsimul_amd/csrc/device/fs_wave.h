@@ -34,15 +34,6 @@
 
 template <typename T> FS_DEV T fs_lane_read(T v, int src) { return __shfl(v, src, 64); }
 
-FS_DEV float fs_readlane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
-FS_DEV double fs_readlane(double v, int l)
-{
-    const long long b = __double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 // Element i of an array whose base is wave-uniform, addressed with a 32-bit
 // unsigned byte offset: the load or store then takes the base in SGPRs and the
 // offset in one VGPR, with no 64-bit address arithmetic per lane.
@@ -259,7 +250,9 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
             }
 #pragma unroll 1
             for (int g = 0; g < NB; g++) {
-                F xo[DOF], xn[DOF], xr[DOF], z[KDRAW];
+                // xb: this lane's slot in block g + 1; on lane r (position 0) that is the
+                // right end of the block's last segment, read from there by the bridge
+                F xo[DOF], xn[DOF], xb[DOF], z[KDRAW];
                 F vOld;
                 const int g1 = (g + 1) & (NB - 1);
                 int s = 0;
@@ -268,7 +261,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                     for (int d = 0; d < DOF; d++) {
                         const F a0 = xs[d][g], a1 = xs[d][g1];
                         xo[d] = wrap ? a1 : a0;
-                        xr[d] = fs_readlane(a1, r);
+                        xb[d] = a1;
                     }
                     const F v0 = vc[g], v1 = vc[g1];
                     vOld = wrap ? v1 : v0;
@@ -280,7 +273,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
 #pragma unroll
                     for (int d = 0; d < DOF; d++) xnext[d] = fs_at(row[d], snext);
 #pragma unroll
-                    for (int d = 0; d < DOF; d++) xr[d] = fs_readlane(xnext[d], r);
+                    for (int d = 0; d < DOF; d++) xb[d] = xnext[d];
                     if constexpr (Sys::POTCACHE) vOld = fs_at(pot, s);
                     else vOld = Sys::V(xo);
                 }
@@ -299,7 +292,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
 #pragma unroll
                 for (int d = 0; d < DOF; d++) {
                     const F xL = fs_lane_read(xo[d], laneL);
-                    const F xR = fs_lane_read(p == 0 ? xr[d] : xo[d], laneR);
+                    const F xR = fs_lane_read(p == 0 ? xb[d] : xo[d], laneR);
                     F x = xL + (xR - xL) * frac;
 #pragma unroll
                     for (int n = 1; n <= S; n++) x += wN[n] * fs_lane_read(z[d], srcN[n]);

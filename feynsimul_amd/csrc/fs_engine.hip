@@ -276,6 +276,11 @@ std::vector<std::string> compile_options(const fs_config *c, const std::string &
         opts.push_back("-fno-signed-zeros");
         opts.push_back("-fno-trapping-math");
         opts.push_back("-fapprox-func");
+        // FP32 subnormals flush to zero, so exp and log map to one v_exp_f32 /
+        // v_log_f32 each instead of a range-scaled sequence of six instructions
+        // (below 2^-126 no result of a move, a potential or an estimator is
+        // distinguishable from zero)
+        opts.push_back("-fgpu-flush-denormals-to-zero");
     }
     return opts;
 }
