@@ -57,12 +57,61 @@ FS_DEV long fs_uniform(long v)
     return (long)(((unsigned long long)hi << 32) | lo);
 }
 
-// Sum over aligned groups of G lanes (xor butterfly); every lane of a group
-// ends with the same bits.
+// One stage of an xor butterfly, v(l) + v(l ^ OFF), for a butterfly whose
+// stages with smaller offsets are done, so aligned groups of OFF lanes hold
+// equal values.  Every stage is a VALU lane exchange (no LDS round trip):
+// OFF 1 and 2 are quad permutations; 4 and 8 the half-row and row mirrors
+// (lane i <- 7 - i, 15 - i: a lane of the partner group, which holds the
+// partner's value); 16 and 32 the gfx950 row and half-wave swaps, after which
+// one register holds the even and the other the odd member of each pair.
+// Addition is commutative, so every lane ends with the bits of the shuffle
+// butterfly v += shfl_xor(v, OFF).
+template <int OFF, typename T> FS_DEV T fs_bfly_stage(T v)
+{
+    static_assert(sizeof(T) % 4 == 0, "32-bit words");
+    constexpr int NW = sizeof(T) / 4;
+    union words {
+        T v;
+        unsigned w[NW];
+    };
+    words a, b;
+    a.v = v;
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        if constexpr (OFF == 1) {
+            b.w[k] = __builtin_amdgcn_update_dpp(0u, a.w[k], 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+        } else if constexpr (OFF == 2) {
+            b.w[k] = __builtin_amdgcn_update_dpp(0u, a.w[k], 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+        } else if constexpr (OFF == 4) {
+            b.w[k] = __builtin_amdgcn_update_dpp(0u, a.w[k], 0x141, 0xF, 0xF, false);  // row_half_mirror
+        } else if constexpr (OFF == 8) {
+            b.w[k] = __builtin_amdgcn_update_dpp(0u, a.w[k], 0x140, 0xF, 0xF, false);  // row_mirror
+        } else if constexpr (OFF == 16) {
+            const auto sw = __builtin_amdgcn_permlane16_swap(a.w[k], a.w[k], false, false);
+            a.w[k] = sw[0];
+            b.w[k] = sw[1];
+        } else {
+            static_assert(OFF == 32, "offsets 1 .. 32");
+            const auto sw = __builtin_amdgcn_permlane32_swap(a.w[k], a.w[k], false, false);
+            a.w[k] = sw[0];
+            b.w[k] = sw[1];
+        }
+    }
+    if constexpr (OFF >= 16) return a.v + b.v;
+    else return v + b.v;
+}
+
+// Sum over aligned groups of G lanes (xor butterfly, offsets 1, 2, ..., G/2);
+// every lane of a group ends with the same bits.
 template <int G, typename T> FS_DEV T fs_group_sum(T v)
 {
-#pragma unroll
-    for (int off = 1; off < G; off <<= 1) v += __shfl_xor(v, off, 64);
+    static_assert(G >= 1 && G <= 64 && (G & (G - 1)) == 0, "groups of 1 .. 64 lanes");
+    if constexpr (G > 1) v = fs_bfly_stage<1>(v);
+    if constexpr (G > 2) v = fs_bfly_stage<2>(v);
+    if constexpr (G > 4) v = fs_bfly_stage<4>(v);
+    if constexpr (G > 8) v = fs_bfly_stage<8>(v);
+    if constexpr (G > 16) v = fs_bfly_stage<16>(v);
+    if constexpr (G > 32) v = fs_bfly_stage<32>(v);
     return v;
 }
 
@@ -302,11 +351,12 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                 const F dv = (ip != 0 && active) ? vNew - vOld : (F)0.0;
                 // segment sums and decisions in position space (lane L = position L)
                 const F dvp = fs_group_sum<SEG>(fs_lane_read(dv, toPos));
-                const F u0p = fs_lane_read(u[0], toPos);
-                const F ua = fs_lane_read(u0p, lane & ~(SEG - 1));
+                // the acceptance uniform: first draw of the lane that plays the segment's left end
+                const F ua = fs_lane_read(u[0], ((lane & ~(SEG - 1)) + r) & (B - 1));
                 const bool accp = !(exp(-(F)Sys::EPS * dvp) < ua);
                 if (accp && (lane & (SEG - 1)) == 0 && active) nacc++;
-                const bool acc = fs_lane_read((int)accp, p) != 0;
+                // the decision is equal on every position of a segment; lane j takes it at its position p
+                const bool acc = ((__ballot(accp) >> p) & 1ull) != 0;
                 const bool put = acc && ip != 0 && active;
                 if constexpr (Sys::RESIDENT) {
                     if constexpr (NB == 1) {
