@@ -46,12 +46,13 @@ def lm2m2_kelvin(r):
     """LM2M2 pair energy in K at distance r (angstrom); FP64 reference formula."""
     p = LM2M2
     x = r / p["rm"]
-    if x < p["D"]:
-        damp = math.exp(-(p["D"] / x - 1.0) ** 2) if x > 0.0005 / p["rm"] else 0.0
+    # the damped dispersion term, zero at and below x = 0.0005 / r_m (lm2m2.h:55-66)
+    if x <= 0.0005 / p["rm"]:
+        ddisp = 0.0
     else:
-        damp = 1.0
-    disp = p["c6"] / x ** 6 + p["c8"] / x ** 8 + p["c10"] / x ** 10
-    vb = p["A"] * math.exp(-p["alpha"] * x + p["beta"] * x * x) - damp * disp
+        disp = p["c6"] / x ** 6 + p["c8"] / x ** 8 + p["c10"] / x ** 10
+        ddisp = math.exp(-(p["D"] / x - 1.0) ** 2) * disp if x < p["D"] else disp
+    vb = p["A"] * math.exp(-p["alpha"] * x + p["beta"] * x * x) - ddisp
     va = 0.0
     if p["xa1"] <= x <= p["xa2"]:
         B = 2.0 * math.pi / (p["xa2"] - p["xa1"])
@@ -73,19 +74,23 @@ FLOAT_TYPE fs_sqr(FLOAT_TYPE a) { return a * a; }
  * both 1/q and q, the exponentials are base 2 with log2(e) folded into their
  * coefficients, eps/k is folded into every term (into the repulsive
  * exponent as log2(eps/k A)), and the add-on window w in [0, 1] is one
- * comparison |w - 1/2| <= 1/2 (same function). */
+ * comparison |w - 1/2| <= 1/2 (same function).  At and below q = 0.0005 / r_m
+ * the damped dispersion term is zero, as in the reference: there the damping
+ * factor exp2(-t^2) is exactly 0 (t > 7000), and the dispersion sum, whose
+ * 1/q^10 overflows first, is held finite by the clamp so that the product
+ * stays 0.  The same clamp on 1/r gives q = 0 at r2 = 0, where the pair
+ * energy is eps/k A; above the cut-off neither clamp changes a value. */
 FLOAT_TYPE fs_lm2m2_pair(FLOAT_TYPE r2)
 {
     const FLOAT_TYPE s = 1.0 / sqrt(r2);
-    const FLOAT_TYPE q = r2 * s * %(qscale)r;
+    const FLOAT_TYPE q = r2 * fmin(s, 3.0e38) * %(qscale)r;
     const FLOAT_TYPE iq = s * %(iqscale)r;
     const FLOAT_TYPE iq2 = iq * iq;
-    const FLOAT_TYPE disp = iq2 * iq2 * iq2 * (%(ec6)r + iq2 * (%(ec8)r + iq2 * %(ec10)r));
+    const FLOAT_TYPE disp = fmin(iq2 * iq2 * iq2 * (%(ec6)r + iq2 * (%(ec8)r + iq2 * %(ec10)r)), 3.0e38);
     const FLOAT_TYPE t = fmax(%(Dl)r * iq - %(sl)r, 0.0);
-    const FLOAT_TYPE damp = q > %(qmin)r ? exp2(-t * t) : 0.0;
     const FLOAT_TYPE w = (q - %(xa1)r) * %(iwidth)r;
     const FLOAT_TYPE va = fabs(w - 0.5) <= 0.5 ? %(eAa)r * (1.0 - fs_cos2pi(w)) : 0.0;
-    return exp2(q * (%(betal)r * q - %(alphal)r) + %(log2eA)r) - damp * disp + va;
+    return exp2(q * (%(betal)r * q - %(alphal)r) + %(log2eA)r) - exp2(-t * t) * disp + va;
 }
 /* V(q) + r dV/dr / 2 in K, the pair term of the virial energy estimator */
 FLOAT_TYPE fs_lm2m2_virial_q(FLOAT_TYPE q)
@@ -96,15 +101,19 @@ FLOAT_TYPE fs_lm2m2_virial_q(FLOAT_TYPE q)
     const FLOAT_TYPE disp = iq6 * (%(c6)r + iq2 * (%(c8)r + iq2 * %(c10)r));
     const FLOAT_TYPE qdisp = -iq6 * (6.0 * %(c6)r + iq2 * (8.0 * %(c8)r + iq2 * 10.0 * %(c10)r));
     const FLOAT_TYPE t = fmax(%(D)r * iq - 1.0, 0.0);
-    const FLOAT_TYPE damp = q > %(qmin)r ? exp(-t * t) : 0.0;
+    const FLOAT_TYPE damp = exp(-t * t);
     const FLOAT_TYPE qdamp = damp * 2.0 * t * (%(D)r * iq);
     const FLOAT_TYPE rep = %(A)r * exp(q * (%(beta)r * q - %(alpha)r));
     const FLOAT_TYPE w = (q - %(xa1)r) * %(iwidth)r;
     const bool win = w >= 0.0 && w <= 1.0;
     const FLOAT_TYPE va = win ? %(Aa)r * (1.0 - fs_cos2pi(w)) : 0.0;
     const FLOAT_TYPE qva = win ? %(Aa)r * %(B)r * q * fs_sin2pi(w) : 0.0;
-    const FLOAT_TYPE v = rep - damp * disp + va;
-    const FLOAT_TYPE qdv = rep * q * (2.0 * %(beta)r * q - %(alpha)r) - (qdamp * disp + damp * qdisp) + qva;
+    /* the damped dispersion term and its q-derivative, zero below q = 0.0005 / r_m */
+    const bool core = q > %(qmin)r;
+    const FLOAT_TYPE ddisp = core ? damp * disp : 0.0;
+    const FLOAT_TYPE qddisp = core ? qdamp * disp + damp * qdisp : 0.0;
+    const FLOAT_TYPE v = rep - ddisp + va;
+    const FLOAT_TYPE qdv = rep * q * (2.0 * %(beta)r * q - %(alpha)r) - qddisp + qva;
     return %(eps_k)r * (v + 0.5 * qdv);
 }
 FLOAT_TYPE fs_lm2m2_r2(FLOAT_TYPE r2) { return fs_lm2m2_pair(r2); }

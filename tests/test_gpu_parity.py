@@ -270,6 +270,24 @@ def test_system_section_matches_oracle(gpu):
         assert abs(acc[0] - ops[i, 0]) <= 1e-9 * max(1.0, abs(acc[0]))
 
 
+@pytest.mark.parametrize("double", [False, True])
+def test_lm2m2_core_on_gpu(gpu, double):
+    """The hiprtc-compiled LM2M2 pair energy and virial term through the core (r = 0 up to 2 A), FP32
+    with relaxed math and FP64: finite, with the reference's zero damped dispersion term at and below
+    x = 0.0005 / r_m, against the FP64 formula."""
+    from test_oracle import _core_points
+    _case("eval_system lm2m2 core double=%d" % double)
+    system = Lm2m2Cluster(3)
+    ka = _ka(system, 8, 64, 3, 10.0, 1, 1, True, False, double, "wave")
+    ka.fastMath = not double
+    k = PIMCKernel(ka)
+    pts, pot, vir = _core_points(system)
+    got, ops = k.evalSystem(pts)
+    assert np.isfinite(got).all() and np.isfinite(ops).all()
+    np.testing.assert_allclose(got, pot, rtol=1e-9 if double else 1e-5)
+    np.testing.assert_allclose(ops[:, 0], vir, rtol=1e-4)
+
+
 @pytest.mark.parametrize("storage", ["memory", "registers"])
 def test_wave_observable_histogram_matches_oracle(gpu, storage):
     """256-bin pair-distance histogram of the He3 cluster (the observables of every slice at every

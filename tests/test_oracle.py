@@ -383,6 +383,46 @@ def test_lm2m2_system_strings_in_oracle():
         assert abs(acc[1] - np.mean(dists)) < 1e-9
 
 
+# pair distances (angstrom) through the core: r = 0, below, at and above the 0.0005 A cut-off of the
+# damped dispersion term (lm2m2.h:55-66), where its 1/x^10 factor overflows FP32 first
+CORE_DISTANCES = (0.0, 1e-5, 1e-4, 4e-4, 5e-4, 5.2e-4, 1e-3, 1e-2, 0.3, 1.0, 2.0)
+
+
+def _core_points(sysobj):
+    """Atom 2 at distance r from atom 1 on the x axis, atom 3 far away; the exact pair sums."""
+    far = np.array([40.0, 40.0, 40.0])
+    pts, pot, vir = [], [], []
+    h = 1e-7
+    for r in CORE_DISTANCES:
+        atoms = np.array([[0.0, 0.0, 0.0], [r, 0.0, 0.0], far])
+        pts.append((atoms * sysobj.massScale).reshape(9))
+        d = [np.linalg.norm(atoms[i] - atoms[j]) for i, j in sysobj.pairs]
+        pot.append(sum(lm2m2_kelvin(x) for x in d))
+        vir.append(sum(lm2m2_kelvin(x) + 0.5 * x * (lm2m2_kelvin(x + h) - lm2m2_kelvin(abs(x - h))) / (2 * h)
+                       for x in d if x > 0) + (lm2m2_kelvin(0.0) if r == 0.0 else 0.0))
+    return np.array(pts), np.array(pot), np.array(vir)
+
+
+@pytest.mark.parametrize("double", [False, True])
+def test_lm2m2_core_matches_reference_branches(double):
+    """Inside the core the generated pair energy keeps the reference's branches: the damped
+    dispersion term is zero at and below x = 0.0005 / r_m, and r = 0 gives eps/k A (2.08e6 K),
+    finite in FP32 as in FP64 (no 0 * inf from the overflowing 1/x^10)."""
+    sysobj = Lm2m2Cluster(3)
+    assert abs(lm2m2_kelvin(0.0) - 10.97 * 1.89635353e5) < 1e-6
+    lib = O.build(sysobj, 64, 10.0, operators=(sysobj.energyOp,), double=double)
+    pts, pot, vir = _core_points(sysobj)
+    ft = np.float64 if double else np.float32
+    tol = 1e-9 if double else 2e-6
+    for x, v, e in zip(pts, pot, vir):
+        xx = np.ascontiguousarray(x.astype(ft))
+        got = lib.fs_oracle_potential(xx.ctypes.data)
+        assert np.isfinite(got) and abs(got - v) <= tol * abs(v), (got, v)
+        acc = np.zeros(1, ft)
+        lib.fs_oracle_operators(xx.ctypes.data, acc.ctypes.data)
+        assert np.isfinite(acc[0]) and abs(acc[0] - e) <= 1e-4 * abs(e), (acc[0], e)
+
+
 @pytest.mark.parametrize("moves", ["shift", "single", "all"])
 def test_path_shift_and_single_node_keep_the_exact_law(moves):
     """Path-shift (kernel.c:407-473, 640-663) and single-node (kernel.c:665-703) moves,
