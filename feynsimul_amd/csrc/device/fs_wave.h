@@ -34,6 +34,21 @@
 
 template <typename T> FS_DEV T fs_lane_read(T v, int src) { return __shfl(v, src, 64); }
 
+// The value of lane l (wave-uniform l) in scalar registers, one v_readlane_b32
+// per 32-bit word.
+template <typename T> FS_DEV T fs_readlane(T v, int l)
+{
+    static_assert(sizeof(T) % 4 == 0, "32-bit words");
+    union words {
+        T v;
+        int w[sizeof(T) / 4];
+    } a;
+    a.v = v;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) a.w[k] = __builtin_amdgcn_readlane(a.w[k], l);
+    return a.v;
+}
+
 // Element i of an array whose base is wave-uniform, addressed with a 32-bit
 // unsigned byte offset: the load or store then takes the base in SGPRs and the
 // offset in one VGPR, with no 64-bit address arithmetic per lane.
@@ -259,7 +274,9 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
 
     for (int i = 0; i < a.operatorRuns; i++) {
         for (int j = 0; j < a.metroSteps; j++) {
-            const int r = (int)grp.below((fs_u32)SEG);
+            // the group stream is the walker's, so r is wave-uniform: keep it in a scalar register
+            const int r = __builtin_amdgcn_readfirstlane((int)grp.below((fs_u32)SEG));
+            const int srcN1 = ((SEG >> 1) + r) & (B - 1);  // the level-1 node when SEG == 64
             // this lane plays block position p; its slice is r + gB + p
             const int p = active ? ((lane - r) & (B - 1)) : lane;
             const int ip = p & (SEG - 1);
@@ -340,11 +357,23 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                 }
 #pragma unroll
                 for (int d = 0; d < DOF; d++) {
-                    const F xL = fs_lane_read(xo[d], laneL);
-                    const F xR = fs_lane_read(p == 0 ? xb[d] : xo[d], laneR);
-                    F x = xL + (xR - xL) * frac;
+                    F x;
+                    if constexpr (SEG == 64) {
+                        // one segment per block: both ends (lane r of this block and of the
+                        // next) and the level-1 node are the same for every lane, so they
+                        // come from scalar lane reads instead of LDS permutes
+                        const F xL = fs_readlane(xo[d], r);
+                        const F xR = fs_readlane(xb[d], r);
+                        x = xL + (xR - xL) * frac;
+                        x += wN[1] * fs_readlane(z[d], srcN1);
+                    } else {
+                        const F xL = fs_lane_read(xo[d], laneL);
+                        const F xR = fs_lane_read(p == 0 ? xb[d] : xo[d], laneR);
+                        x = xL + (xR - xL) * frac;
+                        x += wN[1] * fs_lane_read(z[d], srcN[1]);
+                    }
 #pragma unroll
-                    for (int n = 1; n <= S; n++) x += wN[n] * fs_lane_read(z[d], srcN[n]);
+                    for (int n = 2; n <= S; n++) x += wN[n] * fs_lane_read(z[d], srcN[n]);
                     xn[d] = x;
                 }
                 const F vNew = Sys::V(xn);
