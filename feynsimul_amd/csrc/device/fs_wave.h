@@ -34,6 +34,21 @@
 
 template <typename T> FS_DEV T fs_lane_read(T v, int src) { return __shfl(v, src, 64); }
 
+// The value of lane (l + 1) mod 64 on every lane l: the DPP wave rotation
+// (wave_rol:1), one VALU lane exchange per 32-bit word.
+template <typename T> FS_DEV T fs_lane_next(T v)
+{
+    static_assert(sizeof(T) % 4 == 0, "32-bit words");
+    union words {
+        T v;
+        int w[sizeof(T) / 4];
+    } a;
+    a.v = v;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) a.w[k] = __builtin_amdgcn_mov_dpp(a.w[k], 0x134, 0xF, 0xF, true);
+    return a.v;
+}
+
 // The value of lane l (wave-uniform l) in scalar registers, one v_readlane_b32
 // per 32-bit word.
 template <typename T> FS_DEV T fs_readlane(T v, int l)
@@ -221,7 +236,6 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
     F sigtab[S + 1];
 #pragma unroll
     for (int n = 1; n <= S; n++) sigtab[n] = sqrt(((F)SEG * (F)Sys::EPS / ((F)2.0 * (F)(1 << n))));
-    sigtab[0] = sigtab[1];
 
     // lane j draws from stream 64 w + j, i.e. the stream follows the slice residue
     fs_stream rng, grp;
@@ -282,16 +296,13 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
             const int ip = p & (SEG - 1);
             const bool wrap = lane < r;              // slice lies in the next block of slots
             const int toPos = (lane + r) & (B - 1);  // lane that plays position `lane`
-            const int lvl = ip ? S - __builtin_ctz((unsigned)ip) : 0;
-            F sig = sigtab[0];
-#pragma unroll
-            for (int n = 1; n <= S; n++) sig = (lvl == n) ? sigtab[n] : sig;
             // The Levy bisection of a segment, written in closed form: the
             // midpoint recursion x_m = (x_{m-h} + x_{m+h}) / 2 + sigma_n z_m
             // resolves to the linear interpolation of the two ends plus, for
             // each level n, the level-n node covering position i times the tent
             // 1 - |i - m_n| / h_n (zero at nodes of coarser levels and at the
-            // ends).  Same bridge, no per-level masking.
+            // ends).  Same bridge, no per-level masking.  The node's sigma_n is
+            // folded into the weight, so the normals z are drawn with unit variance.
             const int segL = p - ip;
             const int laneL = (segL + r) & (B - 1);
             const int laneR = (segL + SEG + r) & (B - 1);  // lane r when the end is the block end
@@ -303,8 +314,14 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                 const int h = SEG >> n;
                 const int m = (ip & ~(2 * h - 1)) + h;
                 srcN[n] = (segL + m + r) & (B - 1);
-                wN[n] = (F)1.0 - (F)abs(ip - m) * ((F)1.0 / (F)h);
+                wN[n] = ((F)1.0 - (F)abs(ip - m) * ((F)1.0 / (F)h)) * sigtab[n];
             }
+            // With 64-slice blocks the finest node (h = 1) of position p is p + 1 for even p
+            // and p itself for odd p, i.e. lane j + 1 (mod 64) or lane j: a wave rotation, no
+            // LDS permute.  Its tent weight is split between the two sources.
+            constexpr bool FINE_DPP = B == 64;
+            const bool pEven = (p & 1) == 0;
+            const F wFineNext = pEven ? wN[S] : (F)0.0, wFineOwn = pEven ? (F)0.0 : wN[S];
 
             // memory storage: block g + 1 is loaded while block g is moved; its
             // position 0 is the right end of block g, which no move of this step changes
@@ -349,9 +366,10 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                 for (int k = 0; k < KDRAW; k++) u[k] = rng.uniform();
 #pragma unroll
                 for (int k = 0; k < KPAIR; k++) {
-                    // v == 0 gives a zero displacement, as kernel.c:346-349 does
+                    // v == 0 gives a zero displacement, as kernel.c:346-349 does; -2 ln v is
+                    // written with the base-2 logarithm the hardware computes
                     const F v = u[2 * k + 1];
-                    const F rho = v != (F)0.0 ? sig * sqrt(-2.0 * log(v)) : (F)0.0;
+                    const F rho = v != (F)0.0 ? sqrt((F)(-2.0 * 0.69314718055994531) * log2(v)) : (F)0.0;
                     z[2 * k] = rho * fs_sin2pi(u[2 * k]);
                     z[2 * k + 1] = rho * fs_cos2pi(u[2 * k]);
                 }
@@ -365,15 +383,22 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                         const F xL = fs_readlane(xo[d], r);
                         const F xR = fs_readlane(xb[d], r);
                         x = xL + (xR - xL) * frac;
-                        x += wN[1] * fs_readlane(z[d], srcN1);
                     } else {
                         const F xL = fs_lane_read(xo[d], laneL);
                         const F xR = fs_lane_read(p == 0 ? xb[d] : xo[d], laneR);
                         x = xL + (xR - xL) * frac;
-                        x += wN[1] * fs_lane_read(z[d], srcN[1]);
                     }
 #pragma unroll
-                    for (int n = 2; n <= S; n++) x += wN[n] * fs_lane_read(z[d], srcN[n]);
+                    for (int n = 1; n <= S; n++) {
+                        if (FINE_DPP && n == S) {
+                            x += wFineNext * fs_lane_next(z[d]);
+                            x += wFineOwn * z[d];
+                        } else if (SEG == 64 && n == 1) {
+                            x += wN[1] * fs_readlane(z[d], srcN1);
+                        } else {
+                            x += wN[n] * fs_lane_read(z[d], srcN[n]);
+                        }
+                    }
                     xn[d] = x;
                 }
                 const F vNew = Sys::V(xn);
@@ -381,8 +406,11 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                 // segment sums and decisions in position space (lane L = position L)
                 const F dvp = fs_group_sum<SEG>(fs_lane_read(dv, toPos));
                 // the acceptance uniform: first draw of the lane that plays the segment's left end
-                const F ua = fs_lane_read(u[0], ((lane & ~(SEG - 1)) + r) & (B - 1));
-                const bool accp = !(exp(-(F)Sys::EPS * dvp) < ua);
+                F ua;
+                if constexpr (SEG == 64) ua = fs_readlane(u[0], r);
+                else ua = fs_lane_read(u[0], ((lane & ~(SEG - 1)) + r) & (B - 1));
+                // exp(-eps dS) as exp2 with log2(e) folded into the constant
+                const bool accp = !(exp2((F)(-1.4426950408889634 * (double)Sys::EPS) * dvp) < ua);
                 if (accp && (lane & (SEG - 1)) == 0 && active) nacc++;
                 // the decision is equal on every position of a segment; lane j takes it at its position p
                 const bool acc = ((__ballot(accp) >> p) & 1ull) != 0;
