@@ -331,7 +331,10 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
 #pragma unroll
                 for (int d = 0; d < DOF; d++) xnext[d] = fs_at(row[d], snext);
             }
-#pragma unroll 1
+            // two blocks per iteration in memory storage: the loaded block and the
+            // stream state then change registers in place, with no copies at the loop edge
+            constexpr int BLOCK_UNROLL = NB >= 2 && !Sys::RESIDENT ? 2 : 1;
+#pragma unroll BLOCK_UNROLL
             for (int g = 0; g < NB; g++) {
                 // xb: this lane's slot in block g + 1; on lane r (position 0) that is the
                 // right end of the block's last segment, read from there by the bridge
