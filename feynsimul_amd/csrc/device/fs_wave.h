@@ -385,7 +385,9 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                         // come from scalar lane reads instead of LDS permutes
                         const F xL = fs_readlane(xo[d], r);
                         const F xR = fs_readlane(xb[d], r);
-                        x = xL + (xR - xL) * frac;
+                        // (1 - frac) xL + frac xR as two FMAs that read the scalar ends directly
+                        x = xL - frac * xL;
+                        x += frac * xR;
                     } else {
                         const F xL = fs_lane_read(xo[d], laneL);
                         const F xR = fs_lane_read(p == 0 ? xb[d] : xo[d], laneR);
