@@ -114,6 +114,11 @@ FS_DEV void fs_lp_bias_init(const float *b, int ob, int g, fs_f16v (&c)[FS_LP_CB
     }
 }
 
+// ReLU as an integer max on the FP32 bit pattern: a float with its sign bit set
+// is a negative 32-bit integer, so max(bits, 0) zeroes exactly the negative
+// values, one v_max_i32 each (a float max would first quiet signalling NaNs,
+// a second instruction per value); then pairs round to BF16 in one
+// v_cvt_pk_bf16_f32.
 FS_DEV void fs_lp_relu_pack(const fs_f16v (&acc)[4][FS_LP_CB], fs_bf8 (&h)[8][FS_LP_CB])
 {
 #pragma unroll
@@ -123,7 +128,8 @@ FS_DEV void fs_lp_relu_pack(const fs_f16v (&acc)[4][FS_LP_CB], fs_bf8 (&h)[8][FS
 #pragma unroll
             for (int hh = 0; hh < 2; hh++)
 #pragma unroll
-                for (int e = 0; e < 8; e++) h[2 * ob + hh][cb][e] = (__bf16)fmaxf(acc[ob][cb][8 * hh + e], 0.0f);
+                for (int e = 0; e < 8; e++)
+                    h[2 * ob + hh][cb][e] = (__bf16)__int_as_float(max(__float_as_int(acc[ob][cb][8 * hh + e]), 0));
 }
 
 FS_DEV void fs_lp_hidden(const char *img, int woff, int boff, const fs_bf8 (&hin)[8][FS_LP_CB],
