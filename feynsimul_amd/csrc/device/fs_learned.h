@@ -276,10 +276,15 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
 #pragma unroll 1
         for (int t0 = 0; t0 < total; t0 += 32 * FS_LP_CB) {
             fs_bf8 xin[2][FS_LP_CB];
+            // midpoint of the row's ends for this lane's coordinates d = 2q + g, kept
+            // across the network instead of reading the ends again
+            float mid[FS_LP_CB][8];
 #pragma unroll
             for (int cb = 0; cb < FS_LP_CB; cb++) {
                 const int R = t0 + cb * 32 + c32;
                 float f[32];
+#pragma unroll
+                for (int q = 0; q < 8; q++) mid[cb][q] = 0.0f;
                 if (R < total) {
                     const bool old = R >= NR;
                     const int Rl = old ? R - NR : R, k = Rl >> (n - 1), m = (2 * (Rl & (nn - 1)) + 1) * h;
@@ -288,6 +293,13 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
                     for (int d = 0; d < D; d++) {
                         ea[d] = value(k, m - h, old, d);
                         eb[d] = value(k, m + h, old, d);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const float m0 = 2 * q < D ? 0.5f * (ea[2 * q < D ? 2 * q : 0] + eb[2 * q < D ? 2 * q : 0]) : 0.0f;
+                        const float m1 =
+                            2 * q + 1 < D ? 0.5f * (ea[2 * q + 1 < D ? 2 * q + 1 : 0] + eb[2 * q + 1 < D ? 2 * q + 1 : 0]) : 0.0f;
+                        mid[cb][q] = g ? m1 : m0;
                     }
                     fs_lp_features<D>(ea, eb, level, a.inScale, f);
                 } else {
@@ -312,8 +324,7 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
                         const int d = 2 * q + g;
                         if (q % 2 == 0 && !old) px = fs_philox(make_uint4((fs_u32)s, wl, a.sweep, FS_LP_TAG_NORMAL | (q / 2)), a.key0, a.key1);
                         if (d < D) {
-                            const float mid = 0.5f * (value(k, m - h, old, d) + value(k, m + h, old, d));
-                            const float mu = mid + sig * out[cb][2 * q];
+                            const float mu = mid[cb][q] + sig * out[cb][2 * q];
                             const float sv = fminf(fmaxf(out[cb][2 * q + 1], -FS_LP_SMAX), FS_LP_SMAX);
                             const float sd = sig * expf(sv);
                             const float xo = path[d * N + s];
