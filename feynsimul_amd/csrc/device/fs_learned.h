@@ -45,6 +45,7 @@
 // SIMD).
 #pragma once
 
+#include "fs_lane.h"
 #include "fs_lp_abi.h"
 #include "fs_rng.h"
 

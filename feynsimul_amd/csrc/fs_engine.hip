@@ -38,6 +38,9 @@ const char kWaveHeader[] = {
 const char kStreamHeader[] = {
 #embed "device/fs_stream.h"
     , 0};
+const char kLaneHeader[] = {
+#embed "device/fs_lane.h"
+    , 0};
 
 thread_local std::string g_error;
 
@@ -385,9 +388,9 @@ int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs
     e->source = build_source(cfg, system_src, wpb);
 
     hiprtcProgram prog;
-    const char *hdrs[] = {kRngHeader, kArgsHeader, kSamplerHeader, kWaveHeader, kStreamHeader};
-    const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_sampler.h", "fs_wave.h", "fs_stream.h"};
-    if (hiprtcCreateProgram(&prog, e->source.c_str(), "fs_module.hip", 5, hdrs, names) != HIPRTC_SUCCESS) {
+    const char *hdrs[] = {kRngHeader, kArgsHeader, kSamplerHeader, kWaveHeader, kStreamHeader, kLaneHeader};
+    const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_sampler.h", "fs_wave.h", "fs_stream.h", "fs_lane.h"};
+    if (hiprtcCreateProgram(&prog, e->source.c_str(), "fs_module.hip", 6, hdrs, names) != HIPRTC_SUCCESS) {
         delete e;
         return fail(FS_ERR_COMPILE, "hiprtcCreateProgram failed");
     }
@@ -764,9 +767,9 @@ int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32
     b->smem = lds * wpb;
     b->source = batch_source(b->cfgs, system_srcs, wpb, lds);
     hiprtcProgram prog;
-    const char *hdrs[] = {kRngHeader, kArgsHeader, kWaveHeader, kStreamHeader};
-    const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_wave.h", "fs_stream.h"};
-    if (hiprtcCreateProgram(&prog, b->source.c_str(), "fs_batch.hip", 4, hdrs, names) != HIPRTC_SUCCESS) {
+    const char *hdrs[] = {kRngHeader, kArgsHeader, kWaveHeader, kStreamHeader, kLaneHeader};
+    const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_wave.h", "fs_stream.h", "fs_lane.h"};
+    if (hiprtcCreateProgram(&prog, b->source.c_str(), "fs_batch.hip", 5, hdrs, names) != HIPRTC_SUCCESS) {
         batch_free(b);
         return fail(FS_ERR_COMPILE, "hiprtcCreateProgram failed");
     }
