@@ -82,7 +82,9 @@ template <> FS_DEV double fs_xs_float<double>(fs_xs128 &s) { return fs_xs_float6
 // RANLUX
 // ---------------------------------------------------------------------------
 // In-memory layout (112 bytes): s01..s24, carry, dummy, in24, stepnr -- the
-// field order of ranluxcl_state_t, so a state table is interchangeable.
+// field order of ranluxcl_state_t, so a state table is interchangeable.  In
+// registers `carry` is kept in units of 2^-24 (0 or 1); fs_rl_load and
+// fs_rl_store convert.
 struct fs_ranlux {
     float s[24];
     float carry;
@@ -103,22 +105,22 @@ __host__ __device__ constexpr int fs_ranlux_nskip(int lux)
 // One SWB step at cycle position T: x = s[lag 10] - s[lag 24] - carry, taken
 // mod 1 with a borrow; the result replaces the lag-24 entry.  Values below
 // 2^-12 get 24 more low bits from the next lag-10 entry (never 0 returned).
+// Every value is a multiple of 2^-24 in (-1, 1), so each operation below is
+// exact: fract(x) is x + 1 for the negative differences (the borrow), the new
+// carry in units of 2^-24 is fract(x) - x, and the low-bit fill is one FMA
+// whose result is 0 or at least 2^-48, so the max replaces exactly a 0 by
+// 2^-48.  Four instructions per step and no branches.
 template <int T> FS_DEV float fs_rl_step(fs_ranlux &r)
 {
     constexpr int tgt = 23 - T;
     constexpr int lag10 = (9 - T + 24) % 24;
     constexpr int lag9 = (8 - T + 24) % 24;
-    float uni = r.s[lag10] - r.s[tgt] - r.carry;
-    const bool borrow = uni < 0.0f;
-    uni = borrow ? uni + 1.0f : uni;
-    r.carry = borrow ? FS_TWOM24 : 0.0f;
+    const float x = fmaf(r.carry, -FS_TWOM24, r.s[lag10] - r.s[tgt]);
+    const float uni = __builtin_amdgcn_fractf(x);
+    r.carry = uni - x;
     r.s[tgt] = uni;
-    float out = uni;
-    if (uni < FS_TWOM12) {
-        out += FS_TWOM24 * r.s[lag9];
-        if (out == 0.0f) out = FS_TWOM24 * FS_TWOM24;
-    }
-    return out;
+    const float out = fmaf(uni < FS_TWOM12 ? FS_TWOM24 : 0.0f, r.s[lag9], uni);
+    return fmaxf(out, FS_TWOM24 * FS_TWOM24);
 }
 
 // Steps [FIRST, LAST) of one cycle, outputs discarded.
@@ -138,28 +140,34 @@ template <int T0> FS_DEV void fs_rl_quad(fs_ranlux &r, float &a, float &b, float
     d = fs_rl_step<T0 + 3>(r);
 }
 
-// Luxury skip of NSKIP values starting at the current cycle position.
+// Luxury skip of NSKIP values starting at cycle position T0 (a multiple of 4):
+// finish the cycle, run the whole cycles, then the first `rest` steps of the
+// next one; every count is a compile-time constant for a given T0.
+template <int NSKIP, int T0> FS_DEV void fs_rl_skip_from(fs_ranlux &r)
+{
+    constexpr int initial = T0 ? 24 - T0 : 0;
+    constexpr int cycles = (NSKIP - initial) / 24;
+    constexpr int rest = NSKIP - initial - cycles * 24;
+    static_assert(NSKIP >= initial && rest % 4 == 0, "skips restart at a multiple of four");
+    if constexpr (T0 != 0) fs_rl_run<T0, 24>(r);
+#pragma unroll 1
+    for (int c = 0; c < cycles; c++) fs_rl_run<0, 24>(r);
+    if constexpr (rest > 0) fs_rl_run<0, rest>(r);
+    r.stepnr = rest;
+}
+
+// Luxury skip of NSKIP values starting at the current cycle position (one
+// branch on the position, which every lane of a sampler shares).
 template <int NSKIP> FS_DEV void fs_rl_skip(fs_ranlux &r)
 {
-    // initial skips finish the current cycle
     switch (r.stepnr) {
-    case 4: fs_rl_run<4, 24>(r); break;
-    case 8: fs_rl_run<8, 24>(r); break;
-    case 12: fs_rl_run<12, 24>(r); break;
-    case 16: fs_rl_run<16, 24>(r); break;
-    case 20: fs_rl_run<20, 24>(r); break;
-    default: break;
+    case 4: fs_rl_skip_from<NSKIP, 4>(r); break;
+    case 8: fs_rl_skip_from<NSKIP, 8>(r); break;
+    case 12: fs_rl_skip_from<NSKIP, 12>(r); break;
+    case 16: fs_rl_skip_from<NSKIP, 16>(r); break;
+    case 20: fs_rl_skip_from<NSKIP, 20>(r); break;
+    default: fs_rl_skip_from<NSKIP, 0>(r); break;
     }
-    const int initial = r.stepnr ? 24 - r.stepnr : 0;
-    const int cycles = (NSKIP - initial) / 24;
-    const int rest = NSKIP - initial - cycles * 24;
-    for (int c = 0; c < cycles; c++) fs_rl_run<0, 24>(r);
-    if (rest > 0) fs_rl_run<0, 4>(r);
-    if (rest > 4) fs_rl_run<4, 8>(r);
-    if (rest > 8) fs_rl_run<8, 12>(r);
-    if (rest > 12) fs_rl_run<12, 16>(r);
-    if (rest > 16) fs_rl_run<16, 20>(r);
-    r.stepnr = rest;
 }
 
 // ranluxcl32: four values in (0, 1).
@@ -225,7 +233,7 @@ FS_DEV void fs_rl_init(fs_u64 x, fs_ranlux &r)
     r.dummy = 0.0f;
     r.in24 = 0;
     r.stepnr = 0;
-    r.carry = (r.s[23] == 0.0f) ? FS_TWOM24 : 0.0f;
+    r.carry = (r.s[23] == 0.0f) ? 1.0f : 0.0f;
     for (int i = 0; i < 32; i++) fs_rl_run<0, 24>(r);
 }
 
@@ -241,7 +249,7 @@ FS_DEV void fs_rl_load(const fs_ranlux *__restrict__ tab, long idx, fs_ranlux &r
         r.s[4 * q + 3] = v.w;
     }
     float4 t = p[6];
-    r.carry = t.x;
+    r.carry = t.x * 16777216.0f;
     r.dummy = t.y;
     r.in24 = __float_as_int(t.z);
     r.stepnr = __float_as_int(t.w);
@@ -252,7 +260,7 @@ FS_DEV void fs_rl_store(fs_ranlux *__restrict__ tab, long idx, const fs_ranlux &
     float4 *p = reinterpret_cast<float4 *>(tab + idx);
 #pragma unroll
     for (int q = 0; q < 6; q++) p[q] = make_float4(r.s[4 * q], r.s[4 * q + 1], r.s[4 * q + 2], r.s[4 * q + 3]);
-    p[6] = make_float4(r.carry, r.dummy, __int_as_float(r.in24), __int_as_float(r.stepnr));
+    p[6] = make_float4(r.carry * FS_TWOM24, r.dummy, __int_as_float(r.in24), __int_as_float(r.stepnr));
 }
 
 // Per-launch buffered draw (kernel.c:180-207): refill four values whenever the
