@@ -72,10 +72,12 @@ FS_DEV uint4 fs_philox(uint4 c, fs_u32 k0, fs_u32 k1)
             k0 += 0x9E3779B9u;
             k1 += 0xBB67AE85u;
         }
-        // one 32 x 32 -> 64-bit product per multiplier (v_mad_u64_u32) gives both halves
+        // one 32 x 32 -> 64-bit product per multiplier (v_mad_u64_u32) gives both halves, and
+        // each three-input exclusive or is one v_bitop3_b32 (truth table 0x96)
         const fs_u64 p0 = (fs_u64)0xD2511F53u * c.x;
         const fs_u64 p1 = (fs_u64)0xCD9E8D57u * c.z;
-        c = make_uint4((fs_u32)(p1 >> 32) ^ c.y ^ k0, (fs_u32)p1, (fs_u32)(p0 >> 32) ^ c.w ^ k1, (fs_u32)p0);
+        c = make_uint4(__builtin_amdgcn_bitop3_b32((fs_u32)(p1 >> 32), c.y, k0, 0x96), (fs_u32)p1,
+                       __builtin_amdgcn_bitop3_b32((fs_u32)(p0 >> 32), c.w, k1, 0x96), (fs_u32)p0);
     }
     return c;
 }
