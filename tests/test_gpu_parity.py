@@ -201,6 +201,10 @@ CASES_WAVE = [
     ("lm2m2", 16, 256, 6, 10.0, False, False, "registers"),
     ("lm2m2", 16, 1024, 6, 10.0, False, False, "registers"),
     ("lm2m2", 16, 1024, 6, 10.0, False, False, "memory"),
+    # several segments per 64-slice block with the potential cache; 2^S = N < 64; RANLUX with ten draws per block
+    ("lm2m2", 16, 512, 4, 10.0, False, False, "memory"),
+    ("ho", 32, 32, 5, 4.0, False, False, None),
+    ("lm2m2", 16, 256, 5, 10.0, True, False, "memory"),
 ]
 
 
