@@ -331,7 +331,12 @@ struct fs_sys_main {
     static FS_DEV void hobs(const FLOAT_TYPE *x, FLOAT_TYPE *out) { fs_hist_obs_v(x, out); }
 };
 
-extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_wave_sweep(fs_kargs a)
+#if defined(FS_MIN_WAVES) && FS_MIN_WAVES > 0
+#define FS_WAVE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(FS_MIN_WAVES)))
+#else
+#define FS_WAVE_OCCUPANCY
+#endif
+extern "C" __global__ __launch_bounds__(64 * FS_WPB) FS_WAVE_OCCUPANCY void fs_wave_sweep(fs_kargs a)
 {
     const int lane = threadIdx.x & 63;
     const long w = (long)blockIdx.x * FS_WPB + (threadIdx.x >> 6);

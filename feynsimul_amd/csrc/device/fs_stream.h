@@ -41,3 +41,74 @@ struct fs_stream {
 #endif
 };
 
+#if FS_RANLUX
+// The RANLUX stream that every lane of a wavefront holds in the same state
+// (the wave sampler's group stream, loaded by every lane from one table
+// entry).  The state sits in LDS between refills, in register format, so its
+// 26 words take registers only while a refill runs instead of through the
+// whole sweep; the cycle position is read into a scalar register, so the
+// refill's position switch is a uniform branch.  Lane 0 writes the state
+// back; the fences keep the next refill from reusing a register copy.
+struct fs_wave_stream {
+    typedef fs_f vec4 __attribute__((ext_vector_type(4)));
+    fs_ranlux *sh;
+    vec4 buf;
+    fs_u32 cnt;
+
+    FS_DEV void load(const fs_ranlux *tab, long idx, fs_ranlux *lds)
+    {
+        sh = lds;
+        fs_ranlux st;
+        fs_rl_load(tab, idx, st);
+        if (__lane_id() == 0) *sh = st;
+        cnt = 0u;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // lane 0 writes the table entry
+    FS_DEV void store(fs_ranlux *tab, long idx) const
+    {
+        if (__lane_id() == 0) {
+            const fs_ranlux st = *sh;
+            fs_rl_store(tab, idx, st);
+        }
+    }
+    FS_DEV fs_f uniform()
+    {
+        if ((cnt & 3u) == 0u) {
+            fs_ranlux st = *sh;
+            st.stepnr = __builtin_amdgcn_readfirstlane(st.stepnr);
+            st.in24 = __builtin_amdgcn_readfirstlane(st.in24);
+            if constexpr (sizeof(fs_f) == 8) {
+                double v[4];
+                fs_rl_next4_double<FS_NSKIP>(st, v);
+                buf = vec4{(fs_f)v[0], (fs_f)v[1], (fs_f)v[2], (fs_f)v[3]};
+            } else {
+                float v[4];
+                fs_rl_next4<FS_NSKIP>(st, v);
+                buf = vec4{v[0], v[1], v[2], v[3]};
+            }
+            // the four values are the same in every lane: scalar registers
+            if constexpr (sizeof(fs_f) == 4) {
+                buf.x = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(buf.x)));
+                buf.y = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(buf.y)));
+                buf.z = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(buf.z)));
+                buf.w = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(buf.w)));
+            }
+            if (__lane_id() == 0) *sh = st;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        const fs_u32 k = (cnt++) & 3u;
+        const vec4 b = buf;
+        fs_f v = b.x;
+        v = (k == 1u) ? b.y : v;
+        v = (k == 2u) ? b.z : v;
+        v = (k == 3u) ? b.w : v;
+        return v;
+    }
+    FS_DEV fs_u32 below(fs_u32 n) { return (fs_u32)(uniform() * (fs_f)n); }
+};
+#endif

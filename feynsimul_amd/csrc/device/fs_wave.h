@@ -34,11 +34,16 @@
 #include "fs_stream.h"
 
 // Bytes of LDS one wavefront uses for a system: correlator staging, then a
-// private histogram of the scalar observables; rounded up to 16 bytes.
-template <class Sys> constexpr int fs_wave_lds_bytes()
+// private histogram of the scalar observables, rounded up to 16 bytes; then
+// the group stream's RANLUX state.
+template <class Sys> constexpr int fs_wave_meas_bytes()
 {
     return ((Sys::NCORR > 0 ? fs_cor_stage_len(Sys::N, Sys::CORLAGS) * (int)sizeof(FLOAT_TYPE) : 0) +
             (Sys::NHOBS > 0 ? Sys::HBINS * 4 : 0) + 15) & ~15;
+}
+template <class Sys> constexpr int fs_wave_lds_bytes()
+{
+    return fs_wave_meas_bytes<Sys>() + (FS_RANLUX ? (int)sizeof(fs_ranlux) : 0);
 }
 
 FS_DEV void fs_wave_sync()
@@ -126,11 +131,13 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
     for (int n = 1; n <= S; n++) sigtab[n] = sqrt(((F)SEG * (F)Sys::EPS / ((F)2.0 * (F)(1 << n))));
 
     // lane j draws from stream 64 w + j, i.e. the stream follows the slice residue
-    fs_stream rng, grp;
 #if FS_RANLUX
+    fs_stream rng;
+    fs_wave_stream grp;
     rng.load(a.rl, w * 64 + lane);
-    grp.load(a.grl, w);
+    grp.load(a.grl, w, reinterpret_cast<fs_ranlux *>(lds + fs_wave_meas_bytes<Sys>()));
 #else
+    fs_stream rng, grp;
     rng.load(a.seeds, w * 64 + lane);
     grp.load(a.seeds, a.nWalkers * 64 + w);
 #endif
@@ -532,7 +539,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
     }
 #if FS_RANLUX
     rng.store(a.rl, w * 64 + lane);
-    if (lane == 0) grp.store(a.grl, w);
+    grp.store(a.grl, w);
 #else
     rng.store(a.seeds, w * 64 + lane);
     if (lane == 0) grp.store(a.seeds, a.nWalkers * 64 + w);

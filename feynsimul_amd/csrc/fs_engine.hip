@@ -259,8 +259,9 @@ size_t wave_lds_bytes(const fs_config *c)
     // the same figure as fs_wave_lds_bytes<Sys>() in device/fs_wave.h
     const size_t fsize = c->double_precision ? 8 : 4;
     const int lags = c->cor_lags > 0 ? c->cor_lags : c->N / 2;
-    return ((c->n_corr > 0 ? (size_t)fs_cor_stage_len(c->N, lags) * fsize : 0) +
-            (c->n_hist_obs > 0 ? (size_t)c->hist_bins * 4 : 0) + 15) & ~(size_t)15;
+    return (((c->n_corr > 0 ? (size_t)fs_cor_stage_len(c->N, lags) * fsize : 0) +
+             (c->n_hist_obs > 0 ? (size_t)c->hist_bins * 4 : 0) + 15) & ~(size_t)15) +
+           (c->ranlux ? sizeof(fs_ranlux) : 0);
 }
 
 std::vector<std::string> compile_options(const fs_config *c, const std::string &arch)
@@ -271,6 +272,9 @@ std::vector<std::string> compile_options(const fs_config *c, const std::string &
         opts.push_back("-Xclang");
         opts.push_back("-cl-single-precision-constant");
     }
+    // FS_MIN_WAVES=n: register budget of the wave sampler for n wavefronts per SIMD
+    if (const char *mw = getenv("FS_MIN_WAVES"); mw && atoi(mw) > 0)
+        opts.push_back("-DFS_MIN_WAVES=" + std::to_string(atoi(mw)));
     // FS_RESOURCE_USAGE=1: registers, spills, LDS and occupancy of every kernel in the build log
     if (const char *ru = getenv("FS_RESOURCE_USAGE"); ru && ru[0] == '1')
         opts.push_back("-Rpass-analysis=kernel-resource-usage");
