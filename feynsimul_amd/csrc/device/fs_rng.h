@@ -297,6 +297,59 @@ template <typename F, int NSKIP> struct fs_rl_wrap {
         return v;
     }
 
+    // K draws in a row, the same values next_float gives K times.  The
+    // buffer's phase p = cnt mod 4 is the same in every lane, m = (4 - p) mod
+    // 4 values are left in it, and the rest come from ceil((K - m) / 4)
+    // refills: RMIN of them always, one more for the phases that need it.  So
+    // a block has at most two refill sites instead of one per draw.
+    template <int K> FS_DEV void draw(F u[K])
+    {
+        constexpr int RMIN = K > 3 ? (K - 3 + 3) / 4 : 0;
+        constexpr int RMAX = (K + 3) / 4;
+        const fs_u32 p = cnt & 3u;
+        const int m = (int)((4u - p) & 3u);
+        const vec4 old = buf;
+        vec4 fresh[RMAX];
+#pragma unroll
+        for (int r = 0; r < RMIN; r++) fresh[r] = refill();
+        if constexpr (RMAX > RMIN) {
+            if (K - m > 4 * RMIN) fresh[RMAX - 1] = refill();
+            else fresh[RMAX - 1] = old;
+        }
+        // value j of the concatenation (old[p..3], fresh...)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            F v = fresh[k / 4][k % 4];  // p = 0
+            if (k < 3) {
+                const F o1 = old[(k + 1) & 3], o2 = old[(k + 2) & 3], o3 = old[(k + 3) & 3];
+                v = p == 1u ? (k < 3 ? o1 : v) : v;
+                v = p == 2u ? (k < 2 ? o2 : v) : v;
+                v = p == 3u ? (k < 1 ? o3 : v) : v;
+            }
+            if (k >= 1) v = p == 3u ? fresh[(k - 1) / 4][(k - 1) % 4] : v;
+            if (k >= 2) v = p == 2u ? fresh[(k - 2) / 4][(k - 2) % 4] : v;
+            if (k >= 3) v = p == 1u ? fresh[(k - 3) / 4][(k - 3) % 4] : v;
+            u[k] = v;
+        }
+        // the group the next draw continues from
+        const int R = (K - m + 3) / 4;
+        if (R > 0) buf = R == RMAX ? fresh[RMAX - 1] : fresh[RMIN > 0 ? RMIN - 1 : 0];
+        cnt += (fs_u32)K;
+    }
+
+    FS_DEV vec4 refill()
+    {
+        if constexpr (sizeof(F) == 8) {
+            double v[4];
+            fs_rl_next4_double<NSKIP>(st, v);
+            return vec4{v[0], v[1], v[2], v[3]};
+        } else {
+            float v[4];
+            fs_rl_next4<NSKIP>(st, v);
+            return vec4{v[0], v[1], v[2], v[3]};
+        }
+    }
+
     // ranluxIntWrapper: (uint)(u * float(ranluxIntMax + 1)).
     FS_DEV fs_u32 next_int(F scale) { return (fs_u32)(next_float() * scale); }
 };

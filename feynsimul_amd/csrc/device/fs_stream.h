@@ -17,6 +17,8 @@ struct fs_stream {
     }
     FS_DEV void store(fs_ranlux *tab, long idx) const { fs_rl_store(tab, idx, g.st); }
     FS_DEV fs_f uniform() { return g.next_float(); }
+    // K uniforms in a row, with at most two refill sites
+    template <int K> FS_DEV void uniforms(fs_f u[K]) { g.template draw<K>(u); }
     FS_DEV fs_u32 integer() { return g.next_int((fs_f)FS_ISCALE); }
     // Uniform index in [0, n), n a power of two <= 2^24: the top log2(n) bits
     // of one buffered uniform.  ranluxIntWrapper's (uint)(u * 2^31) & (n - 1)
@@ -35,6 +37,11 @@ struct fs_stream {
         reinterpret_cast<uint4 *>(rows)[idx] = make_uint4(g.x, g.y, g.z, g.w);
     }
     FS_DEV fs_f uniform() { return fs_xs_float<fs_f>(g); }
+    template <int K> FS_DEV void uniforms(fs_f u[K])
+    {
+#pragma unroll
+        for (int k = 0; k < K; k++) u[k] = uniform();
+    }
     FS_DEV fs_u32 integer() { return fs_xs_int(g); }
     // kernel.c:715: the low bits of one xorshift word
     FS_DEV fs_u32 below(fs_u32 n) { return fs_xs_int(g) & (n - 1u); }

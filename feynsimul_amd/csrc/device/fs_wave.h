@@ -260,8 +260,7 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                 }
                 // every lane draws the same count so buffered streams stay in step
                 F u[KDRAW];
-#pragma unroll
-                for (int k = 0; k < KDRAW; k++) u[k] = rng.uniform();
+                rng.template uniforms<KDRAW>(u);
 #pragma unroll
                 for (int k = 0; k < KPAIR; k++) {
                     // v == 0 gives a zero displacement, as kernel.c:346-349 does; -2 ln v is
