@@ -136,6 +136,25 @@ def test_module_compiles_for_gfx950(name, sampler):
     e.close()
 
 
+@pytest.mark.parametrize("min_waves", [None, "5"])
+@pytest.mark.parametrize("double", [0, 1])
+@pytest.mark.parametrize("lux", [0, 2])
+def test_ranlux_wave_module_compiles_for_gfx950(lux, double, min_waves, monkeypatch):
+    """The wave sampler with RANLUX streams (per-lane streams drawn a block at a time, the group stream's state in
+    LDS) builds for both precisions and luxury levels, also with the FS_MIN_WAVES register budget."""
+    from feynsimul_amd import _native
+    if min_waves:
+        monkeypatch.setenv("FS_MIN_WAVES", min_waves)
+    else:
+        monkeypatch.delenv("FS_MIN_WAVES", raising=False)
+    system = Lm2m2Cluster(3)
+    e = _native.Engine(_cfg(system, 256, 4, 1, path_storage=1, ranlux=1, ranlux_lux=lux,
+                            double_precision=double),
+                       system_section(system, (system.energyOp,)), compile_only=True)
+    assert "#define FS_RANLUX 1" in e.source
+    e.close()
+
+
 def test_compile_error_reports_build_log():
     from feynsimul_amd import _native
 

@@ -205,6 +205,9 @@ CASES_WAVE = [
     ("lm2m2", 16, 512, 4, 10.0, False, False, "memory"),
     ("ho", 32, 32, 5, 4.0, False, False, None),
     ("lm2m2", 16, 256, 5, 10.0, True, False, "memory"),
+    # RANLUX in FP64 (two float4 draws per four doubles): two and ten draws per block
+    ("aho", 32, 256, 5, 10.0, True, True, None),
+    ("lm2m2", 16, 256, 5, 10.0, True, True, "memory"),
 ]
 
 
