@@ -226,6 +226,17 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
 #pragma unroll
                 for (int d = 0; d < DOF; d++) xnext[d] = fs_at(row[d], snext);
             }
+            // one segment per block: block g's left end (its position 0, on lane r) is block
+            // g - 1's right end, which no move of this step changes, so it is carried from
+            // block to block in scalar registers instead of read again
+            F xEnd[DOF];
+            if constexpr (SEG == 64) {
+#pragma unroll
+                for (int d = 0; d < DOF; d++) {
+                    if constexpr (Sys::RESIDENT) xEnd[d] = fs_readlane(xs[d][0], r);
+                    else xEnd[d] = fs_readlane(xnext[d], r);
+                }
+            }
             // two blocks per iteration in memory storage: the loaded block and the
             // stream state then change registers in place, with no copies at the loop edge
             constexpr int BLOCK_UNROLL = NB >= 2 && !Sys::RESIDENT ? 2 : 1;
@@ -277,11 +288,14 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
                         // one segment per block: both ends (lane r of this block and of the
                         // next) and the level-1 node are the same for every lane, so they
                         // come from scalar lane reads instead of LDS permutes
-                        const F xL = fs_readlane(xo[d], r);
+                        const F xL = xEnd[d];
                         const F xR = fs_readlane(xb[d], r);
+                        xEnd[d] = xR;
                         // (1 - frac) xL + frac xR as two FMAs that read the scalar ends directly
-                        x = xL - frac * xL;
-                        x += frac * xR;
+                        // (written as fma calls: frac xR of one block is frac xL of the next,
+                        // and a shared product would cost a multiply and a subtraction)
+                        x = fma(-frac, xL, xL);
+                        x = fma(frac, xR, x);
                     } else {
                         const F xL = fs_lane_read(xo[d], laneL);
                         const F xR = fs_lane_read(p == 0 ? xb[d] : xo[d], laneR);
