@@ -52,13 +52,14 @@ def measured_traffic(profile=TRAFFIC_PROFILE):
 
 
 def measured_valu_issue(profile=TRAFFIC_PROFILE):
-    """Fraction of VALU issue capacity the kernel used, from the SQ counters in its profile summary."""
+    """Fraction of VALU issue capacity the kernel used, from the SQ counters in its profile summary (the last
+    measurement the summary records)."""
     import re
     try:
         with open(profile) as f:
-            m = re.search(r"VALU issue (?:is )?[0-9.e]+ (?:wave-instructions )?x 4 (?:cycles )?/ \(1024[^)]*\) = ([0-9.]+)",
-                          f.read().replace("×", "x"))
-        return float(m.group(1)) if m else None
+            m = re.findall(r"VALU issue (?:is )?[0-9.e]+ (?:wave-instructions )?x 4 (?:cycles )?/ \(1024[^)]*\) = ([0-9.]+)",
+                           f.read().replace("×", "x"))
+        return float(m[-1]) if m else None
     except OSError:
         return None
 VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s, one FP32 op per lane per clock
