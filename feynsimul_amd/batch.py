@@ -7,7 +7,10 @@ configuration into one module: a persistent kernel whose wavefronts take
 walkers from a device work queue, in a dispatch order that starts with the
 most expensive walkers (longest first, so the tail of the launch is short),
 and run each with its own type's specialisation of the wave sampler.
-Launch loops can be captured once into a HIP graph and replayed.
+Launch loops can be captured once into a HIP graph and replayed.  A graph
+keeps its own copy of the dispatch tables (buffer addresses and order) from
+the capture; ``replay`` refuses a graph whose buffers or order have changed
+since (``setSeriesCapacity``, ``setOrder``, a new RANLUX table ...).
 
 Members are ordinary ``PIMCKernel`` objects for everything but running:
 ``batch.kernels[t].getOperators()``, ``setPaths``, ``setSeriesCapacity`` ...
@@ -71,6 +74,14 @@ class BatchedPIMC:
     def _buffers(self):
         return [k._buffers() for k in self.kernels]
 
+    def _dispatchKey(self):
+        """What a captured graph depends on: every member's buffer addresses, series capacity and the order."""
+        key = []
+        for k, b in zip(self.kernels, self._buffers()):
+            cap = k._series.shape[0] if k._series is not None else -1
+            key.append(tuple(getattr(b, f) for f, _ in b._fields_) + (cap,))
+        return (tuple(key), tuple(self.order))
+
     def _check_series(self, launches):
         opRuns = int(self.kernels[0]._operatorRuns)
         for t, k in enumerate(self.kernels):
@@ -118,10 +129,14 @@ class BatchedPIMC:
         g.launches = int(launches)
         g.stream = stream
         g.seriesStart = self._measurements
+        g.dispatchKey = self._dispatchKey()
         return g
 
     def replay(self, graph, wait=True):
         """Runs a captured graph on its capture stream (after the current stream's work)."""
+        if graph.dispatchKey != self._dispatchKey():
+            raise Exception("the batch's buffers or dispatch order changed since this graph was captured; "
+                            "capture it again")
         graph.stream.wait_stream(self._stream)
         self._ev0.record(graph.stream)
         graph.launch(graph.stream.cuda_stream)

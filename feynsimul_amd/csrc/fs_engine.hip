@@ -572,6 +572,23 @@ int fs_eval_system(fs_engine *e, const void *x, void *pot, void *ops, int64_t n,
 // ---- batched systems ----
 }  // extern "C"
 
+// Device copy of one dispatch, in one allocation: the work counter, the
+// per-type kernel arguments, the dispatch prefix sums and the type of each
+// dispatch slot.  A batch keeps one for plain launches; every captured graph
+// owns its own, so later changes of buffers or order never reach a graph.
+struct fs_dispatch_layout {
+    size_t types, start, type_of, bytes;
+    static constexpr size_t up(size_t v) { return (v + 255) & ~(size_t)255; }
+    explicit constexpr fs_dispatch_layout(int nt)
+        : types(256),  // the counter sits in the first 256 bytes
+          start(up(256 + sizeof(fs_kargs) * (size_t)nt)),
+          type_of(up(up(256 + sizeof(fs_kargs) * (size_t)nt) + sizeof(long long) * (size_t)(nt + 1))),
+          bytes(up(up(up(256 + sizeof(fs_kargs) * (size_t)nt) + sizeof(long long) * (size_t)(nt + 1)) +
+                   sizeof(int) * (size_t)nt))
+    {
+    }
+};
+
 struct fs_batch {
     std::vector<fs_config> cfgs;
     std::string source;
@@ -583,19 +600,16 @@ struct fs_batch {
     int wpb = 4;
     size_t smem = 0;
     int max_blocks = 0;
-    // device tables and the host copies last uploaded
-    fs_kargs *d_types = nullptr;
-    long long *d_start = nullptr;
-    int *d_type_of = nullptr;
-    unsigned long long *d_counter = nullptr;
-    std::vector<fs_kargs> h_types;
-    std::vector<long long> h_start;
-    std::vector<int> h_type_of;
+    int device = -1;
+    // dispatch tables of plain launches and the host image last uploaded
+    char *d_tables = nullptr;
+    std::vector<char> h_tables;
 };
 
 struct fs_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    char *d_tables = nullptr;  // this graph's own dispatch tables
 };
 
 namespace {
@@ -645,11 +659,14 @@ extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_batch_sweep(fs_batc
     return s;
 }
 
-int batch_tables(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
-                 int32_t metro_steps, int32_t series_offset)
+// Host image of the dispatch tables (fs_dispatch_layout, counter bytes zero).
+int batch_tables(const fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
+                 int32_t metro_steps, int32_t series_offset, std::vector<char> &image)
 {
     const int nt = (int)b->cfgs.size();
-    std::vector<fs_kargs> types(nt);
+    const fs_dispatch_layout L(nt);
+    image.assign(L.bytes, 0);
+    fs_kargs *types = reinterpret_cast<fs_kargs *>(image.data() + L.types);
     for (int t = 0; t < nt; t++) {
         const fs_config &c = b->cfgs[t];
         const fs_buffers &x = bufs[t];
@@ -677,9 +694,10 @@ int batch_tables(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int3
         a.operatorRuns = operator_runs;
         a.metroSteps = metro_steps;
         a.seriesOffset = series_offset;
-        types[t] = a;
+        memcpy(&types[t], &a, sizeof(a));
     }
-    std::vector<int> type_of(nt);
+    int *type_of = reinterpret_cast<int *>(image.data() + L.type_of);
+    long long *start = reinterpret_cast<long long *>(image.data() + L.start);
     std::vector<char> seen(nt, 0);
     for (int d = 0; d < nt; d++) {
         const int t = order ? order[d] : d;
@@ -687,33 +705,35 @@ int batch_tables(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int3
         seen[t] = 1;
         type_of[d] = t;
     }
-    std::vector<long long> start(nt + 1, 0);
+    start[0] = 0;
     for (int d = 0; d < nt; d++) start[d + 1] = start[d] + b->cfgs[type_of[d]].n_walkers;
-    // upload only what changed; the tables are tiny, and an unchanged set needs no copy
-    if (b->h_types.size() != types.size() || memcmp(types.data(), b->h_types.data(), sizeof(fs_kargs) * nt) != 0) {
-        FS_HIP(hipMemcpy(b->d_types, types.data(), sizeof(fs_kargs) * nt, hipMemcpyHostToDevice));
-        b->h_types = types;
-    }
-    if (start != b->h_start || type_of != b->h_type_of) {
-        FS_HIP(hipMemcpy(b->d_start, start.data(), sizeof(long long) * (nt + 1), hipMemcpyHostToDevice));
-        FS_HIP(hipMemcpy(b->d_type_of, type_of.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
-        b->h_start = start;
-        b->h_type_of = type_of;
-    }
     return FS_OK;
 }
 
-int batch_launch(fs_batch *b, int32_t operator_runs, int32_t metro_steps, int32_t series_offset,
+// Copies the tables (not the counter) of `image` to device memory `dst`,
+// ordered on `st`.  The source is pageable host memory, which the runtime
+// stages before the call returns.
+int upload_tables(char *dst, const std::vector<char> &image, hipStream_t st)
+{
+    const size_t off = 256;
+    FS_HIP(hipMemcpyAsync(dst + off, image.data() + off, image.size() - off, hipMemcpyHostToDevice, st));
+    return FS_OK;
+}
+
+int batch_launch(fs_batch *b, char *tables, int32_t operator_runs, int32_t metro_steps, int32_t series_offset,
                  int32_t grid_blocks, hipStream_t st)
 {
-    void *rargs[] = {&b->d_counter};
+    const int nt = (int)b->cfgs.size();
+    const fs_dispatch_layout L(nt);
+    unsigned long long *counter = reinterpret_cast<unsigned long long *>(tables);
+    void *rargs[] = {&counter};
     FS_HIP(hipModuleLaunchKernel(b->fn_reset, 1, 1, 1, 1, 1, 1, 0, st, rargs, nullptr));
     fs_batch_args ba{};
-    ba.types = b->d_types;
-    ba.start = b->d_start;
-    ba.typeOf = b->d_type_of;
-    ba.counter = b->d_counter;
-    ba.ntypes = (int)b->cfgs.size();
+    ba.types = reinterpret_cast<const fs_kargs *>(tables + L.types);
+    ba.start = reinterpret_cast<const long long *>(tables + L.start);
+    ba.typeOf = reinterpret_cast<const int *>(tables + L.type_of);
+    ba.counter = counter;
+    ba.ntypes = nt;
     ba.operatorRuns = operator_runs;
     ba.metroSteps = metro_steps;
     ba.seriesOffset = series_offset;
@@ -726,10 +746,7 @@ int batch_launch(fs_batch *b, int32_t operator_runs, int32_t metro_steps, int32_
 void batch_free(fs_batch *b)
 {
     if (!b) return;
-    if (b->d_types) (void)hipFree(b->d_types);
-    if (b->d_start) (void)hipFree(b->d_start);
-    if (b->d_type_of) (void)hipFree(b->d_type_of);
-    if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->d_tables) (void)hipFree(b->d_tables);
     if (b->module) (void)hipModuleUnload(b->module);
     delete b;
 }
@@ -803,10 +820,7 @@ int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32
         hipError_t e = hipModuleLoadData(&b->module, b->code.data());
         if (e == hipSuccess) e = hipModuleGetFunction(&b->fn_sweep, b->module, "fs_batch_sweep");
         if (e == hipSuccess) e = hipModuleGetFunction(&b->fn_reset, b->module, "fs_batch_reset");
-        if (e == hipSuccess) e = hipMalloc(&b->d_types, sizeof(fs_kargs) * ntypes);
-        if (e == hipSuccess) e = hipMalloc(&b->d_start, sizeof(long long) * (ntypes + 1));
-        if (e == hipSuccess) e = hipMalloc(&b->d_type_of, sizeof(int) * ntypes);
-        if (e == hipSuccess) e = hipMalloc(&b->d_counter, sizeof(unsigned long long));
+        if (e == hipSuccess) e = hipMalloc(&b->d_tables, fs_dispatch_layout(ntypes).bytes);
         int per_cu = 0, dev = 0, cus = 0;
         if (e == hipSuccess)
             e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn_sweep, 64 * wpb, b->smem);
@@ -817,6 +831,7 @@ int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32
             return fail(FS_ERR_HIP, "batch module setup failed: %s", hipGetErrorString(e));
         }
         b->max_blocks = std::max(1, per_cu) * std::max(1, cus);
+        b->device = dev;
     }
     *out = b;
     return FS_OK;
@@ -847,9 +862,15 @@ int fs_batch_run(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int3
     if (!b || !bufs) return fail(FS_ERR_INVALID, "batch and buffers are required");
     if (!b->module) return fail(FS_ERR_INVALID, "batch was created compile-only");
     if (operator_runs < 0 || metro_steps < 0) return fail(FS_ERR_INVALID, "negative run length");
-    if (int rc = batch_tables(b, bufs, order, operator_runs, metro_steps, series_offset)) return rc;
-    return batch_launch(b, operator_runs, metro_steps, series_offset, grid_blocks,
-                        reinterpret_cast<hipStream_t>(stream));
+    std::vector<char> image;
+    if (int rc = batch_tables(b, bufs, order, operator_runs, metro_steps, series_offset, image)) return rc;
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // the tables are tiny; an unchanged set needs no copy
+    if (image != b->h_tables) {
+        if (int rc = upload_tables(b->d_tables, image, st)) return rc;
+        b->h_tables.swap(image);
+    }
+    return batch_launch(b, b->d_tables, operator_runs, metro_steps, series_offset, grid_blocks, st);
 }
 
 int fs_batch_capture(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
@@ -861,27 +882,33 @@ int fs_batch_capture(fs_batch *b, const fs_buffers *bufs, const int32_t *order, 
     if (!b->module) return fail(FS_ERR_INVALID, "batch was created compile-only");
     if (!stream) return fail(FS_ERR_INVALID, "capture needs a non-default stream");
     if (n_launches < 1 || operator_runs < 0 || metro_steps < 0) return fail(FS_ERR_INVALID, "bad run length");
-    if (int rc = batch_tables(b, bufs, order, operator_runs, metro_steps, series_offset)) return rc;
+    std::vector<char> image;
+    if (int rc = batch_tables(b, bufs, order, operator_runs, metro_steps, series_offset, image)) return rc;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    FS_HIP(hipStreamSynchronize(st));
-    FS_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    fs_graph *g = new fs_graph();
+    auto drop = [&](int rc) {
+        (void)fs_graph_destroy(g);
+        return rc;
+    };
+    if (hipMalloc(&g->d_tables, image.size()) != hipSuccess) {
+        g->d_tables = nullptr;
+        return drop(fail(FS_ERR_HIP, "graph dispatch tables: out of device memory"));
+    }
+    if (int rc = upload_tables(g->d_tables, image, st)) return drop(rc);
+    if (hipStreamSynchronize(st) != hipSuccess) return drop(fail(FS_ERR_HIP, "stream synchronisation failed"));
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return drop(fail(FS_ERR_HIP, "hipStreamBeginCapture failed"));
     int rc = FS_OK;
     for (int k = 0; k < n_launches && rc == FS_OK; k++)
-        rc = batch_launch(b, operator_runs, metro_steps, series_offset + k * operator_runs, grid_blocks, st);
-    hipGraph_t graph = nullptr;
-    const hipError_t ec = hipStreamEndCapture(st, &graph);
-    if (rc != FS_OK) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return rc;
-    }
-    if (ec != hipSuccess) return fail(FS_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec));
-    fs_graph *g = new fs_graph();
-    g->graph = graph;
-    const hipError_t ei = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        rc = batch_launch(b, g->d_tables, operator_runs, metro_steps, series_offset + k * operator_runs,
+                          grid_blocks, st);
+    const hipError_t ec = hipStreamEndCapture(st, &g->graph);
+    if (rc != FS_OK) return drop(rc);
+    if (ec != hipSuccess) return drop(fail(FS_ERR_HIP, "hipStreamEndCapture failed: %s", hipGetErrorString(ec)));
+    const hipError_t ei = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
     if (ei != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        delete g;
-        return fail(FS_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+        g->exec = nullptr;
+        return drop(fail(FS_ERR_HIP, "hipGraphInstantiate failed: %s", hipGetErrorString(ei)));
     }
     *out = g;
     return FS_OK;
@@ -899,6 +926,7 @@ int fs_graph_destroy(fs_graph *g)
     if (!g) return FS_OK;
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->d_tables) (void)hipFree(g->d_tables);
     delete g;
     return FS_OK;
 }

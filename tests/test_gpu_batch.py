@@ -40,8 +40,20 @@ MEMBERS = [(HarmOsc(), 40, 256, 4, 10.0), (Lm2m2Cluster(3), 24, 256, 6, 10.0),
            (AnharmOsc(0.1), 16, 2048, 5, 20.0), (Lm2m2Cluster(4), 8, 512, 5, 10.0)]
 
 
+def _same_state(k1, k2, what):
+    """Bit equality of two kernels' streams, acceptance counts, paths and operator means."""
+    for name in ("_seeds", "_rl", "_grl", "_accepts"):
+        a, b = getattr(k1, name), getattr(k2, name)
+        if a is not None:
+            np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy(), err_msg="%s: %s" % (what, name))
+    np.testing.assert_array_equal(k1.getPaths(), k2.getPaths(), err_msg=what + ": paths")
+    np.testing.assert_array_equal(k1.getOperators(), k2.getOperators(), err_msg=what + ": operators")
+
+
 @pytest.mark.parametrize("order", ["cost", "given"])
 def test_batch_matches_separate_kernels(gpu, order):
+    """A member of a batch runs its type's specialisation of the same device code as a kernel of its
+    own, on its own buffers: the results are bit-identical."""
     _case("batch vs separate, order " + order)
     rng = np.random.default_rng(0)
     starts = [_paths(s, W, N, rng) for s, W, N, S, b in MEMBERS]
@@ -54,18 +66,12 @@ def test_batch_matches_separate_kernels(gpu, order):
         single = PIMCKernel(_ka(s, W, N, S, b, 2, 3, seed=100 + i))
         single.setPaths(starts[i])
         single.run()
-        kb = batch.kernels[i]
-        np.testing.assert_array_equal(kb._seeds.cpu().numpy(), single._seeds.cpu().numpy())
-        acc_b = kb._accepts.cpu().numpy()
-        acc_s = single._accepts.cpu().numpy()
-        same = acc_b == acc_s
-        assert same.mean() >= 0.9, "member %d: acceptance differs on %d walkers" % (i, (~same).sum())
-        err = np.abs(kb.getPaths()[same] - single.getPaths()[same]).max()
-        assert err < 1e-3, "member %d path error %g" % (i, err)
-        np.testing.assert_allclose(kb.getOperators()[0][same], single.getOperators()[0][same], rtol=1e-4, atol=1e-6)
+        _same_state(batch.kernels[i], single, "member %d" % i)
 
 
 def test_graph_replay_matches_plain_runs(gpu):
+    """Replays of a captured graph equal the same number of plain launches bit for bit, with the
+    program's own eager work between replays."""
     _case("batch graph replay")
     rng = np.random.default_rng(1)
     starts = [_paths(s, W, N, rng) for s, W, N, S, b in MEMBERS[:2]]
@@ -75,21 +81,126 @@ def test_graph_replay_matches_plain_runs(gpu):
     for k1, k2, p in zip(plain.kernels, graphed.kernels, starts):
         k1.setPaths(p)
         k2.setPaths(p)
+        k1.setSeriesCapacity(3)
         k2.setSeriesCapacity(3)
     g = graphed.captureGraph(3)
     for rep in range(2):
+        plain.resetSeries()
         for _ in range(3):
             plain.run()
         graphed.replay(g)
-        # the program's own eager work between replays
-        for k in graphed.kernels:
-            assert k.getSeries().shape[0] == 3
-            assert np.isfinite(k.getSeries()).all()
-        for k1, k2 in zip(plain.kernels, graphed.kernels):
-            np.testing.assert_array_equal(k1._seeds.cpu().numpy(), k2._seeds.cpu().numpy())
-            same = k1._accepts.cpu().numpy() == k2._accepts.cpu().numpy()
-            assert same.mean() >= 0.9
-            assert np.abs(k1.getPaths()[same] - k2.getPaths()[same]).max() < 1e-3
+        for i, (k1, k2) in enumerate(zip(plain.kernels, graphed.kernels)):
+            assert k2.getSeries().shape[0] == 3
+            np.testing.assert_array_equal(k1.getSeries(), k2.getSeries())
+            _same_state(k1, k2, "replay %d member %d" % (rep, i))
+
+
+def test_stale_graph_is_refused_and_recapture_works(gpu):
+    """A graph keeps the dispatch tables of its capture: after setSeriesCapacity (a new series buffer)
+    or setOrder, replay refuses it; a new capture replays correctly, and a plain run with another
+    order in between does not change what the graph dispatches."""
+    _case("stale graph")
+    rng = np.random.default_rng(3)
+    starts = [_paths(s, W, N, rng) for s, W, N, S, b in MEMBERS[:3]]
+    kas = [_ka(s, W, N, S, b, 1, 2, seed=30 + i) for i, (s, W, N, S, b) in enumerate(MEMBERS[:3])]
+    plain = BatchedPIMC(kas)
+    graphed = BatchedPIMC(kas)
+    for k1, k2, p in zip(plain.kernels, graphed.kernels, starts):
+        k1.setPaths(p)
+        k2.setPaths(p)
+        k2.setSeriesCapacity(2)
+    g = graphed.captureGraph(1)
+    graphed.kernels[1].setSeriesCapacity(4)
+    with pytest.raises(Exception, match="capture it again"):
+        graphed.replay(g)
+    g = graphed.captureGraph(1)
+    order0 = list(graphed.order)
+    graphed.setOrder(order0[::-1])
+    with pytest.raises(Exception, match="capture it again"):
+        graphed.replay(g)
+    # a plain run with the reversed order uploads other tables; the graph keeps its own
+    graphed.run()
+    plain.run()
+    graphed.setOrder(order0)
+    graphed.replay(g)
+    plain.run()
+    for i, (k1, k2) in enumerate(zip(plain.kernels, graphed.kernels)):
+        _same_state(k1, k2, "member %d" % i)
+
+
+def _oracle_member(k, p0, seed):
+    """One launch of a member on the CPU oracle (wave ordering) from the member's initial paths and
+    the same seeds; returns the oracle's results and final paths."""
+    from oracle import oracle as O
+    from feynsimul_amd.kernel import _GROUP_OFFSET
+    s = k._system
+    W, N = k._nbrOfWalkers, k._N
+    lib = O.build(s, N, k._beta, operators=k._operators)
+    cfg = O.make_cfg(N, k._S, s.DOF, W, k._beta, int(k._operatorRuns), int(k._metroStepsPerOperatorRun),
+                     ranlux=bool(k._enableRanlux), nskip=O.nskip_of_lux(int(k._luxuaryFactor or 0)),
+                     nOps=len(k._operators))
+    paths = np.ascontiguousarray(p0.astype(np.float32))
+    if k._enableRanlux:
+        lst = O.ranlux_init(lib, O.ranlux_seeds(seed, 0, W * 64))
+        gst = O.ranlux_init(lib, O.ranlux_seeds(seed, _GROUP_OFFSET, W))
+        ref = O.run_wave(lib, cfg, paths, lane_rl=lst, group_rl=gst)
+        np.testing.assert_array_equal(k._rl.cpu().numpy().view(np.uint8).reshape(W * 64, 112),
+                                      lst.view(np.uint8).reshape(W * 64, 112))
+        np.testing.assert_array_equal(k._grl.cpu().numpy().view(np.uint8).reshape(W, 112),
+                                      gst.view(np.uint8).reshape(W, 112))
+    else:
+        lane = O.xorshift_rows(seed, 0, W * 64)
+        grp = O.xorshift_rows(seed, W * 64, W)
+        ref = O.run_wave(lib, cfg, paths, lane_seeds=lane, group_seeds=grp)
+        dev = k._seeds.cpu().numpy().view(np.uint32)
+        np.testing.assert_array_equal(dev[: W * 64], lane)
+        np.testing.assert_array_equal(dev[W * 64:], grp)
+    return ref, paths
+
+
+# LM2M2 clusters of 3, 4, 6 and 8 atoms (DOF 9 to 24) beside 1D oscillators, ragged N and S
+ORACLE_MEMBERS = [(HarmOsc(), 32, 256, 4, 10.0), (Lm2m2Cluster(3), 16, 256, 6, 10.0),
+                  (AnharmOsc(0.1), 16, 1024, 5, 20.0), (Lm2m2Cluster(4), 16, 512, 5, 10.0),
+                  (Lm2m2Cluster(6), 16, 128, 4, 10.0), (Lm2m2Cluster(8), 16, 256, 6, 10.0)]
+
+
+@pytest.mark.parametrize("ranlux", [False, True], ids=["xorshift", "ranlux-lux2"])
+def test_batch_members_match_oracle(gpu, ranlux):
+    """Every member of one batched launch against the CPU oracle's restatement of the wave sampler
+    on the same initial paths and seeds: stream states bit-identical, acceptance counts on at least
+    90 % of walkers (a Metropolis decision whose margin is below the rounding difference of gcc and
+    hiprtc may flip), paths to 2e-4 (1D) / 2e-3 (LM2M2) and operator means to 1e-3 relative on the
+    walkers whose decisions agree."""
+    _case("batch members vs oracle, ranlux=%d" % ranlux)
+    rng = np.random.default_rng(21)
+    members = ORACLE_MEMBERS if not ranlux else ORACLE_MEMBERS[1:4]
+    starts = [_paths(s, W, N, rng) for s, W, N, S, b in members]
+    kas = []
+    for i, (s, W, N, S, b) in enumerate(members):
+        ka = _ka(s, W, N, S, b, 2, 2, seed=500 + i)
+        ka.fastMath = False
+        if ranlux:
+            ka.enableRanlux, ka.luxuaryFactor, ka.ranluxIntMax = True, 1, 2 ** 31 - 1
+        kas.append(ka)
+    batch = BatchedPIMC(kas)
+    for k, p in zip(batch.kernels, starts):
+        if ranlux:
+            k.initRanlux()
+            k.groupInitRanlux()
+        k.setPaths(p)
+    batch.run()
+    for i, (k, p0) in enumerate(zip(batch.kernels, starts)):
+        ref, paths = _oracle_member(k, p0, 500 + i)
+        W = k._nbrOfWalkers
+        acc = k._accepts.cpu().numpy().view(np.uint32)
+        same = acc == ref["accepts"]
+        assert same.mean() >= 0.9, "member %d: acceptance differs on %d of %d walkers" % (i, (~same).sum(), W)
+        tol = 2e-3 if isinstance(k._system, Lm2m2Cluster) else 2e-4
+        err = np.abs(k.getPaths()[same] - paths[same]).max()
+        assert err < tol, "member %d: path error %g" % (i, err)
+        ops = k.getOperators()[0]
+        rel = np.abs(ops[same] - ref["opMeans"][same]) / (np.abs(ref["opMeans"][same]) + 1e-3)
+        assert rel.max() < 1e-3, "member %d: operator mean error %g" % (i, rel.max())
 
 
 def test_ragged_lm2m2_batch_full_size(gpu):

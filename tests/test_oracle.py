@@ -19,7 +19,12 @@ What fixes each oracle function, so that a plausible mistake fails a check:
   and transfer-matrix quadrature, for both orderings.
 * HO and AHO estimators: closed forms and FP64 transfer matrices.
 * LM2M2 potential: literature values of the formula and finite-difference
-  derivatives of the virial operator.
+  derivatives of the virial operator; the generated strings of 3-, 4-, 6-
+  and 8-atom clusters against the FP64 pair sum.
+* histograms (kernel.c:478-506): the observable histogram (pair distances)
+  and the DOF-grid bins after one measurement equal an independent numpy
+  recount of the oracle's own final paths, bin for bin up to values within
+  a few FP32 ulps of a bin edge; a one-bin shift fails the check.
 Parity unpinned: none of the functions above.
 """
 import math
@@ -364,14 +369,17 @@ def test_lm2m2_literature_values():
     assert abs(mass_scale(4.0026) ** 2 - 1 / 12.119282) < 2e-6
 
 
-def test_lm2m2_system_strings_in_oracle():
-    """The generated potential and virial strings, compiled by gcc in FP64."""
-    sysobj = Lm2m2Cluster(3)
+@pytest.mark.parametrize("n", [3, 4, 6, 8])
+def test_lm2m2_system_strings_in_oracle(n):
+    """The generated potential and virial strings of an n-atom cluster (DOF 3n: 9, 12, 18, 24),
+    compiled by gcc in FP64, against the FP64 pair sum of the formula and its finite-difference
+    virial."""
+    sysobj = Lm2m2Cluster(n)
     lib = O.build(sysobj, 64, 10.0, operators=(sysobj.energyOp, sysobj.meanPairDistanceOp), double=True)
-    rng = np.random.default_rng(4)
-    for _ in range(50):
-        r_atoms = rng.normal(0, 2.5, (3, 3))
-        x = np.ascontiguousarray((r_atoms * sysobj.massScale).reshape(9))
+    rng = np.random.default_rng(4 + n)
+    for _ in range(50 if n <= 4 else 20):
+        r_atoms = rng.normal(0, 2.5 + 0.3 * n, (n, 3))
+        x = np.ascontiguousarray((r_atoms * sysobj.massScale).reshape(3 * n))
         dists = [np.linalg.norm(r_atoms[i] - r_atoms[j]) for i, j in sysobj.pairs]
         V = sum(lm2m2_kelvin(d) for d in dists)
         assert abs(lib.fs_oracle_potential(x.ctypes.data) - V) < 1e-9 * max(1, abs(V))
@@ -381,6 +389,88 @@ def test_lm2m2_system_strings_in_oracle():
         vir = sum(lm2m2_kelvin(d) + 0.5 * d * (lm2m2_kelvin(d + h) - lm2m2_kelvin(d - h)) / (2 * h) for d in dists)
         assert abs(acc[0] - vir) < 1e-5 * max(1, abs(vir))
         assert abs(acc[1] - np.mean(dists)) < 1e-9
+    assert len(sysobj.pairs) == n * (n - 1) // 2
+
+
+def _recount(values, lo, hi, bins, ulps=8):
+    """Independent FP64 recount of a histogram: np.histogram of the values that are not within a few
+    FP32 ulps of a bin edge, plus the near-edge values (in bin units) whose bin rounding may decide."""
+    v = (np.asarray(values, np.float64).ravel() - lo) * (bins / (hi - lo))
+    tol = ulps * np.finfo(np.float32).eps * np.maximum(np.abs(v), 1.0)
+    near = np.abs(v - np.round(v)) <= tol
+    far = np.histogram(v[~near], bins=bins, range=(0.0, float(bins)))[0]
+    return far, v[near]
+
+
+def _assert_histogram(counts, far, near_v, bins):
+    """counts must equal the far-from-edge recount, plus at most one count per near-edge value in one
+    of the two bins beside its edge."""
+    counts = np.asarray(counts, np.int64)
+    diff = counts - far
+    assert (diff >= 0).all(), "bins below the recount: %s" % np.nonzero(diff < 0)[0]
+    allow = np.zeros(bins, np.int64)
+    for e in np.round(near_v).astype(np.int64):
+        for b in (e - 1, e):
+            if 0 <= b < bins:
+                allow[b] += 1
+    assert (diff <= allow).all(), "bins above the recount: %s" % np.nonzero(diff > allow)[0]
+    assert diff.sum() <= len(near_v)
+
+
+def test_observable_histogram_matches_numpy_recount():
+    """The oracle's histogram of scalar observables (kernel.c:478-506 style accumulation applied to the
+    pair distances r_ij of a 3-atom LM2M2 cluster, 256 bins on [0, 12) angstrom) after one launch with
+    a single measurement equals an independent numpy recount of the pair distances of the oracle's own
+    final paths, bin for bin (values within a few FP32 ulps of a bin edge may fall on either side)."""
+    sysobj = Lm2m2Cluster(3)
+    W, N, S, beta, bins, lo, hi = 24, 256, 6, 10.0, 256, 0.0, 12.0
+    obs = tuple(sysobj.pairDistance(i, j) for i, j in sysobj.pairs)
+    lib = O.build(sysobj, N, beta, operators=(sysobj.energyOp,), observables=obs)
+    cfg = O.make_cfg(N, S, sysobj.DOF, W, beta, 1, 3, nOps=1, nHistObs=3, histBins=bins, histRange=(lo, hi))
+    rng = np.random.default_rng(17)
+    pos = sysobj.initialPositions(W, rng, sigma=1.5)
+    p0 = np.repeat(pos[:, :, None], N, axis=2) + rng.normal(0, 0.05, (W, sysobj.DOF, N))
+    paths = np.ascontiguousarray(p0.reshape(W, sysobj.DOF * N).astype(np.float32))
+    ref = O.run_wave(lib, cfg, paths, lane_seeds=O.xorshift_rows(3, 0, W * 64),
+                     group_seeds=O.xorshift_rows(3, W * 64, W))
+    assert ref["accepts"].sum() > 0
+    x = paths.reshape(W, sysobj.nAtoms, 3, N).astype(np.float64) / sysobj.massScale
+    r = np.stack([np.sqrt(((x[:, i] - x[:, j]) ** 2).sum(axis=1)) for i, j in sysobj.pairs])
+    far, near_v = _recount(r, lo, hi, bins)
+    _assert_histogram(ref["obsHist"], far, near_v, bins)
+    # some distances fall outside the range: they are not counted
+    inside = ((r >= lo) & (r < hi)).sum()
+    assert inside < r.size and abs(int(ref["obsHist"].sum()) - int(inside)) <= len(near_v)
+    # a planted off-by-one in the bin index fails the same check
+    shifted = np.roll(ref["obsHist"], 1)
+    with pytest.raises(AssertionError):
+        _assert_histogram(shifted, far, near_v, bins)
+
+
+@pytest.mark.parametrize("dof", [1, 2])
+def test_dof_grid_bins_match_numpy_recount(dof):
+    """The oracle's DOF-grid histogram (kernel.c:478-506: binIndex = sum_p floor((x_p - xMin) /
+    binSize) * res^p over raw coordinates, slices with any coordinate outside [xMin, xMax) dropped)
+    after one launch with a single measurement equals numpy's histogramdd of the final paths."""
+    class _Osc:
+        DOF = dof
+        potential = " + ".join("0.5f * x%d * x%d" % (i + 1, i + 1) for i in range(dof))
+        userCode = ""
+    W, N, S, beta, res, xmin, xmax = 16, 64, 3, 5.0, 12, -1.5, 1.5
+    lib = O.build(_Osc(), N, beta, operators=("x1 * x1",))
+    cfg = O.make_cfg(N, S, dof, W, beta, 1, 4, nOps=1, bins=True, binRes=res, xMin=xmin, xMax=xmax)
+    paths = np.ascontiguousarray(np.random.default_rng(2).uniform(-1.2, 1.2, (W, dof * N)).astype(np.float32))
+    ref = O.run_reference(lib, cfg, paths, seeds=O.xorshift_rows(5, 0, cfg.nThreads + 1))
+    x = paths.reshape(W, dof, N).transpose(0, 2, 1).reshape(-1, dof).astype(np.float64)
+    v = (x - xmin) * (res / (xmax - xmin))
+    near = (np.abs(v - np.round(v)) <= 8 * np.finfo(np.float32).eps * np.maximum(np.abs(v), 1)).any(axis=1)
+    h = np.histogramdd(v[~near], bins=[res] * dof, range=[(0, res)] * dof)[0]
+    # kernel.c:478-506 puts coordinate p at stride res^p: index = sum_p b_p res^p, so axis 0 varies fastest
+    flat = h.transpose(tuple(range(dof))[::-1]).reshape(-1) if dof > 1 else h
+    got = ref["bins"].astype(np.int64)
+    diff = got - flat.astype(np.int64)
+    assert (diff >= 0).all() and diff.sum() <= near.sum()
+    assert got.sum() > 0.5 * W * N
 
 
 # pair distances (angstrom) through the core: r = 0, below, at and above the 0.0005 A cut-off of the
