@@ -18,6 +18,7 @@ FS_POTCACHE_AUTO, FS_POTCACHE_ON, FS_POTCACHE_OFF = 0, 1, 2
 
 EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_path_storage",
            "fs_batch_create", "fs_batch_destroy", "fs_batch_path_storage", "fs_batch_build_log", "fs_batch_run",
+           "fs_batch_run_noalloc",
            "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
            "fs_source", "fs_run", "fs_set_waves_per_simd", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
            "fs_ranlux_draw", "fs_rng_bench", "fs_lp_create", "fs_lp_destroy", "fs_lp_get_info", "fs_lp_build_log", "fs_lp_pack",
@@ -95,6 +96,8 @@ def lib():
     L.fs_batch_path_storage.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int)]
     L.fs_batch_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_batch_run.argtypes = [vp, ctypes.POINTER(FsBuffers), ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32, vp]
+    L.fs_batch_run_noalloc.argtypes = [vp, ctypes.POINTER(FsBuffers), ctypes.POINTER(ctypes.c_int32), i32, i32, i32,
+                                       i32, vp]
     L.fs_batch_capture.argtypes = [vp, ctypes.POINTER(FsBuffers), ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32,
                                    i32, vp, ctypes.POINTER(vp)]
     L.fs_graph_launch.argtypes = [vp, vp]
@@ -228,6 +231,12 @@ class Batch:
         arr = (FsBuffers * self.ntypes)(*bufs)
         check(lib().fs_batch_run(self._h, arr, self._order(order, self.ntypes), operator_runs, metro_steps,
                                  series_offset, grid_blocks, stream))
+
+    def run_noalloc(self, bufs, order, operator_runs, metro_steps, series_offset, grid_blocks, stream):
+        """fs_batch_run_noalloc: dispatch tables in the library's static per-GPU array."""
+        arr = (FsBuffers * self.ntypes)(*bufs)
+        check(lib().fs_batch_run_noalloc(self._h, arr, self._order(order, self.ntypes), operator_runs, metro_steps,
+                                         series_offset, grid_blocks, stream))
 
     def capture(self, bufs, order, operator_runs, metro_steps, series_offset, n_launches, grid_blocks, stream):
         arr = (FsBuffers * self.ntypes)(*bufs)

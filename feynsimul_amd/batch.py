@@ -88,14 +88,19 @@ class BatchedPIMC:
             if k._series is not None and self._measurements + launches * opRuns > k._series.shape[0]:
                 raise Exception("member %d: series buffer full" % t)
 
-    def run(self, wait=True):
-        """One launch: operatorRuns x metroStepsPerOperatorRun steps for every walker of every member."""
+    def run(self, wait=True, noAlloc=False):
+        """One launch: operatorRuns x metroStepsPerOperatorRun steps for every walker of every member.
+
+        With ``noAlloc`` the launch keeps its dispatch tables in the library's
+        static per-GPU array instead of the batch's own device memory
+        (fs_batch_run_noalloc); launches of that kind must not overlap on one GPU.
+        """
         k0 = self.kernels[0]
         opRuns, M = int(k0._operatorRuns), int(k0._metroStepsPerOperatorRun)
         self._check_series(1)
         self._ev0.record(self._stream)
-        self._batch.run(self._buffers(), self.order, opRuns, M, self._measurements, self.gridBlocks,
-                        self._stream.cuda_stream)
+        launch = self._batch.run_noalloc if noAlloc else self._batch.run
+        launch(self._buffers(), self.order, opRuns, M, self._measurements, self.gridBlocks, self._stream.cuda_stream)
         self._ev1.record(self._stream)
         self._after(1)
         if wait:

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -589,6 +590,15 @@ struct fs_dispatch_layout {
     }
 };
 
+// Dispatch tables of fs_batch_run_noalloc: one fixed array per GPU, held
+// until the process ends, shared by every call on that GPU.  Host code uses
+// the device address that hipGetSymbolAddress returns for the current
+// device (scratch_address), never the array's name.
+constexpr int kBatchMaxTypes = 64;
+constexpr size_t kScratchBytes = 16384;
+static_assert(fs_dispatch_layout(kBatchMaxTypes).bytes <= kScratchBytes, "scratch too small for 64 types");
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 struct fs_batch {
     std::vector<fs_config> cfgs;
     std::string source;
@@ -871,6 +881,60 @@ int fs_batch_run(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int3
         b->h_tables.swap(image);
     }
     return batch_launch(b, b->d_tables, operator_runs, metro_steps, series_offset, grid_blocks, st);
+}
+
+}  // extern "C"
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) on the first call.
+int scratch_address(int device, char **out)
+{
+    int cur = -1;
+    FS_HIP(hipGetDevice(&cur));
+    if (cur != device)
+        return fail(FS_ERR_INVALID, "the batch was created on device %d; make it current (device %d is)", device, cur);
+    if (device < 0 || device >= kMaxDevices) return fail(FS_ERR_INVALID, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(FS_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = p;
+    }
+    *out = static_cast<char *>(g_scratch_ptr[device]);
+    return FS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_batch_run_noalloc(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
+                         int32_t metro_steps, int32_t series_offset, int32_t grid_blocks, void *stream)
+{
+    if (!b || !bufs) return fail(FS_ERR_INVALID, "batch and buffers are required");
+    if (!b->module) return fail(FS_ERR_INVALID, "batch was created compile-only");
+    if (operator_runs < 0 || metro_steps < 0) return fail(FS_ERR_INVALID, "negative run length");
+    const int nt = (int)b->cfgs.size();
+    if (nt > kBatchMaxTypes || fs_dispatch_layout(nt).bytes > kScratchBytes)
+        return fail(FS_ERR_INVALID, "%d types need %zu bytes of dispatch tables; the static scratch holds %zu "
+                    "(at most %d types)", nt, fs_dispatch_layout(nt).bytes, kScratchBytes, kBatchMaxTypes);
+    std::vector<char> image;
+    if (int rc = batch_tables(b, bufs, order, operator_runs, metro_steps, series_offset, image)) return rc;
+    char *tables = nullptr;
+    if (int rc = scratch_address(b->device, &tables)) return rc;
+    const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // every call uploads its tables: the last call on this GPU may have left another batch's
+    if (int rc = upload_tables(tables, image, st)) return rc;
+    // the counter is cleared by the reset kernel of batch_launch
+    return batch_launch(b, tables, operator_runs, metro_steps, series_offset, grid_blocks, st);
 }
 
 int fs_batch_capture(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,

@@ -198,6 +198,16 @@ int fs_batch_path_storage(const fs_batch *b, int32_t type, int *storage);
 int fs_batch_build_log(const fs_batch *b, char *buf, size_t len, size_t *needed);
 int fs_batch_run(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
                  int32_t metro_steps, int32_t series_offset, int32_t grid_blocks, void *stream);
+/* fs_batch_run that allocates nothing and uses no device memory of the
+ * batch's own: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The dispatch tables and the work
+ * counter live in one fixed 16 KiB static device array per GPU (at most 64
+ * types), which each call overwrites (tables copied on `stream`, counter
+ * cleared by a kernel) and every call on that GPU shares: calls that overlap
+ * on one GPU race.  The batch's device must be current. */
+int fs_batch_run_noalloc(fs_batch *b, const fs_buffers *bufs, const int32_t *order, int32_t operator_runs,
+                         int32_t metro_steps, int32_t series_offset, int32_t grid_blocks, void *stream);
 /* Records n_launches consecutive fs_batch_run launches (measurement series
  * offsets advancing by operator_runs) into a HIP graph; replay with
  * fs_graph_launch.  Buffers and order are fixed at capture. */
