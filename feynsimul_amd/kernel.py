@@ -567,9 +567,16 @@ class PIMCKernel:
         return 1e-3 * self._ev0.elapsed_time(self._ev1)
 
     def getAcceptanceRate(self):
-        """Mean acceptance ratio of the bisection moves of the last run (kernel.py:648-658)."""
+        """Mean acceptance ratio of the bisection moves of the last launch (kernel.py:648-658).
+
+        The denominator is the length of the last launch (operator runs x steps), which is the
+        configured length for ``run`` and differs for the launches a learned proposal or a batch
+        makes; a launch that only measured (no steps) has no acceptance rate."""
         acc = self._accepts.cpu().numpy().view(np.uint32).astype(np.float64)
-        moves = float(self._operatorRuns * self._metroStepsPerOperatorRun)
+        opRuns, M = self._lastRunLength or (self._operatorRuns, self._metroStepsPerOperatorRun)
+        moves = float(opRuns * M)
+        if moves == 0:
+            raise Exception("the last launch made no Metropolis moves (it only measured)")
         if self._sampler == "wave":
             return acc.sum() / (self._nbrOfWalkers * (self._N // 2 ** self._S) * moves)
         return (acc / moves).mean()

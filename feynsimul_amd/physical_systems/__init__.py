@@ -16,6 +16,8 @@ contract of FeynSimul/physical_systems/__init__.py:1-18.
 from .harm_osc import HarmOsc
 from .anharm_osc import AnharmOsc
 from .lm2m2 import Lm2m2Cluster
-from .lm2m2_cluster import Lm2m2_cluster
+from .lm2m2_cluster import Lm2m2_cluster, JacobiHeCluster
+from .lm2m2_3part import Lm2m2_3part
+from .lm2m2_4part import Lm2m2_4part
 
-__all__ = ["HarmOsc", "AnharmOsc", "Lm2m2Cluster", "Lm2m2_cluster"]
+__all__ = ["HarmOsc", "AnharmOsc", "Lm2m2Cluster", "Lm2m2_cluster", "JacobiHeCluster", "Lm2m2_3part", "Lm2m2_4part"]

@@ -21,7 +21,7 @@ m = M_He angstrom^2 k_B / hbar^2 (lm2m2_cluster.py:23-32).
 """
 import numpy as np
 
-from .lm2m2 import LM2M2, _user_code
+from .lm2m2 import LM2M2, _user_code, _KB as _KB_SI
 
 _HE_MASS = 6.64647617e-27   # kg, as lm2m2_cluster.py:23
 _HBAR = 1.05457148e-34      # m^2 kg / s, lm2m2_cluster.py:24
@@ -94,3 +94,52 @@ class Lm2m2_cluster:
         self.meanSquaredRadiusOp = " + ".join("(%s)" % e for e in s)
         # the pair functions take r^2 in angstrom^2 (qscale = 1 / r_m)
         self.userCode = _user_code(1.0 / LM2M2["rm"])
+
+
+_EV = 1.602176634e-19   # J
+
+
+class JacobiHeCluster:
+    """n He atoms in unit-mass Jacobi coordinates, lengths in ``xUnit`` metres and energies in
+    ``potentialUnit`` joules, with hbar = m_He = 1 in those units (potentialUnit = hbar^2 /
+    (m_He xUnit^2)).  This is the unit convention of the reference's fixed-size systems
+    (physical_systems/lm2m2_3part.py:22-31, lm2m2_4part.py:22-33); the strings are generated
+    here from the Jacobi map of ``jacobi_matrix`` (unit mass), not copied.
+
+    Strings (in x1 .. x{3(n-1)}):
+      potential, energyOp      sum_pairs V(r) and the virial sum_pairs V + r V'(r) / 2, in potentialUnit
+      meanSquaredRadiusOp      (1/n) sum_i |r_i - R|^2            [xUnit^2]
+      meanRadiusOp             (1/n) sum_i |r_i - R|              [xUnit]
+      meanSquaredAtomDistOp    mean over pairs of |r_a - r_b|^2   [xUnit^2]
+    ``userCode`` also defines ``sqr(a)`` and ``lm2m2(r)`` (pair energy in eV at r angstrom), the
+    helper names of the reference's strings.
+    """
+
+    def __init__(self, n, xUnit, potentialUnit):
+        if n < 2:
+            raise Exception("n must be >=2")
+        self.nAtoms = n
+        self.DOF = 3 * (n - 1)
+        self.xUnit = float(xUnit)
+        self.potentialUnit = float(potentialUnit)
+        xu = self.xUnit * 1e10                        # angstrom per length unit
+        scale = _KB_SI / self.potentialUnit           # kelvin -> potentialUnit
+        self.pairs = sorted(pair_coefficients(n, 1.0).items())
+        r2 = ["fs_sqr(%s) + fs_sqr(%s) + fs_sqr(%s)" % tuple(_linear(c, k) for k in range(3)) for _, c in self.pairs]
+        self.potential = "%r * (%s)" % (scale, " + ".join("fs_lm2m2_r2(%s)" % e for e in r2))
+        self.energyOp = "%r * (%s)" % (scale, " + ".join("fs_lm2m2_virial_r2(%s)" % e for e in r2))
+        sq = " + ".join("fs_sqr(x%d)" % (i + 1) for i in range(self.DOF))
+        self.meanSquaredRadiusOp = "%r * (%s)" % (1.0 / n, sq)
+        self.meanSquaredAtomDistOp = "%r * (%s)" % (2.0 / (n - 1), sq)
+        # r_i - R in Jacobi coordinates: row i of J^-1 minus the mean row; the centre of mass drops out
+        Jinv = np.linalg.inv(jacobi_matrix(n, 1.0))
+        rel = Jinv - Jinv.mean(axis=0)
+        assert np.abs(rel[:, n - 1]).max() < 1e-12
+        radii = ["sqrt(fs_sqr(%s) + fs_sqr(%s) + fs_sqr(%s))" % tuple(_linear(rel[i, : n - 1], k) for k in range(3))
+                 for i in range(n)]
+        self.meanRadiusOp = "%r * (%s)" % (1.0 / n, " + ".join(radii))
+        kev = _KB_SI / _EV
+        self.userCode = _user_code(xu / LM2M2["rm"]) + (
+            "FLOAT_TYPE sqr(FLOAT_TYPE a) { return a * a; }\n"
+            "/* LM2M2 pair energy in eV at a distance of r angstrom */\n"
+            "FLOAT_TYPE lm2m2(FLOAT_TYPE r) { return %r * fs_lm2m2_pair(fs_sqr(r * %r)); }\n" % (kev, 1.0 / xu))

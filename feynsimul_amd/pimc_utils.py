@@ -28,7 +28,10 @@ from time import time
 
 import numpy as np
 
-from .kernel import PIMCKernel
+# the reference's pimc_utils star-imports kernel_args and kernel (pimc_utils.py:25-26), and its
+# scripts take KernelArgs from ``from FeynSimul.pimc_utils import *``
+from .kernel_args import KernelArgs  # noqa: F401
+from .kernel import PIMCKernel, humanReadableSize  # noqa: F401
 
 
 def doublePaths(oldPaths, N, DOF):

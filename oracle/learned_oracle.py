@@ -239,8 +239,10 @@ def sigma_table(S, eps):
 
 
 def learned_sweep(paths, p, V, *, S, eps, seed, sweep, atoms3d=False, in_scale=1.0, emulate_bf16=True,
-                  return_detail=False):
+                  return_detail=False, walker0=0):
     """paths: float32 (W, D, N), updated in place.  V: vectorised potential (..., D) -> (...).
+    ``walker0``: the walker number of paths[0] in the random-number counters (a slice of a
+    larger ensemble draws what those walkers draw in the whole).
 
     Returns accepted segments per walker (and, with return_detail, the per-segment
     log alpha, uniforms and proposals).
@@ -250,7 +252,8 @@ def learned_sweep(paths, p, V, *, S, eps, seed, sweep, atoms3d=False, in_scale=1
     nseg = N // SEG
     sig = sigma_table(S, eps)
     wi = np.arange(W)
-    r = offsets(seed, wi, sweep, SEG)                                   # (W,)
+    wg = wi + int(walker0)
+    r = offsets(seed, wg, sweep, SEG)                                   # (W,)
     j = np.arange(nseg)
     pos = np.arange(SEG + 1)
     slices = (r[:, None, None] + j[None, :, None] * SEG + pos[None, None, :]) % N   # (W, nseg, SEG+1)
@@ -269,7 +272,7 @@ def learned_sweep(paths, p, V, *, S, eps, seed, sweep, atoms3d=False, in_scale=1
             o = o.reshape(f.shape[:-1] + (o.shape[-1],))
             mu, s, sd = proposal_params(o, a, b, sig[n], D)
             if which == "new":
-                z = normals(seed, wi[:, None, None], sweep, slices[:, :, m], D)
+                z = normals(seed, wg[:, None, None], sweep, slices[:, :, m], D)
                 new[:, :, m, :] = (mu + sd * z).astype(np.float32)
                 lq -= np.sum(-0.5 * z * z - s, axis=(2, 3)).astype(np.float32)
                 if n == 1:
@@ -284,7 +287,7 @@ def learned_sweep(paths, p, V, *, S, eps, seed, sweep, atoms3d=False, in_scale=1
     dK = (kin(new) - kin(old)).astype(np.float32)
     dV = (np.sum(V(new[:, :, 1:SEG, :]), axis=2) - np.sum(V(old[:, :, 1:SEG, :]), axis=2)).astype(np.float32)
     logA = (-dK - np.float32(eps) * dV + lq).astype(np.float32)
-    u = accept_uniforms(seed, wi[:, None], sweep, j[None, :])
+    u = accept_uniforms(seed, wg[:, None], sweep, j[None, :])
     with np.errstate(over="ignore"):
         acc = ~(np.exp(logA) < u)
     for w in range(W):

@@ -227,3 +227,122 @@ def test_training_improves_a_stiff_oscillator(gpu):
     print("NLL %.5f -> %.5f, acceptance plain %.3f learned %.3f" % (first, last, plain, ar))
     assert last < first - 0.01
     assert ar > plain
+
+
+def test_fallback_below_twenty_percent_keeps_ho_energy(gpu):
+    """C1 sizes with a network so poor that learned blocks accept under 20 %: every such block is
+    followed by a plain bisection block (the fallback), and <x^2> stays at the discretised-HO value
+    0.499950082044 within 4 standard errors."""
+    _case("learned fallback, C1")
+    system = HarmOsc()
+    W, N, S, beta = 4096, 256, 4, 10.0
+    k = _kernel(system, W, N, S, beta, opRuns=1, M=1500)
+    k.setPaths(np.random.default_rng(1).uniform(-1, 1, (W, N)).astype(np.float32))
+    k.run()
+    lp = LearnedProposal(k, batch=2048, seed=10, fallbackAcceptance=0.2)
+    lp.setParameters(L.init_params(1, 4, scale_last=0.3))
+    ar = lp.sweep(16)
+    k.setSeriesCapacity(120)
+    lp.run(120, 16)
+    s = k.getSeries()[:, :, 0].astype(np.float64)
+    per = s.mean(axis=0)
+    E, se = per.mean(), per.std() / np.sqrt(W)
+    exact = O.ho_discrete_x2(N, beta)
+    print("E %.6f +- %.6f exact %.9f, learned AR %.3f, fallback blocks %d" % (E, se, exact, ar, lp.fallbackBlocks))
+    assert ar < 0.2
+    assert lp.fallbackBlocks > 0
+    # the last launch was a fallback block of 16 plain sweeps: its acceptance is a probability
+    assert 0.0 < k.getAcceptanceRate() < 1.0
+    assert abs(E - exact) < 4 * se
+
+
+def test_c5_learned_leg_matches_transfer_matrix(gpu):
+    """The C5 system (x^2/2 + 0.1 x^4, N = 4096, beta = 40) sampled with the trained learned
+    proposal: the virial E0 and the two-time correlator over 96 lags against the FP64 transfer
+    matrix fixture (tests/golden/aho_c5.json), with the plain test's bounds (E0 within 4 standard
+    errors, correlator max z < 5.5)."""
+    import json
+    import os
+    from feynsimul_amd.physical_systems import AnharmOsc
+    _case("learned C5 AHO N=4096")
+    fix = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "aho_c5.json")))
+    system = AnharmOsc(0.1)
+    W, N, S, beta, lags = 2048, 4096, 4, 40.0, 96
+    ka = KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=S, enableOperator=True,
+                    enableCorrelator=True, correlators=("x1",), correlatorLags=lags, enableBisection=True,
+                    enablePathShift=False, enableSingleNodeMove=False, enableParallelizePath=True,
+                    enableGlobalPath=False, enableGlobalOldPath=False, enableBins=False,
+                    metroStepsPerOperatorRun=4000, operatorRuns=1, nbrOfWalkersPerWorkGroup=1,
+                    operators=(system.energyOp,), seed=31, sampler="wave", pathStorage="memory")
+    k = PIMCKernel(ka)
+    k.setPaths(np.random.default_rng(5).uniform(-1, 1, (W, N)).astype(np.float32))
+    k.run()
+    lp = LearnedProposal(k, batch=65536, seed=32)
+    first = lp.train(1)
+    last = lp.trainOnline(40)
+    k.resetCorrelator()
+    k.setSeriesCapacity(48)
+    lp.run(48, 8)
+    s = k.getSeries()[:, :, 0].astype(np.float64)
+    per = s.mean(axis=0)
+    E, se = per.mean(), per.std() / np.sqrt(W)
+    mean, cse = k.getCorrelator()
+    cor = np.array(fix["values"]["correlator"][:lags])
+    z = np.abs(mean[0] - cor) / cse[0]
+    print("NLL %.4f -> %.4f, learned AR %.3f, fallback %d; E0 %.5f +- %.5f vs %.5f; correlator max z %.2f"
+          % (first, last, lp.lastAcceptance, lp.fallbackBlocks, E, se, fix["values"]["E_virial"], z.max()))
+    assert lp.fallbackBlocks == 0 and lp.lastAcceptance > 0.5
+    assert abs(E - fix["values"]["E_virial"]) < 4 * se
+    assert z.max() < 5.5
+
+
+def test_c4_shape_learned_sweep_and_training_step(gpu):
+    """One learned sweep and one Adam step at the learned-proposal workload's shape: 3 He atoms,
+    N = 1024, S = 6, 16384 chains, batch 65536.  Properties of the whole sweep (acceptance is a
+    probability, paths finite, segment ends and rejected segments unchanged), and a 32-walker slice
+    of it against the oracle's sweep of the same walkers (their random numbers are counter-based)."""
+    import torch
+    _case("learned He3 N=1024 x 16384 chains")
+    system = Lm2m2Cluster(3)
+    W, N, S, beta, D = 16384, 1024, 6, 10.0, 9
+    rng = np.random.default_rng(6)
+    base = system.initialPositions(W, rng)
+    paths0 = (base[:, :, None] + 0.05 * rng.standard_normal((W, D, N))).astype(np.float32)
+    k = _kernel(system, W, N, S, beta)
+    k.setPaths(paths0.reshape(W, D * N))
+    lp = LearnedProposal(k, batch=65536, seed=4242)
+    theta = L.init_params(D, 8, scale_last=0.05)
+    lp.setParameters(theta)
+    lp._sweepCount = 3
+    lp.traceSweeps()
+    ar = lp.sweep(1)
+    got = k.getPaths().reshape(W, D, N)
+    tr = lp.getTrace()
+    assert 0.0 < ar < 1.0 and np.isfinite(got).all() and np.isfinite(tr).all()
+    # a segment either moved as a whole interior or not at all; log alpha decides (trace column 3)
+    changed = (got != paths0).any(axis=1)                        # (W, N)
+    SEG = 1 << S
+    w0 = 5000
+    sl = slice(w0, w0 + 32)
+    ref = paths0[sl].copy()
+    _, d = L.learned_sweep(ref, L.unflatten(theta, D), _lm2m2_V(system), S=S, eps=beta / N, seed=4242, sweep=3,
+                           atoms3d=True, return_detail=True, walker0=w0)
+    for i, w in enumerate(range(w0, w0 + 32)):
+        ends = (d["r"][i] + SEG * np.arange(N // SEG)) % N
+        assert not changed[w, ends].any()
+    gpu_acc = _segment_decisions(paths0[sl], got[sl], d["r"], S, N)
+    clear = np.abs(d["logA"] - np.log(np.maximum(d["u"], 1e-30))) > 0.05
+    assert clear.mean() > 0.8
+    assert np.array_equal(gpu_acc[clear], d["accept"][clear])
+    np.testing.assert_allclose(tr[sl][..., 3][clear], d["logA"][clear], rtol=2e-2, atol=2e-2)
+    both = gpu_acc & d["accept"]
+    for i, j in zip(*np.nonzero(both)):
+        s_ = (d["r"][i] + j * SEG + np.arange(1, SEG)) % N
+        assert np.abs(got[w0 + i][:, s_] - ref[i][:, s_]).max() < 5e-3
+    # one Adam step on 65536 rows of the whole ensemble
+    loss = lp.train(1)
+    th1 = lp.getParameters()
+    print("C4 shape: acceptance %.3f, NLL %.4f" % (ar, loss))
+    assert np.isfinite(loss) and np.isfinite(th1).all() and not np.array_equal(th1, theta)
+    del lp, k
+    torch.cuda.empty_cache()

@@ -56,3 +56,30 @@ def test_modn_doubles_n_writes_files_and_resumes(gpu, tmp_path):
     assert list(rows2[:, 0]) == [64] * 6 and int(rows2[0, 4]) == int(last[4])
     per2 = rows2[-1, 5:]
     assert abs(per2.mean() - exact) < 5 * per2.std() / np.sqrt(W)
+
+
+def test_reference_he3_harness_runs_through_the_alias(gpu, tmp_path, monkeypatch):
+    """examples/lm2m2_3particles.py:28-77 through ``install_feynsimul_alias``: Lm2m2_3part (Jacobi,
+    DOF 6, kernel beta 1000 = 15 mK), three operators, modN from N = 8 (straight paths in a 0.01-unit
+    cube) to N = 32 with the script's run formulas; verbosity 2 dumps each generated module source."""
+    from test_reference_harness import run_harness
+    print("[case] lm2m2_3particles harness N 8 -> 32", flush=True)
+    monkeypatch.chdir(tmp_path)
+    run_harness(finalN=32, simTime=0, verbosity=2, save=5)
+    d = glob.glob(str(tmp_path / "results" / "lm2m2_3part" / "*" / "*"))[0]
+    beta, rows = readModN(os.path.join(d, "data.tsv"))
+    assert beta == 1000.0
+    assert list(rows[:, 0]) == [8] * 10 + [16] * 10 + [32] * 10
+    assert rows.shape[1] == 5 + 3 * 64
+    assert np.isfinite(rows).all()
+    ar, S = rows[:, 3], rows[:, 4]
+    assert ((ar > 0) & (ar <= 1)).all()
+    assert ((S >= 1) & (S <= np.log2(rows[:, 0]))).all()
+    ops = rows[:, 5:].reshape(len(rows), 3, 64)
+    # the atoms start within 0.01 length units of each other (mean squared radius <= 2e-4), deep in
+    # the repulsive core; the sampler pulls them apart
+    msr = ops[:, 1].mean(axis=1)
+    assert msr[-1] > 100 * 2e-4 and (ops[:, 1] > 0).all() and (ops[:, 2] > 0).all()
+    assert os.path.exists("cleaned.hip")
+    for N in (8, 16, 32):
+        assert os.path.exists(os.path.join(d, "pathsN%depisode2" % N))
