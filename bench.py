@@ -51,24 +51,65 @@ def measured_traffic(profile=TRAFFIC_PROFILE):
         return None
 
 
-def measured_valu_issue(profile=TRAFFIC_PROFILE):
-    """Fraction of VALU issue capacity the kernel used, from the SQ counters in its profile summary (the last
-    measurement the summary records)."""
+VALU_NOMINAL = 256 * 64 * 2.4e9  # lane-ops/s: one wave64 VALU instruction per 4 cycles per SIMD at 2.4 GHz
+VALU_CALIBRATION = os.path.join(ROOT, "profiles", "valu_issue_calibration_xorshift_1m_streams.md")
+
+
+def _profile_value(profile, key):
     import re
     try:
         with open(profile) as f:
-            m = re.findall(r"VALU issue (?:is )?[0-9.e]+ (?:wave-instructions )?x 4 (?:cycles )?/ \(1024[^)]*\) = ([0-9.]+)",
-                           f.read().replace("×", "x"))
+            m = re.findall(r"^%s:\s*([0-9.eE+-]+)" % re.escape(key), f.read(), re.M)
         return float(m[-1]) if m else None
     except OSError:
         return None
-VALU_PEAK = 256 * 64 * 2.4e9  # lane-ops/s, one FP32 op per lane per clock
+
+
+def valu_fields(achieved_ops, profile):
+    """VALU side of the roofline.  achieved = as-written operations per second of this run; the issue
+    fraction comes from the kernel's profile summary (its SQ_INSTS_VALU per SIMD per cycle, the last
+    measurement the summary records) divided by the highest issue rate measured on the box
+    (fs_k_xs_bench at 1,048,576 streams, profiles/valu_issue_calibration_xorshift_1m_streams.md)."""
+    issue = _profile_value(profile, "valu_issue_per_simd_cycle")
+    peak_issue = _profile_value(VALU_CALIBRATION, "valu_issue_per_simd_cycle_max")
+    out = {"ops_per_update": None, "achieved": achieved_ops, "nominal_peak": VALU_NOMINAL,
+           "nominal_frac": achieved_ops / VALU_NOMINAL,
+           "issue_per_simd_cycle_from_profile": issue, "issue_peak_per_simd_cycle_measured": peak_issue,
+           "issue_frac": issue / peak_issue if issue and peak_issue else None,
+           "source": "SQ counters in %s and %s (profile summaries, not this run)"
+                     % (os.path.relpath(profile, ROOT), os.path.relpath(VALU_CALIBRATION, ROOT))}
+    return out
+
+
 MFMA_BF16_PEAK = 2.5e15       # dense BF16 MFMA FLOP/s
 # learned proposal: one network evaluation = 2 (19*128 + 128*128*2 + 128*18) = 75,008 FLOP; a
 # segment evaluates 2^S - 1 proposals and 2^S - 2 reverse rows (level 1 shares its row)
 LP_FLOP_PER_ROW = 2 * (19 * 128 + 2 * 128 * 128 + 128 * 18)
 LP_ROWS_PER_SEGMENT = 2 * (SEG - 1) - 1
 LP_FLOP_PER_UPDATE = LP_FLOP_PER_ROW * LP_ROWS_PER_SEGMENT / (SEG - 1)
+
+
+ORACLE_LAUNCHES = 23  # 736 sweeps: a sweep count the FP64 reference holds walker by walker
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def reap_children():
+    """Waits for (then ends) any child process still running, so the bench leaves nothing behind."""
+    try:
+        import psutil
+    except ImportError:
+        return
+    kids = psutil.Process().children(recursive=True)
+    for c in kids:
+        progress("child process still running at exit: pid %d %s" % (c.pid, " ".join(c.cmdline()[:4])))
+        try:
+            c.terminate()
+        except psutil.NoSuchProcess:
+            pass
+    psutil.wait_procs(kids, timeout=5)
 
 
 def make_kernel(steps_per_launch, seed, storage=None, operator_runs=1):
@@ -135,9 +176,21 @@ def chain_mean_ess(series):
     return float(x.var() / se2), float(np.sqrt(se2))
 
 
+def drift(series, sweeps_per_measurement=SWEEPS_PER_STEP):
+    """Linear trend of the ensemble mean of a (measurements, chains) series: slope per 1000 sweeps and its
+    standard error from the chain-by-chain slopes (chains are independent)."""
+    import numpy as np
+    x = np.asarray(series, dtype=np.float64)
+    t = np.arange(x.shape[0], dtype=np.float64) * sweeps_per_measurement / 1000.0
+    tc = t - t.mean()
+    slopes = (tc[:, None] * (x - x.mean(axis=0))).sum(axis=0) / (tc * tc).sum()
+    return float(slopes.mean()), float(slopes.std(ddof=1) / np.sqrt(len(slopes)))
+
+
 def ess_phase(paths, storage, burn_launches, meas_launches, runs_per_launch=16):
-    """ESS/s of the virial energy E0: series of per-chain slice means, one every
-    32 sweeps, after burn-in; tau_int by Sokal windowing over all chains."""
+    """Stationarity of E0 on the He3 workload: the virial energy series of every chain (slice mean, one
+    measurement every 32 sweeps) after burn-in.  An ESS is only quoted when Sokal's tau_int is at most
+    1/5 of the series; otherwise the series' drift is reported instead."""
     import torch
     from feynsimul_amd.stats import effective_sample_size
     _, k = make_kernel(SWEEPS_PER_STEP, seed=777, storage=storage, operator_runs=runs_per_launch)
@@ -154,31 +207,33 @@ def ess_phase(paths, storage, burn_launches, meas_launches, runs_per_launch=16):
     dt = time.perf_counter() - t0
     series = k.getSeries()[:, :, 0]
     r = effective_sample_size(series)
-    ess_cm, se_cm = chain_mean_ess(series)
-    sweeps = meas_launches * runs_per_launch * SWEEPS_PER_STEP
-    return {"metric": "ESS/s of E0 (virial energy), plain bisection sampler", "value": r["ess"] / dt,
-            "ess_chain_means": ess_cm, "ess_chain_means_per_s": ess_cm / dt, "se_chain_means_K": se_cm,
-            "unit": "effective samples/s", "E0_K": r["mean"], "sigma_K": r["se"],
-            "tau_int_measurements": r["tau_int"], "tau_int_sweeps": r["tau_int"] * SWEEPS_PER_STEP,
-            "ess": r["ess"], "seconds": dt, "sweeps": sweeps,
-            "burn_in_sweeps": burn_launches * runs_per_launch * SWEEPS_PER_STEP,
-            "seconds_to_sigma_1e-3_K": dt * (r["se"] / 1e-3) ** 2}
+    slope, slope_se = drift(series)
+    M = series.shape[0]
+    out = {"metric": "E0 (virial energy) series of the plain bisection sampler", "unit": "K",
+           "measurements": M, "sweeps_per_measurement": SWEEPS_PER_STEP,
+           "burn_in_sweeps": burn_launches * runs_per_launch * SWEEPS_PER_STEP, "seconds": dt,
+           "E0_K": r["mean"], "tau_int_measurements": r["tau_int"],
+           "drift_K_per_1000_sweeps": slope, "drift_se": slope_se, "drift_z": abs(slope) / slope_se}
+    if r["tau_int"] <= M / 5.0:
+        out["ess"], out["ess_per_s"] = r["ess"], r["ess"] / dt
+    else:
+        out["ess"] = None
+        out["note"] = ("not stationary: tau_int exceeds 1/5 of the series and E0 drifts (the trimer is unbound at "
+                       "beta = 10/K and slowly dissociates), so no ESS is quoted; the ESS/s comparison at fixed sigma "
+                       "runs on the stationary workloads (ess_at_fixed_sigma)")
+    return out
 
 
-def learned_phase(paths, train_steps, measurements, plain_ess):
-    """The learned proposal on the same workload, started from the paths the plain
-    ESS phase starts from (after its burn-in), on a similar wall-clock budget: online training (Adam, 65,536 rows a step, each step after 2
-    plain sweeps that produce its rows), then learned sweeps with one measurement every
-    32; ESS/s counts the training time.  Also the learned sweep's throughput against
-    the BF16 MFMA peak."""
-    import numpy as np
+def learned_phase(paths, train_steps, measurements):
+    """The learned proposal on the He3 workload, started from the paths the plain series starts from:
+    learned sweep throughput against the BF16 MFMA peak, online training (Adam, 65,536 rows a step, each
+    step after 2 plain sweeps that produce its rows), then learned sweeps with one measurement every 32
+    (the E0 series drifts as the plain one does, so no ESS is quoted here)."""
     import torch
     from feynsimul_amd.learned import LearnedProposal
-    from feynsimul_amd.stats import effective_sample_size
     _, k = make_kernel(SWEEPS_PER_STEP, seed=778, storage="memory")
     k.setPaths(paths)
     lp = LearnedProposal(k, batch=65536, lr=1e-3, seed=779)
-    # learned sweep throughput and the untrained (Levy) acceptance
     lp.sweep(2)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     ev0.record(k._stream)
@@ -187,42 +242,145 @@ def learned_phase(paths, train_steps, measurements, plain_ess):
     torch.cuda.synchronize()
     ms_sweep = ev0.elapsed_time(ev1) / 8
     ar0 = lp.lastAcceptanceOfQueued(8)
-    # online training on paths from the plain sampler
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     first = lp.train(1)
     last = lp.trainOnline(train_steps - 1, plainSweepsPerStep=2)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    # learned sampling with measurements
     k.setSeriesCapacity(measurements)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
     lp.run(measurements, SWEEPS_PER_STEP)
-    t_run = time.perf_counter() - t1
     series = k.getSeries()[:, :, 0]
-    r = effective_sample_size(series)
-    ess_cm, se_cm = chain_mean_ess(series)
-    dt = t_train + t_run
+    slope, slope_se = drift(series)
     flop = LP_FLOP_PER_UPDATE * UPDATES_PER_SWEEP
-    out = {"metric": "ESS/s of E0 (virial energy), learned proposal (training time included)",
-           "value": ess_cm / dt, "unit": "effective samples/s (chain-mean estimator)",
-           "ess_chain_means": ess_cm, "se_chain_means_K": se_cm, "ess_sokal": r["ess"],
-           "tau_int_measurements": r["tau_int"], "E0_K": r["mean"],
-           "seconds_train": t_train, "seconds_sampling": t_run, "adam_steps": train_steps,
-           "nll_first": first, "nll_last": last, "acceptance_untrained": ar0,
-           "acceptance_trained": lp.lastAcceptance, "fallback_blocks": lp.fallbackBlocks,
-           "sweeps": measurements * SWEEPS_PER_STEP, "ms_per_learned_sweep": ms_sweep,
-           "slice_updates_per_s": UPDATES_PER_SWEEP / (ms_sweep * 1e-3),
-           "mfma": {"bound": "mfma", "kernel": "fs_lp_sweep", "achieved": flop / (ms_sweep * 1e-3),
-                    "peak": MFMA_BF16_PEAK, "unit": "FLOP/s", "frac": flop / (ms_sweep * 1e-3) / MFMA_BF16_PEAK,
-                    "flop_per_update": LP_FLOP_PER_UPDATE, "updates_per_launch": UPDATES_PER_SWEEP}}
-    if plain_ess:
-        out["ess_per_s_ratio_learned_over_plain"] = (ess_cm / dt) / plain_ess["ess_chain_means_per_s"]
-    return out
+    return {"metric": "learned proposal on He3: throughput, training and acceptance",
+            "seconds_train": t_train, "adam_steps": train_steps, "nll_first": first, "nll_last": last,
+            "acceptance_untrained": ar0, "acceptance_trained": lp.lastAcceptance, "fallback_blocks": lp.fallbackBlocks,
+            "E0_K": float(series.mean()), "drift_K_per_1000_sweeps": slope, "drift_se": slope_se,
+            "sweeps": measurements * SWEEPS_PER_STEP, "ms_per_learned_sweep": ms_sweep,
+            "slice_updates_per_s": UPDATES_PER_SWEEP / (ms_sweep * 1e-3),
+            "mfma": {"bound": "mfma", "kernel": "fs_lp_sweep", "achieved": flop / (ms_sweep * 1e-3),
+                     "peak": MFMA_BF16_PEAK, "unit": "FLOP/s", "frac": flop / (ms_sweep * 1e-3) / MFMA_BF16_PEAK,
+                     "flop_per_update": LP_FLOP_PER_UPDATE, "updates_per_launch": UPDATES_PER_SWEEP}}
 
 
-def ragged_bench(steps, warmup):
+# ---- ESS/s of E0 at fixed sigma, learned vs plain, on the stationary workloads ----
+def _series_stats(series, exact):
+    import numpy as np
+    from feynsimul_amd.stats import effective_sample_size
+    r = effective_sample_size(series)
+    x = np.asarray(series, dtype=np.float64)
+    return {"E0": r["mean"], "sigma": r["se"], "tau_int_measurements": r["tau_int"],
+            "measurements": int(x.shape[0]), "ess": float(x.var() / r["se"] ** 2),
+            "z": abs(r["mean"] - exact) / r["se"]}
+
+
+def run_to_sigma(block, get_series, sigma, exact, max_seconds, measurements_per_block):
+    """Calls block() (one block of measurements on the GPU) until the chain-mean standard error of E0
+    is at most sigma and Sokal's tau_int is at most 1/5 of the series, or max_seconds of sampling have
+    passed.  Wall time counts the blocks only (each ends synchronised), not the host analysis."""
+    import torch
+    t = 0.0
+    while True:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        block()
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
+        st = _series_stats(get_series(), exact)
+        if (st["sigma"] <= sigma and st["tau_int_measurements"] <= st["measurements"] / 5.0) or t > max_seconds:
+            break
+    st["seconds"] = t
+    st["sigma_target"] = sigma
+    st["reached"] = bool(st["sigma"] <= sigma)
+    # time to exactly sigma (the ESS grows linearly with sampling time)
+    st["seconds_to_sigma"] = t * (st["sigma"] / sigma) ** 2
+    st["measurements_per_block"] = measurements_per_block
+    return st
+
+
+def fixed_sigma(workload, sigma, max_seconds=40.0, train_steps=100, S=None):
+    """Plain and learned samplers from the same equilibrated paths, each run to the same standard
+    error sigma of E0 (virial energy, slice mean, one measurement every 32 sweeps).  ESS/s counts the
+    learned proposal's online training.  ratio = ESS/s learned / ESS/s plain at that sigma."""
+    import numpy as np
+    import torch
+    from feynsimul_amd.kernel_args import KernelArgs
+    from feynsimul_amd.kernel import PIMCKernel
+    from feynsimul_amd.learned import LearnedProposal
+    from feynsimul_amd.physical_systems import HarmOsc, AnharmOsc
+    if workload == "ho":
+        system, W, NN, beta = HarmOsc(), HO_W, HO_N, HO_BETA
+        S = S or HO_S
+        exact = ho_exact_x2(HO_N, HO_BETA)
+        burn = 2000
+    else:
+        with open(AHO_GOLDEN) as f:
+            exact = json.load(f)["values"]["E_virial"]
+        system, W, NN, beta = AnharmOsc(AHO_LAM), AHO_W, AHO_N, AHO_BETA
+        S = S or AHO_S
+        burn = 2048
+    runs = 16
+
+    def kernel(M, opRuns, seed):
+        return PIMCKernel(KernelArgs(system=system, nbrOfWalkers=W, N=NN, beta=beta, S=S, enableOperator=True,
+                                     enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                                     enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                                     enableGlobalOldPath=False, enableBins=False, metroStepsPerOperatorRun=M,
+                                     operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=1, operators=(system.energyOp,),
+                                     seed=seed, sampler="wave", pathStorage="memory"))
+    kb = kernel(burn, 1, 41)
+    kb.setPaths(np.random.default_rng(40).uniform(-1, 1, (W, NN)).astype(np.float32))
+    kb.run()
+    start = kb.getPaths()
+    del kb
+    cap = 4096 if W * 4096 * 4 <= (1 << 30) else 1024
+    # plain
+    kp = kernel(SWEEPS_PER_STEP, runs, 42)
+    kp.setPaths(start)
+    kp.setSeriesCapacity(cap)
+    plain = run_to_sigma(lambda: kp.run(wait=False), lambda: kp.getSeries()[:, :, 0], sigma, exact,
+                         max_seconds, runs)
+    plain["acceptance"] = kp.getAcceptanceRate()
+    plain["ms_per_sweep"] = None
+    del kp
+    torch.cuda.empty_cache()
+    # learned: online training from the same paths, then learned sweeps
+    kl = kernel(SWEEPS_PER_STEP, runs, 43)
+    kl.setPaths(start)
+    lp = LearnedProposal(kl, batch=65536, lr=1e-3, seed=44)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    first = lp.train(1)
+    last = lp.trainOnline(train_steps - 1, plainSweepsPerStep=2)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    kl.setSeriesCapacity(cap)
+    learned = run_to_sigma(lambda: lp.run(runs, SWEEPS_PER_STEP), lambda: kl.getSeries()[:, :, 0], sigma, exact,
+                           max(max_seconds - t_train, 1.0), runs)
+    learned.update({"seconds_train": t_train, "adam_steps": train_steps, "nll_first": first, "nll_last": last,
+                    "acceptance": lp.lastAcceptance, "fallback_blocks": lp.fallbackBlocks})
+    del kl, lp
+    torch.cuda.empty_cache()
+    t_plain = plain["seconds_to_sigma"]
+    t_learned = learned["seconds_to_sigma"] + t_train
+    for r, t in ((plain, t_plain), (learned, t_learned)):
+        r["ess_per_s"] = r["ess"] / t * (r["sigma"] / sigma) ** 2
+    # the ceiling if the network cost nothing: effective samples per sweep, learned over plain
+    per_sweep = [r["ess"] / (r["measurements"] * SWEEPS_PER_STEP) for r in (plain, learned)]
+    return {"workload": {"ho": "harmonic oscillator N=256, beta=10, 4096 chains",
+                         "aho": "anharmonic oscillator x^2/2 + 0.1 x^4, N=4096, beta=40, 65536 chains"}[workload]
+                        + ", S=%d, E0 every 32 sweeps" % S,
+            "sigma": sigma, "E0_exact": exact, "plain": plain, "learned": learned,
+            "ratio_learned_over_plain": learned["ess_per_s"] / plain["ess_per_s"],
+            "ess_per_sweep_ratio_learned_over_plain": per_sweep[1] / per_sweep[0],
+            "bias_within_2_sigma": {"plain": plain["z"] <= 2.0, "learned": learned["z"] <= 2.0}}
+
+
+RAGGED_PROFILE = os.path.join(ROOT, "profiles", "fs_batch_sweep_ragged.md")
+
+
+def ragged_bench(steps, warmup, cpu=True, levels=None):
     """Batched ragged LM2M2 systems in one persistent launch (32 sweeps per launch), autotuned."""
     import numpy as np
     import torch
@@ -244,11 +402,17 @@ def ragged_bench(steps, warmup):
             pos = system.initialPositions(2048, rng, side=3.5, sigma=3.0)
             starts.append(np.repeat(pos[:, :, None], NN, axis=2).reshape(2048, system.DOF * NN))
     # bisection level per type from measured acceptance (the reference's 20-50 % window), then launch shape
-    levels, table = autotune_levels(kas, initialPaths=starts, warmupSweeps=512)
+    if levels is None:
+        progress("ragged: bisection level per type from measured acceptance")
+        levels, table = autotune_levels(kas, initialPaths=starts, warmupSweeps=512)
+    else:
+        table = []
     for ka, S in zip(kas, levels):
         ka.S = S
-    best, trials = autotune_batch(kas, initialPaths=starts, use_cache=False)
-    batch = BatchedPIMC(kas, wavesPerBlock=best["wavesPerBlock"], order=best["order"])
+    progress("ragged: launch shape (waves per workgroup, order, register classes; cached per device)")
+    best, trials = autotune_batch(kas, initialPaths=starts, use_cache=True, verbose=True)
+    batch = BatchedPIMC(kas, wavesPerBlock=best["wavesPerBlock"], order=best["order"], kernels=best["kernels"],
+                        kernelWaves=best["kernelWaves"])
     for k, p in zip(batch.kernels, starts):
         k.setPaths(p)
     for _ in range(warmup):
@@ -259,11 +423,14 @@ def ragged_bench(steps, warmup):
     batch.replay(graph)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    launch_s = batch.getRunTime() / steps   # device time of the replay (HIP events on its stream)
     upd = batch.sliceUpdatesPerLaunch() * steps
+    # algorithmic HBM bytes: a sweep reads and writes each slice of a walker once, 8 DOF bytes per slice
+    bytes_launch = sum(k._nbrOfWalkers * k._N * 8.0 * int(k._system.DOF) for k in batch.kernels) * SWEEPS_PER_STEP
     acc = [k.getAcceptanceRate() for k in batch.kernels]
     per_type = [k._nbrOfWalkers * (k._N >> k._S) * ((1 << k._S) - 1) * SWEEPS_PER_STEP for k in batch.kernels]
     accepted = sum(u * a for u, a in zip(per_type, acc)) * steps
-    return {"metric": "path-slice updates/s, ragged batch in one persistent launch", "value": upd / wall,
+    out = {"metric": "path-slice updates/s, ragged batch in one persistent launch", "value": upd / wall,
             "unit": "slice-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
             "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic: seeded Gaussian blobs (sigma 3 A)",
@@ -272,8 +439,17 @@ def ragged_bench(steps, warmup):
                                    "step, hipGraph replay",
                        "levels": levels, "autotune": best, "trials": trials,
                        "acceptance_by_level": [{str(S): round(a, 4) for S, a in sorted(t.items())} for t in table]},
+            "roofline": {"bound": "hbm", "achieved": bytes_launch / launch_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": bytes_launch / launch_s / 1e9 / HBM_PEAK_GBS, "traffic": measured_traffic(RAGGED_PROFILE),
+                         "kernel": "fs_batch_sweep_*", "avg_launch_ms": launch_s * 1e3,
+                         "bytes_per_launch": bytes_launch, "updates_per_launch": batch.sliceUpdatesPerLaunch()},
+            "register_classes": {"kernel_of_type": batch.kernelOfType, "waves_of_kernel": batch.kernelWaves},
             "accepted_slice_updates_per_s": accepted / wall,
             "acceptance": [round(a, 4) for a in acc]}
+    if cpu and int(os.environ.get("RANK", "0")) == 0:
+        progress("CPU baseline, ragged types")
+        out["cpu_baseline"] = cpu_baseline_ragged(kas)
+    return out
 
 
 # ---- anharmonic oscillator: the correlator and RNG stress workload ----
@@ -326,29 +502,7 @@ def timed_launches(k, steps, warmup):
     return float(np.mean([a.elapsed_time(b) for a, b in ev])), wall
 
 
-def aho_ess(paths, S, measurements, seed):
-    """ESS/s of E0 with one measurement every 32 sweeps (no correlator), after 512 sweeps at this S."""
-    import torch
-    from feynsimul_amd.stats import effective_sample_size
-    _, k = make_aho_kernel(S, SWEEPS_PER_STEP, 16, False, seed=seed)
-    k.setPaths(paths)
-    k.run()
-    k.setSeriesCapacity(measurements)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(measurements // 16):
-        k.run(wait=False)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    r = effective_sample_size(k.getSeries()[:, :, 0])
-    ess_cm, se_cm = chain_mean_ess(k.getSeries()[:, :, 0])
-    return k, {"S": S, "value": ess_cm / dt, "unit": "effective samples/s (chain-mean estimator)",
-               "ess_sokal_per_s": r["ess"] / dt, "tau_int_measurements": r["tau_int"], "E0": r["mean"],
-               "sigma": se_cm, "seconds": dt, "sweeps": measurements * SWEEPS_PER_STEP,
-               "acceptance": k.getAcceptanceRate()}
-
-
-def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_train=200, cpu=True, ess=True):
+def aho_bench(steps, warmup, cpu=True, ess=True):
     """Anharmonic oscillator x^2/2 + 0.1 x^4, N = 4096, beta = 40, 65536 chains, S = 4: one step = 32
     sweeps + one measurement of the virial energy and the two-time correlator <x(0) x(tau)>,
     tau = 0..511 (FP64 Kahan bins); xorshift128 and RANLUX luxury 2; estimators against the FP64
@@ -391,9 +545,8 @@ def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_trai
                          "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(AHO_TRAFFIC_PROFILE),
                          "kernel": "fs_wave_sweep", "avg_launch_ms": ms, "bytes_per_update": AHO_BYTES_PER_UPDATE,
                          "updates_per_launch": updates},
-            "valu": {"ops_per_update": AHO_VALU_PER_UPDATE, "achieved": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3),
-                     "peak": VALU_PEAK, "frac": AHO_VALU_PER_UPDATE * updates / (ms * 1e-3) / VALU_PEAK,
-                     "issue_measured": measured_valu_issue(AHO_TRAFFIC_PROFILE)},
+            "valu": dict(valu_fields(AHO_VALU_PER_UPDATE * updates / (ms * 1e-3), AHO_TRAFFIC_PROFILE),
+                         ops_per_update=AHO_VALU_PER_UPDATE),
             "acceptance": k.getAcceptanceRate(),
             "oracle": {"kind": "FP64 transfer matrix of the discretised action (tests/golden/aho_c5.json)",
                        "E_virial": E, "E_virial_se": Ese, "E_virial_exact": gold["E_virial"],
@@ -415,95 +568,81 @@ def aho_bench(steps, warmup, learned=True, learned_measurements=24, learned_trai
         if cpu and int(os.environ.get("RANK", "0")) == 0:
             line["cpu_baseline"] = cpu_baseline_aho()
         return line
-    # ESS/s of E0, plain sampler at S = 4 and at S = 6
-    kp, plain = aho_ess(paths, AHO_S, 256, seed=7)
-    plain["E0_exact"] = gold["E_virial"]
-    plain["z"] = abs(plain["E0"] - gold["E_virial"]) / plain["sigma"]
-    start = kp.getPaths()
-    del kp
-    torch.cuda.empty_cache()
-    k6, plain6 = aho_ess(paths, 6, 256, seed=8)
-    plain6["z"] = abs(plain6["E0"] - gold["E_virial"]) / plain6["sigma"]
-    del k6
-    torch.cuda.empty_cache()
-    line["ess"] = plain
-    line["ess_S6"] = plain6
-    if learned:
-        from feynsimul_amd.learned import LearnedProposal
-        from feynsimul_amd.stats import effective_sample_size
-        _, kl = make_aho_kernel(AHO_S, SWEEPS_PER_STEP, 1, False, seed=9)
-        kl.setPaths(start)
-        lp = LearnedProposal(kl, batch=65536, lr=1e-3, seed=10)
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        lp.sweep(1)
-        ev0.record(kl._stream)
-        lp.sweep(4, wait=False)
-        ev1.record(kl._stream)
-        torch.cuda.synchronize()
-        ms_sweep = ev0.elapsed_time(ev1) / 4
-        ar0 = lp.lastAcceptanceOfQueued(4)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        first = lp.train(1)
-        last = lp.trainOnline(learned_train - 1, plainSweepsPerStep=2)
-        torch.cuda.synchronize()
-        t_train = time.perf_counter() - t0
-        kl.setSeriesCapacity(learned_measurements)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        lp.run(learned_measurements, SWEEPS_PER_STEP)
-        t_run = time.perf_counter() - t1
-        ser = kl.getSeries()[:, :, 0]
-        r = effective_sample_size(ser)
-        ess_cm, se_cm = chain_mean_ess(ser)
-        dt = t_train + t_run
-        flop_row = 2 * (3 * 128 + 2 * 128 * 128 + 128 * 2)
-        flop_upd = flop_row * (2 * (AHO_SEG - 1) - 1) / (AHO_SEG - 1)
-        line["learned"] = {"value": ess_cm / dt, "unit": "effective samples/s (chain-mean estimator, training "
-                                                           "time included)",
-                           "E0": r["mean"], "sigma": se_cm, "z": abs(r["mean"] - gold["E_virial"]) / se_cm,
-                           "tau_int_measurements": r["tau_int"], "seconds_train": t_train,
-                           "seconds_sampling": t_run, "adam_steps": learned_train, "nll_first": first,
-                           "nll_last": last, "acceptance_untrained": ar0, "acceptance_trained": lp.lastAcceptance,
-                           "ms_per_learned_sweep": ms_sweep,
-                           "ms_per_plain_sweep": plain["seconds"] / plain["sweeps"] * 1e3,
-                           "mfma": {"kernel": "fs_lp_sweep", "achieved": flop_upd * AHO_UPDATES_PER_SWEEP /
-                                    (ms_sweep * 1e-3), "peak": MFMA_BF16_PEAK, "unit": "FLOP/s",
-                                    "frac": flop_upd * AHO_UPDATES_PER_SWEEP / (ms_sweep * 1e-3) / MFMA_BF16_PEAK},
-                           "ess_per_s_ratio_learned_over_plain": (ess_cm / dt) / plain["value"],
-                           "ess_per_s_ratio_learned_over_plain_S6": (ess_cm / dt) / plain6["value"]}
-        del kl, lp
-        torch.cuda.empty_cache()
+    # ESS/s of E0 at fixed sigma, plain vs learned, at the configured S = 4 and at S = 6
+    progress("ESS/s at fixed sigma: S=4 and S=6")
+    line["ess_at_fixed_sigma"] = {"S4": fixed_sigma("aho", 2e-4, S=4), "S6": fixed_sigma("aho", 2e-4, S=6)}
     if cpu and int(os.environ.get("RANK", "0")) == 0:
         line["cpu_baseline"] = cpu_baseline_aho()
     return line
 
 
-def cpu_baseline_aho(target_s=12.0):
-    """Reference-order CPU restatement (oracle) of the anharmonic workload, bounded sample."""
+def cpu_thread_counts():
+    """Threads for the CPU baseline: the cores os.sched_getaffinity reports, and OMP_NUM_THREADS when it
+    names a different count (a CPU quota smaller than the affinity mask makes more threads slower)."""
+    aff = len(os.sched_getaffinity(0))
+    counts = [aff]
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and 0 < int(env) != aff:
+        counts.append(int(env))
+    return aff, counts
+
+
+def cpu_reference_rate(system, NN, beta, S_, ops, make_paths, updates_per_walker_sweep, describe, run_s=2.5,
+                       repeats=5):
+    """The oracle's reference-order sampler (kernel.c:514-837 restated in C, xorshift128, FP32) on this
+    host: built strict-IEEE and with -ffast-math (the reference builds its kernels with
+    -cl-fast-relaxed-math, kernel.py:372-382), each timed as the median of `repeats` runs of a sample
+    sized to about run_s seconds, at each thread count of cpu_thread_counts.  value = the best median."""
     import numpy as np
     from oracle import oracle as O
+    aff, counts = cpu_thread_counts()
+    variants = {}
+    best = None
+    for name, fast in (("strict", False), ("fast_math", True)):
+        lib = O.build(system, NN, beta, operators=ops, native=True, fast_math=fast)
+        for threads in counts:
+            walkers = max(threads * 4, 16)
+            rng = np.random.default_rng(1)
+            sweeps = 1
+            while True:  # calibration, also the warm-up
+                paths = np.ascontiguousarray(make_paths(rng, walkers).astype(np.float32))
+                cfg = O.make_cfg(NN, S_, int(system.DOF), walkers, beta, 1, sweeps, nOps=len(ops))
+                seeds = O.xorshift_rows(3, 0, cfg.nThreads + 1)
+                t0 = time.perf_counter()
+                O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=threads)
+                dt = time.perf_counter() - t0
+                if dt > run_s / 3 or sweeps >= 1 << 18:
+                    break
+                sweeps = max(sweeps * 2, int(sweeps * run_s / max(dt, 1e-3)))
+            times = []
+            for _ in range(repeats):
+                paths = np.ascontiguousarray(make_paths(rng, walkers).astype(np.float32))
+                seeds = O.xorshift_rows(3, 0, cfg.nThreads + 1)
+                t0 = time.perf_counter()
+                O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=threads)
+                times.append(time.perf_counter() - t0)
+            med = float(np.median(times))
+            rate = walkers * sweeps * updates_per_walker_sweep / med
+            key = "%s_%dt" % (name, threads)
+            variants[key] = {"value": rate, "threads": threads, "median_s": med, "runs_s": times,
+                             "chains": walkers, "sweeps": sweeps}
+            if best is None or rate > best[1]["value"]:
+                best = (key, variants[key])
+    key, b = best
+    return {"value": b["value"], "unit": "slice-updates/s", "cores": b["threads"], "kind": "port",
+            "affinity_cores": aff, "variant": key, "variants": variants,
+            "sample": "%s; median of %d runs of %d chains x %d sweeps; reference work-item order, FP32, "
+                      "xorshift128, gcc -O3 -march=native -fopenmp, strict and -ffast-math builds"
+                      % (describe, repeats, b["chains"], b["sweeps"])}
+
+
+def cpu_baseline_aho():
+    """Reference-order CPU restatement (oracle) of the anharmonic workload (no correlator)."""
     from feynsimul_amd.physical_systems import AnharmOsc
     system = AnharmOsc(AHO_LAM)
-    cores = min(len(os.sched_getaffinity(0)), 16)
-    lib = O.build(system, AHO_N, AHO_BETA, operators=(system.energyOp,), native=True)
-    rng = np.random.default_rng(1)
-    walkers = max(cores * 4, 16)
-    sweeps = 2
-    while True:
-        paths = np.ascontiguousarray(rng.uniform(-1, 1, (walkers, AHO_N)).astype(np.float32))
-        cfg = O.make_cfg(AHO_N, AHO_S, 1, walkers, AHO_BETA, 1, sweeps, nOps=1)
-        seeds = O.xorshift_rows(3, 0, cfg.nThreads + 1)
-        t0 = time.perf_counter()
-        O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=cores)
-        dt = time.perf_counter() - t0
-        if dt > target_s / 2 or sweeps >= 1 << 17:
-            break
-        sweeps = max(sweeps * 2, int(sweeps * target_s / max(dt, 1e-3)))
-    upd = walkers * (AHO_N // AHO_SEG) * (AHO_SEG - 1) * sweeps
-    return {"value": upd / dt, "unit": "slice-updates/s", "cores": cores, "kind": "port",
-            "sample": "%d chains x %d sweeps of the same workload (no correlator), reference work-item order, "
-                      "FP32, xorshift128, gcc -O3 -march=native -fopenmp, %.1f s" % (walkers, sweeps, dt)}
+    return cpu_reference_rate(system, AHO_N, AHO_BETA, AHO_S, (system.energyOp,),
+                              lambda rng, w: rng.uniform(-1, 1, (w, AHO_N)), (AHO_N // AHO_SEG) * (AHO_SEG - 1),
+                              "anharmonic oscillator N=4096, beta=40, S=4, no correlator")
 
 
 # ---- harmonic oscillator: the small plain-method workload ----
@@ -551,18 +690,6 @@ def ho_bench(steps, warmup, cpu=True):
     ms, wall = timed_launches(k, steps, warmup)
     updates = HO_UPDATES_PER_SWEEP * SWEEPS_PER_STEP
     bytes_upd = 8.0 * HO_SEG / (HO_SEG - 1)
-    ke = kernel(SWEEPS_PER_STEP, 64, 3)
-    ke.setPaths(k.getPaths())
-    ke.setSeriesCapacity(256)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(4):
-        ke.run(wait=False)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ser = ke.getSeries()[:, :, 0]
-    r = effective_sample_size(ser)
-    ess_cm, se_cm = chain_mean_ess(ser)
     line = {"metric": "path-slice updates/s (plain bisection sampler)", "value": updates * steps / wall,
             "unit": "slice-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
             "ms_per_step": wall / steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -575,40 +702,19 @@ def ho_bench(steps, warmup, cpu=True):
                          "bytes_per_update": bytes_upd, "updates_per_launch": updates,
                          "note": "the 4 MiB of paths stay in the caches; the VALU bound applies"},
             "acceptance": k.getAcceptanceRate(),
-            "ess": {"value": ess_cm / dt, "unit": "effective samples/s (chain-mean estimator)",
-                    "ess_sokal_per_s": r["ess"] / dt, "tau_int_measurements": r["tau_int"], "E0": r["mean"],
-                    "sigma": se_cm, "E0_exact": exact, "z": abs(r["mean"] - exact) / se_cm, "seconds": dt,
-                    "sweeps": 256 * SWEEPS_PER_STEP, "seconds_to_sigma_1e-4": dt * (se_cm / 1e-4) ** 2}}
+            "ess_at_fixed_sigma": fixed_sigma("ho", 1e-4)}
     if cpu and int(os.environ.get("RANK", "0")) == 0:
         line["cpu_baseline"] = cpu_baseline_ho()
     return line
 
 
-def cpu_baseline_ho(target_s=12.0):
-    """Reference-order CPU restatement (oracle) of the harmonic-oscillator workload, bounded sample."""
-    import numpy as np
-    from oracle import oracle as O
+def cpu_baseline_ho():
+    """Reference-order CPU restatement (oracle) of the harmonic-oscillator workload."""
     from feynsimul_amd.physical_systems import HarmOsc
     system = HarmOsc()
-    cores = min(len(os.sched_getaffinity(0)), 16)
-    lib = O.build(system, HO_N, HO_BETA, operators=(system.energyOp,), native=True)
-    rng = np.random.default_rng(1)
-    walkers = max(cores * 16, 64)
-    sweeps = 16
-    while True:
-        paths = np.ascontiguousarray(rng.uniform(-1, 1, (walkers, HO_N)).astype(np.float32))
-        cfg = O.make_cfg(HO_N, HO_S, 1, walkers, HO_BETA, 1, sweeps, nOps=1)
-        seeds = O.xorshift_rows(3, 0, cfg.nThreads + 1)
-        t0 = time.perf_counter()
-        O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=cores)
-        dt = time.perf_counter() - t0
-        if dt > target_s / 2 or sweeps >= 1 << 18:
-            break
-        sweeps = max(sweeps * 2, int(sweeps * target_s / max(dt, 1e-3)))
-    upd = walkers * (HO_N // HO_SEG) * (HO_SEG - 1) * sweeps
-    return {"value": upd / dt, "unit": "slice-updates/s", "cores": cores, "kind": "port",
-            "sample": "%d chains x %d sweeps of the same workload, reference work-item order, FP32, xorshift128, "
-                      "gcc -O3 -march=native -fopenmp, %.1f s" % (walkers, sweeps, dt)}
+    return cpu_reference_rate(system, HO_N, HO_BETA, HO_S, (system.energyOp,),
+                              lambda rng, w: rng.uniform(-1, 1, (w, HO_N)), (HO_N // HO_SEG) * (HO_SEG - 1),
+                              "harmonic oscillator N=256, beta=10, S=4")
 
 
 # ---- random streams: the reference's RNG benchmark ----
@@ -673,32 +779,40 @@ def rng_bench(streams=REF_RNG_THREADS, draws=None):
             "generators": rows_out}
 
 
-def cpu_baseline(target_s=12.0):
-    """Reference-order CPU restatement (oracle) on this host's cores, bounded sample."""
+def cpu_baseline_ragged(kas):
+    """Reference-order CPU restatement (oracle) of every ragged type (its own S), a shorter sample per
+    type; the batch's rate is its total slice-updates over the sum of the types' CPU times."""
     import numpy as np
-    from oracle import oracle as O
+    per, total_upd, total_s = [], 0.0, 0.0
+    for ka in kas:
+        system = ka.system
+        seg = 1 << ka.S
+        rng0 = np.random.default_rng(7)
+        r = cpu_reference_rate(system, ka.N, ka.beta, ka.S, (system.energyOp,),
+                               lambda rng, w, s_=system, n_=ka.N: np.repeat(
+                                   s_.initialPositions(w, rng0, side=3.5, sigma=3.0)[:, :, None], n_,
+                                   axis=2).reshape(w, s_.DOF * n_),
+                               (ka.N // seg) * (seg - 1), "", run_s=0.3, repeats=3)
+        upd = ka.nbrOfWalkers * (ka.N // seg) * (seg - 1) * SWEEPS_PER_STEP
+        total_upd += upd
+        total_s += upd / r["value"]
+        per.append({"atoms": system.nAtoms, "N": ka.N, "S": ka.S, "value": r["value"], "variant": r["variant"],
+                    "cores": r["cores"]})
+    aff, _ = cpu_thread_counts()
+    return {"value": total_upd / total_s, "unit": "slice-updates/s", "cores": max(p_["cores"] for p_ in per),
+            "affinity_cores": aff, "kind": "port", "per_type": per,
+            "sample": "each of the 16 types: median of 3 runs of about 0.3 s, reference work-item order, FP32, "
+                      "xorshift128, gcc -O3 -march=native -fopenmp, strict and -ffast-math builds (the faster "
+                      "kept per type); the batch rate is its slice-updates over the summed per-type CPU time"}
+
+
+def cpu_baseline():
+    """Reference-order CPU restatement (oracle) of the He3 workload on this host's cores."""
     from feynsimul_amd.physical_systems import Lm2m2Cluster
     system = Lm2m2Cluster(3)
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
-    lib = O.build(system, N, BETA, operators=(system.energyOp, system.meanPairDistanceOp), native=True)
-    rng = np.random.default_rng(1)
-    walkers = max(cores * 4, 16)
-    sweeps = 2
-    while True:
-        paths = np.ascontiguousarray(initial_paths(system, rng, walkers).astype(np.float32))
-        cfg = O.make_cfg(N, S, DOF, walkers, BETA, 1, sweeps, nOps=2)
-        seeds = O.xorshift_rows(3, 0, cfg.nThreads + 1)
-        t0 = time.perf_counter()
-        O.run_reference(lib, cfg, paths, seeds=seeds, nthreads=cores)
-        dt = time.perf_counter() - t0
-        if dt > target_s / 2 or sweeps >= 1 << 14:
-            break
-        sweeps = max(sweeps * 2, int(sweeps * target_s / max(dt, 1e-3)))
-    upd = walkers * (N // SEG) * (SEG - 1) * sweeps
-    return {"value": upd / dt, "unit": "slice-updates/s", "cores": cores, "kind": "port",
-            "sample": "%d chains x %d sweeps of the same workload, reference work-item order, FP32, "
-                      "gcc -O3 -march=native -fopenmp, %.1f s" % (walkers, sweeps, dt)}
+    return cpu_reference_rate(system, N, BETA, S, (system.energyOp, system.meanPairDistanceOp),
+                              lambda rng, w: initial_paths(system, rng, w), (N // SEG) * (SEG - 1),
+                              "He3 LM2M2 N=1024, beta=10/K, S=6, energy and mean pair distance")
 
 
 def main():
@@ -714,11 +828,18 @@ def main():
                          "N=4096, beta=40, 65536 chains, 512 correlator lags, xorshift128 and RANLUX; "
                          "rng: stream throughput of every generator, as the reference's RNG benchmark; "
                          "ho: harmonic oscillator N=256, beta=10, 4096 chains")
-    ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the ESS phase")
-    ap.add_argument("--ess-launches", type=int, default=64, help="ESS launches of 16 x 32 sweeps")
+    ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the E0 series")
+    ap.add_argument("--ess-launches", type=int, default=64, help="E0 series launches of 16 x 32 sweeps")
+    ap.add_argument("--aho-sigma", type=float, default=2e-4, help="standard error target of E0 on the anharmonic "
+                                                                   "oscillator in the fixed-sigma comparison")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal phase")
     ap.add_argument("--learned-train", type=int, default=200, help="online Adam steps before learned sampling")
-    ap.add_argument("--learned-measurements", type=int, default=64, help="learned measurements (32 sweeps each)")
+    ap.add_argument("--learned-measurements", type=int, default=32, help="learned measurements (32 sweeps each)")
+    ap.add_argument("--ragged-levels", default=None,
+                    help="comma-separated S of the 16 ragged types (skips the level autotuner)")
+    ap.add_argument("--valu-calibration", action="store_true",
+                    help="run only the xorshift128 stream kernel on 1,048,576 streams (3 launches), the VALU "
+                         "issue-rate reference for rocprofv3 SQ counters")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (5 times), to scale HBM counters")
     ap.add_argument("--storage", choices=["registers", "memory"], default=None,
@@ -739,8 +860,21 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5}), flush=True)
         return
+    if args.valu_calibration:
+        from feynsimul_amd import _native
+        L = _native.lib()
+        n, count = 1 << 20, 1 << 16
+        st = torch.zeros((n, 4), dtype=torch.int32, device="cuda")
+        chk = torch.zeros(n, dtype=torch.int32, device="cuda")
+        _native.check(L.fs_seed_xorshift(st.data_ptr(), n, 2026, 0, None))
+        for _ in range(3):
+            _native.check(L.fs_rng_bench(_native.FS_RNG_XORSHIFT, 0, 0, st.data_ptr(), chk.data_ptr(), n, count, None))
+        torch.cuda.synchronize()
+        print(json.dumps({"streams": n, "uniforms_per_stream": count, "launches": 3}), flush=True)
+        return
     if args.workload == "ragged":
-        print(json.dumps(ragged_bench(args.steps, args.warmup)), flush=True)
+        lv = [int(v) for v in args.ragged_levels.split(",")] if args.ragged_levels else None
+        print(json.dumps(ragged_bench(args.steps, args.warmup, cpu=not args.no_cpu, levels=lv)), flush=True)
         return
     if args.workload == "ho":
         print(json.dumps(ho_bench(args.steps, args.warmup, cpu=not args.no_cpu)), flush=True)
@@ -749,9 +883,20 @@ def main():
         print(json.dumps(rng_bench()), flush=True)
         return
     if args.workload == "aho":
-        print(json.dumps(aho_bench(args.steps, args.warmup, learned=not args.no_learned,
-                                   cpu=not args.no_cpu, ess=not args.no_ess)), flush=True)
+        print(json.dumps(aho_bench(args.steps, args.warmup, cpu=not args.no_cpu, ess=not args.no_ess)), flush=True)
         return
+    # untimed check against the FP64 CPU reference: 23 launches (736 sweeps) from the initial paths the
+    # reference run starts from, whatever --warmup and --steps are
+    progress("oracle leg: %d launches from the FP64 reference's initial paths" % ORACLE_LAUNCHES)
+    system, ko = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
+    ko.setPaths(initial_paths(system, np.random.default_rng(0)))
+    for _ in range(ORACLE_LAUNCHES):
+        ko.run(wait=False)
+    torch.cuda.synchronize()
+    oracle = he3_oracle(ko.getOperators(), ORACLE_LAUNCHES * SWEEPS_PER_STEP)
+    del ko
+    torch.cuda.empty_cache()
+    progress("timed: %d warmup + %d steps" % (args.warmup, args.steps))
     system, k = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
     rng = np.random.default_rng(0)
     k.setPaths(initial_paths(system, rng))
@@ -797,22 +942,30 @@ def main():
                      "traffic": measured_traffic() if args.storage in (None, "memory") else None,
                      "kernel": "fs_wave_sweep", "avg_launch_ms": kern_s * 1e3,
                      "bytes_per_update": BYTES_PER_UPDATE, "updates_per_launch": updates},
-        "valu": {"ops_per_update": VALU_PER_UPDATE, "achieved": VALU_PER_UPDATE * updates / kern_s,
-                 "peak": VALU_PEAK, "frac": VALU_PER_UPDATE * updates / kern_s / VALU_PEAK,
-                 "issue_measured": measured_valu_issue(),
-                 "note": "the kernel is VALU-issue bound (SQ counters in profiles/); HBM is the second bound"},
+        "valu": dict(valu_fields(VALU_PER_UPDATE * updates / kern_s, TRAFFIC_PROFILE), ops_per_update=VALU_PER_UPDATE),
         "acceptance": k.getAcceptanceRate(),
         "energy_K": E,
-        "oracle": he3_oracle(per_walker, (args.warmup + args.steps) * SWEEPS_PER_STEP),
+        "oracle": oracle,
     }
+    del k
+    torch.cuda.empty_cache()
     if not args.no_ess:
-        line["ess"] = ess_phase(k.getPaths(), args.storage, args.ess_burn, args.ess_launches)
+        progress("He3 E0 series (stationarity)")
+        line["ess"] = ess_phase(initial_paths(system, np.random.default_rng(0)), args.storage, args.ess_burn,
+                                args.ess_launches)
         if not args.no_learned:
-            line["learned"] = learned_phase(ess_phase.start_paths, args.learned_train, args.learned_measurements,
-                                            line["ess"])
+            progress("He3 learned proposal")
+            line["learned"] = learned_phase(ess_phase.start_paths, args.learned_train, args.learned_measurements)
+        progress("ESS/s at fixed sigma: harmonic oscillator")
+        fs = {"ho": fixed_sigma("ho", 1e-4)}
+        progress("ESS/s at fixed sigma: anharmonic oscillator, S=6")
+        fs["aho_S6"] = fixed_sigma("aho", args.aho_sigma, S=6)
+        line["ess_at_fixed_sigma"] = fs
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
+        progress("CPU baseline")
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
+    reap_children()
 
 
 if __name__ == "__main__":

@@ -9,7 +9,7 @@ import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfeynsimul.so")
 
-FS_ABI_VERSION = 3
+FS_ABI_VERSION = 4
 FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
 FS_RNG_XORSHIFT, FS_RNG_RANLUX = 0, 1
@@ -18,6 +18,7 @@ FS_POTCACHE_AUTO, FS_POTCACHE_ON, FS_POTCACHE_OFF = 0, 1, 2
 
 EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_path_storage",
            "fs_batch_create", "fs_batch_destroy", "fs_batch_path_storage", "fs_batch_build_log", "fs_batch_run",
+           "fs_storage_of",
            "fs_batch_run_noalloc",
            "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
            "fs_source", "fs_run", "fs_set_waves_per_simd", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
@@ -90,9 +91,11 @@ def lib():
     L.fs_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(vp)]
     L.fs_destroy.argtypes = [vp]
     L.fs_path_storage.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
-    L.fs_batch_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.POINTER(ctypes.c_char_p), i32, i32, ctypes.c_int,
+    L.fs_batch_create.argtypes = [ctypes.POINTER(FsConfig), ctypes.POINTER(ctypes.c_char_p), i32, i32,
+                                  ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.c_int,
                                   ctypes.POINTER(vp)]
     L.fs_batch_destroy.argtypes = [vp]
+    L.fs_storage_of.argtypes = [ctypes.POINTER(FsConfig), ctypes.POINTER(ctypes.c_int)]
     L.fs_batch_path_storage.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int)]
     L.fs_batch_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_batch_run.argtypes = [vp, ctypes.POINTER(FsBuffers), ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32, vp]
@@ -198,16 +201,27 @@ class Engine:
             pass
 
 
+def storage_of(cfg):
+    """The path storage a wave-sampler module for this configuration uses (FS_STORAGE_REGISTERS or _MEMORY)."""
+    v = ctypes.c_int(0)
+    check(lib().fs_storage_of(ctypes.byref(cfg), ctypes.byref(v)))
+    return v.value
+
+
 class Batch:
     """One compiled batched module (owner of the fs_batch handle)."""
 
-    def __init__(self, cfgs, system_srcs, waves_per_block=0, compile_only=False):
+    def __init__(self, cfgs, system_srcs, waves_per_block=0, compile_only=False, kernel_of_type=None,
+                 waves_of_kernel=None):
         n = len(cfgs)
         self._h = ctypes.c_void_p()
         arr = (FsConfig * n)(*cfgs)
         srcs = (ctypes.c_char_p * n)(*[x.encode() for x in system_srcs])
+        kot = None if kernel_of_type is None else (ctypes.c_int32 * n)(*[int(v) for v in kernel_of_type])
+        wok = None if waves_of_kernel is None else (ctypes.c_int32 * len(waves_of_kernel))(
+            *[int(v) for v in waves_of_kernel])
         L = lib()
-        rc = L.fs_batch_create(arr, srcs, n, int(waves_per_block), int(compile_only), ctypes.byref(self._h))
+        rc = L.fs_batch_create(arr, srcs, n, int(waves_per_block), kot, wok, int(compile_only), ctypes.byref(self._h))
         if rc != FS_OK:
             msg = L.fs_last_error().decode(errors="replace")
             if self._h.value:

@@ -26,7 +26,7 @@ def _cache_path():
 
 
 def config_key(kernelArgsList, device_name):
-    h = hashlib.sha1(device_name.encode())
+    h = hashlib.sha1(("v2:" + device_name).encode())  # v2: trials include the register-class split
     for ka in kernelArgsList:
         sysobj = ka.system
         h.update(repr((sysobj.DOF, sysobj.potential, getattr(sysobj, "userCode", ""), ka.nbrOfWalkers, ka.N, ka.S,
@@ -100,8 +100,11 @@ def autotune_kernel(kernelArgs, initialPaths=None, wavesPerBlock=(2, 4, 8), wave
 
 
 def autotune_batch(kernelArgsList, wavesPerBlock=(1, 2, 4), orders=("cost", "given"), launches=3,
-                   initialPaths=None, use_cache=True, verbose=False):
-    """Returns (best, trials): best = {"wavesPerBlock", "order", "ms_per_launch"}; every trial is timed
+                   initialPaths=None, use_cache=True, verbose=False,
+                   kernels=(("auto", None), ("auto", "fit"), ("one", None))):
+    """Returns (best, trials): best = {"wavesPerBlock", "order", "kernels", "kernelWaves", "ms_per_launch"}
+    (kernels, kernelWaves: the register-class split of BatchedPIMC and its occupancy targets); every
+    trial is timed
     over `launches` launches after one warm-up launch, starting from `initialPaths` (a list with one
     array per member, or None for the members' default paths)."""
     import torch
@@ -111,8 +114,11 @@ def autotune_batch(kernelArgsList, wavesPerBlock=(1, 2, 4), orders=("cost", "giv
     if hit is not None:
         return hit["best"], hit["trials"]
     trials = []
-    for wpb in wavesPerBlock:
-        b = BatchedPIMC(kernelArgsList, wavesPerBlock=wpb)
+    for wpb, (kern, kwaves) in [(w, kk) for kk in kernels for w in wavesPerBlock]:
+        b = BatchedPIMC(kernelArgsList, wavesPerBlock=wpb, kernels=kern, kernelWaves=kwaves)
+        if kern != "one" and kwaves is None and max(b.kernelOfType) == 0:
+            del b
+            continue  # the split gives one class: the same module as "one"
         for order in orders:
             if initialPaths is not None:
                 for k, p in zip(b.kernels, initialPaths):
@@ -125,9 +131,11 @@ def autotune_batch(kernelArgsList, wavesPerBlock=(1, 2, 4), orders=("cost", "giv
                 b.run(wait=False)
             torch.cuda.synchronize()
             ms = (time.perf_counter() - t0) / launches * 1e3
-            trials.append({"wavesPerBlock": wpb, "order": order, "ms_per_launch": ms})
+            trials.append({"wavesPerBlock": wpb, "order": order, "kernels": kern, "kernelWaves": kwaves,
+                           "ms_per_launch": ms})
             if verbose:
-                print("autotune: waves/workgroup %d, order %s: %.3f ms" % (wpb, order, ms), flush=True)
+                print("autotune: waves/workgroup %d, order %s, kernels %s/%s: %.3f ms" % (wpb, order, kern, kwaves,
+                                                                                        ms), flush=True)
         del b
     best = min(trials, key=lambda t: t["ms_per_launch"])
     if use_cache:

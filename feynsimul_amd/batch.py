@@ -12,6 +12,13 @@ keeps its own copy of the dispatch tables (buffer addresses and order) from
 the capture; ``replay`` refuses a graph whose buffers or order have changed
 since (``setSeriesCapacity``, ``setOrder``, a new RANLUX table ...).
 
+The members are split into register classes (path kept in registers or
+streamed, DOF <= 9 or larger) that run in separate persistent kernels,
+launched one after the other, so each class runs at its own occupancy
+instead of all at the largest type's (``kernels="auto"``; ``"one"`` keeps a
+single kernel, or give a class number per member); ``kernelWaves`` asks the
+compiler for an occupancy per class ("fit": ``class_waves``).
+
 Members are ordinary ``PIMCKernel`` objects for everything but running:
 ``batch.kernels[t].getOperators()``, ``setPaths``, ``setSeriesCapacity`` ...
 """
@@ -32,8 +39,41 @@ def walker_cost(kernel):
     return float(kernel._N) * (26.0 * system.DOF + len(system.potential) + len(getattr(system, "userCode", "")) / 8.0)
 
 
+def register_classes(kernels, mode="auto"):
+    """Sweep kernel of each member.  "auto": one class per (path storage, DOF <= 9) pair present --
+    members that keep their whole path in registers need far more of them, and the coordinates a
+    wavefront holds grow with DOF; "one": a single kernel; a list: the class of each member (numbered
+    0, 1, ... without gaps, at most 4)."""
+    if isinstance(mode, str):
+        if mode == "one":
+            return [0] * len(kernels)
+        if mode != "auto":
+            raise ValueError("kernels must be 'auto', 'one' or a list of class numbers")
+        keys = [(_native.storage_of(k._cfg), int(k._system.DOF) > 9) for k in kernels]
+        order = sorted(set(keys))
+        return [order.index(key) for key in keys]
+    classes = [int(c) for c in mode]
+    if len(classes) != len(kernels) or sorted(set(classes)) != list(range(max(classes) + 1)) or max(classes) > 3:
+        raise ValueError("kernels: one class number per member, 0, 1, ... without gaps, at most 4 classes")
+    return classes
+
+
+def class_waves(kernels, classes):
+    """An occupancy target per class for the compiler (wavefronts per SIMD): 5 for streamed paths of
+    DOF <= 9, 4 for larger DOF, none for paths kept in registers."""
+    out = []
+    for c in range(max(classes) + 1):
+        members = [k for k, cl in zip(kernels, classes) if cl == c]
+        if any(_native.storage_of(k._cfg) == _native.FS_STORAGE_REGISTERS for k in members):
+            out.append(0)
+        else:
+            out.append(5 if max(int(k._system.DOF) for k in members) <= 9 else 4)
+    return out
+
+
 class BatchedPIMC:
-    def __init__(self, kernelArgsList, wavesPerBlock=None, order="cost", gridBlocks=0):
+    def __init__(self, kernelArgsList, wavesPerBlock=None, order="cost", gridBlocks=0, kernels="auto",
+                 kernelWaves=None):
         if not kernelArgsList:
             raise ValueError("at least one KernelArgs is needed")
         self.kernels = [PIMCKernel(ka, _batch_member=True) for ka in kernelArgsList]
@@ -44,8 +84,12 @@ class BatchedPIMC:
             if (k._operatorRuns, k._metroStepsPerOperatorRun) != (k0._operatorRuns, k0._metroStepsPerOperatorRun):
                 raise Exception("member %d: every member must have the same operatorRuns and "
                                 "metroStepsPerOperatorRun" % t)
+        self.kernelOfType = register_classes(self.kernels, kernels)
+        if kernelWaves == "fit":
+            kernelWaves = class_waves(self.kernels, self.kernelOfType)
+        self.kernelWaves = kernelWaves
         self._batch = _native.Batch([k._cfg for k in self.kernels], [k._systemSource for k in self.kernels],
-                                    wavesPerBlock or 0)
+                                    wavesPerBlock or 0, kernel_of_type=self.kernelOfType, waves_of_kernel=kernelWaves)
         for t, k in enumerate(self.kernels):
             k._attach(self._batch.path_storage(t))
         self._torch = k0._torch

@@ -457,6 +457,14 @@ int fs_destroy(fs_engine *e)
     return FS_OK;
 }
 
+int fs_storage_of(const fs_config *cfg, int *storage)
+{
+    if (!cfg || !storage) return fail(FS_ERR_INVALID, "config or storage is NULL");
+    if (int rc = validate(cfg)) return rc;
+    *storage = cfg->sampler == FS_SAMPLER_WAVE && resident_storage(cfg) ? FS_STORAGE_REGISTERS : FS_STORAGE_MEMORY;
+    return FS_OK;
+}
+
 int fs_path_storage(const fs_engine *e, int *storage)
 {
     if (!e || !storage) return fail(FS_ERR_INVALID, "engine or storage is NULL");
@@ -573,21 +581,20 @@ int fs_eval_system(fs_engine *e, const void *x, void *pot, void *ops, int64_t n,
 // ---- batched systems ----
 }  // extern "C"
 
-// Device copy of one dispatch, in one allocation: the work counter, the
-// per-type kernel arguments, the dispatch prefix sums and the type of each
-// dispatch slot.  A batch keeps one for plain launches; every captured graph
-// owns its own, so later changes of buffers or order never reach a graph.
+// Device copy of one dispatch, in one allocation: the work counters (one per
+// sweep kernel, in the first 256 bytes), the per-type kernel arguments, and
+// per sweep kernel the prefix sums of its dispatch slots and the type of each
+// slot.  A batch keeps one for plain launches; every captured graph owns its
+// own, so later changes of buffers or order never reach a graph.
 struct fs_dispatch_layout {
-    size_t types, start, type_of, bytes;
+    int nt, nk;
+    constexpr fs_dispatch_layout(int types, int kernels) : nt(types), nk(kernels) {}
     static constexpr size_t up(size_t v) { return (v + 255) & ~(size_t)255; }
-    explicit constexpr fs_dispatch_layout(int nt)
-        : types(256),  // the counter sits in the first 256 bytes
-          start(up(256 + sizeof(fs_kargs) * (size_t)nt)),
-          type_of(up(up(256 + sizeof(fs_kargs) * (size_t)nt) + sizeof(long long) * (size_t)(nt + 1))),
-          bytes(up(up(up(256 + sizeof(fs_kargs) * (size_t)nt) + sizeof(long long) * (size_t)(nt + 1)) +
-                   sizeof(int) * (size_t)nt))
-    {
-    }
+    constexpr size_t types() const { return 256; }
+    constexpr size_t per_kernel() const { return up(sizeof(long long) * (size_t)(nt + 1) + sizeof(int) * (size_t)nt); }
+    constexpr size_t start(int j) const { return up(256 + sizeof(fs_kargs) * (size_t)nt) + per_kernel() * (size_t)j; }
+    constexpr size_t type_of(int j) const { return start(j) + sizeof(long long) * (size_t)(nt + 1); }
+    constexpr size_t bytes() const { return start(nk); }
 };
 
 // Dispatch tables of fs_batch_run_noalloc: one fixed array per GPU, held
@@ -595,8 +602,10 @@ struct fs_dispatch_layout {
 // the device address that hipGetSymbolAddress returns for the current
 // device (scratch_address), never the array's name.
 constexpr int kBatchMaxTypes = 64;
+constexpr int kBatchMaxKernels = 4;
 constexpr size_t kScratchBytes = 16384;
-static_assert(fs_dispatch_layout(kBatchMaxTypes).bytes <= kScratchBytes, "scratch too small for 64 types");
+static_assert(fs_dispatch_layout(kBatchMaxTypes, kBatchMaxKernels).bytes() <= kScratchBytes,
+              "scratch too small for 64 types in 4 sweep kernels");
 static __device__ __align__(256) char scratch[kScratchBytes];
 
 struct fs_batch {
@@ -605,11 +614,15 @@ struct fs_batch {
     std::string log;
     std::vector<char> code;
     hipModule_t module = nullptr;
-    hipFunction_t fn_sweep = nullptr;
+    // one persistent sweep kernel per register class of types (kernel_of_type)
+    int nk = 1;
+    std::vector<int> kernel_of_type;
+    std::vector<int> types_in_kernel;
+    std::vector<hipFunction_t> fn_sweep;
+    std::vector<int> max_blocks;
     hipFunction_t fn_reset = nullptr;
     int wpb = 4;
     size_t smem = 0;
-    int max_blocks = 0;
     int device = -1;
     // dispatch tables of plain launches and the host image last uploaded
     char *d_tables = nullptr;
@@ -624,7 +637,8 @@ struct fs_graph {
 
 namespace {
 
-std::string batch_source(const std::vector<fs_config> &cfgs, const char *const *srcs, int wpb, size_t lds)
+std::string batch_source(const std::vector<fs_config> &cfgs, const std::vector<int> &kernel_of_type, int nk,
+                         const std::vector<int> &waves_of_kernel, const char *const *srcs, int wpb, size_t lds)
 {
     std::string s = "// generated by fs_batch_create\n";
     s += common_prelude(&cfgs[0]);
@@ -637,35 +651,49 @@ std::string batch_source(const std::vector<fs_config> &cfgs, const char *const *
         s += "\n#pragma clang force_cuda_host_device end\n" + wave_trait(&cfgs[k]) + "#undef eps\n}\n";
     }
     s += R"(
-extern "C" __global__ void fs_batch_reset(unsigned long long *counter) { *counter = 0ull; }
-
-// Persistent sweep: each wavefront takes walker numbers from the counter and
-// runs that walker's whole launch with its type's specialisation.
-extern "C" __global__ __launch_bounds__(64 * FS_WPB) void fs_batch_sweep(fs_batch_args b)
+extern "C" __global__ void fs_batch_reset(unsigned long long *counter, int n)
 {
-    const int lane = threadIdx.x & 63;
-    extern __shared__ char fs_smem[];
-    char *lds = fs_smem + (threadIdx.x >> 6) * FS_LDS_PER_WAVE;
-    const long long total = b.start[b.ntypes];
-    for (;;) {
-        unsigned long long item = 0;
-        if (lane == 0) item = atomicAdd(b.counter, 1ull);
-        item = __shfl(item, 0, 64);
-        if ((long long)item >= total) break;
-        int d = 0;
-        while (d + 1 < b.ntypes && (long long)item >= b.start[d + 1]) d++;
-        const int t = b.typeOf[d];
-        fs_kargs a = b.types[t];
-        a.operatorRuns = b.operatorRuns;
-        a.metroSteps = b.metroSteps;
-        a.seriesOffset = b.seriesOffset;
-        const long w = (long)((long long)item - b.start[d]);
+    if ((int)threadIdx.x < n) counter[threadIdx.x] = 0ull;
+}
+
+// Persistent sweep of one register class of types: each wavefront takes
+// walker numbers from the class's counter and runs that walker's whole launch
+// with its type's specialisation.
+#define FS_BATCH_LOOP_BEGIN                                                   \
+    const int lane = threadIdx.x & 63;                                        \
+    extern __shared__ char fs_smem[];                                         \
+    char *lds = fs_smem + (threadIdx.x >> 6) * FS_LDS_PER_WAVE;               \
+    const long long total = b.start[b.ntypes];                                \
+    for (;;) {                                                                \
+        unsigned long long item = 0;                                          \
+        if (lane == 0) item = atomicAdd(b.counter, 1ull);                     \
+        item = __shfl(item, 0, 64);                                           \
+        if ((long long)item >= total) break;                                  \
+        int d = 0;                                                            \
+        while (d + 1 < b.ntypes && (long long)item >= b.start[d + 1]) d++;    \
+        const int t = b.typeOf[d];                                            \
+        fs_kargs a = b.types[t];                                              \
+        a.operatorRuns = b.operatorRuns;                                      \
+        a.metroSteps = b.metroSteps;                                          \
+        a.seriesOffset = b.seriesOffset;                                      \
+        const long w = (long)((long long)item - b.start[d]);                  \
         switch (t) {
+#define FS_BATCH_LOOP_END \
+        default: break;   \
+        }                 \
+    }
 )";
-    for (size_t k = 0; k < cfgs.size(); k++)
-        s += "        case " + std::to_string(k) + ": fs_wave_walker<fs_ty" + std::to_string(k) +
-             "::sys>(a, w, lane, lds); break;\n";
-    s += "        default: break;\n        }\n    }\n}\n";
+    for (int j = 0; j < nk; j++) {
+        if (waves_of_kernel[j] > 0)
+            s += "__attribute__((amdgpu_waves_per_eu(" + std::to_string(waves_of_kernel[j]) + ")))\n";
+        s += "extern \"C\" __global__ __launch_bounds__(64 * FS_WPB) void fs_batch_sweep_" + std::to_string(j) +
+             "(fs_batch_args b)\n{\n    FS_BATCH_LOOP_BEGIN\n";
+        for (size_t k = 0; k < cfgs.size(); k++)
+            if (kernel_of_type[k] == j)
+                s += "        case " + std::to_string(k) + ": fs_wave_walker<fs_ty" + std::to_string(k) +
+                     "::sys>(a, w, lane, lds); break;\n";
+        s += "    FS_BATCH_LOOP_END\n}\n";
+    }
     return s;
 }
 
@@ -674,9 +702,9 @@ int batch_tables(const fs_batch *b, const fs_buffers *bufs, const int32_t *order
                  int32_t metro_steps, int32_t series_offset, std::vector<char> &image)
 {
     const int nt = (int)b->cfgs.size();
-    const fs_dispatch_layout L(nt);
-    image.assign(L.bytes, 0);
-    fs_kargs *types = reinterpret_cast<fs_kargs *>(image.data() + L.types);
+    const fs_dispatch_layout L(nt, b->nk);
+    image.assign(L.bytes(), 0);
+    fs_kargs *types = reinterpret_cast<fs_kargs *>(image.data() + L.types());
     for (int t = 0; t < nt; t++) {
         const fs_config &c = b->cfgs[t];
         const fs_buffers &x = bufs[t];
@@ -706,17 +734,28 @@ int batch_tables(const fs_batch *b, const fs_buffers *bufs, const int32_t *order
         a.seriesOffset = series_offset;
         memcpy(&types[t], &a, sizeof(a));
     }
-    int *type_of = reinterpret_cast<int *>(image.data() + L.type_of);
-    long long *start = reinterpret_cast<long long *>(image.data() + L.start);
     std::vector<char> seen(nt, 0);
+    std::vector<int> dispatch(nt);
     for (int d = 0; d < nt; d++) {
         const int t = order ? order[d] : d;
         if (t < 0 || t >= nt || seen[t]) return fail(FS_ERR_INVALID, "order must be a permutation of the types");
         seen[t] = 1;
-        type_of[d] = t;
+        dispatch[d] = t;
     }
-    start[0] = 0;
-    for (int d = 0; d < nt; d++) start[d + 1] = start[d] + b->cfgs[type_of[d]].n_walkers;
+    // each sweep kernel dispatches its own types in the given order
+    for (int j = 0; j < b->nk; j++) {
+        long long *start = reinterpret_cast<long long *>(image.data() + L.start(j));
+        int *type_of = reinterpret_cast<int *>(image.data() + L.type_of(j));
+        int n = 0;
+        start[0] = 0;
+        for (int d = 0; d < nt; d++) {
+            const int t = dispatch[d];
+            if (b->kernel_of_type[t] != j) continue;
+            type_of[n] = t;
+            start[n + 1] = start[n] + b->cfgs[t].n_walkers;
+            n++;
+        }
+    }
     return FS_OK;
 }
 
@@ -734,22 +773,26 @@ int batch_launch(fs_batch *b, char *tables, int32_t operator_runs, int32_t metro
                  int32_t grid_blocks, hipStream_t st)
 {
     const int nt = (int)b->cfgs.size();
-    const fs_dispatch_layout L(nt);
+    const fs_dispatch_layout L(nt, b->nk);
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(tables);
-    void *rargs[] = {&counter};
-    FS_HIP(hipModuleLaunchKernel(b->fn_reset, 1, 1, 1, 1, 1, 1, 0, st, rargs, nullptr));
-    fs_batch_args ba{};
-    ba.types = reinterpret_cast<const fs_kargs *>(tables + L.types);
-    ba.start = reinterpret_cast<const long long *>(tables + L.start);
-    ba.typeOf = reinterpret_cast<const int *>(tables + L.type_of);
-    ba.counter = counter;
-    ba.ntypes = nt;
-    ba.operatorRuns = operator_runs;
-    ba.metroSteps = metro_steps;
-    ba.seriesOffset = series_offset;
-    void *args[] = {&ba};
-    const unsigned grid = (unsigned)(grid_blocks > 0 ? grid_blocks : b->max_blocks);
-    FS_HIP(hipModuleLaunchKernel(b->fn_sweep, grid, 1, 1, 64 * b->wpb, 1, 1, (unsigned)b->smem, st, args, nullptr));
+    int nk = b->nk;
+    void *rargs[] = {&counter, &nk};
+    FS_HIP(hipModuleLaunchKernel(b->fn_reset, 1, 1, 1, 64, 1, 1, 0, st, rargs, nullptr));
+    for (int j = 0; j < b->nk; j++) {
+        fs_batch_args ba{};
+        ba.types = reinterpret_cast<const fs_kargs *>(tables + L.types());
+        ba.start = reinterpret_cast<const long long *>(tables + L.start(j));
+        ba.typeOf = reinterpret_cast<const int *>(tables + L.type_of(j));
+        ba.counter = counter + j;
+        ba.ntypes = b->types_in_kernel[j];
+        ba.operatorRuns = operator_runs;
+        ba.metroSteps = metro_steps;
+        ba.seriesOffset = series_offset;
+        void *args[] = {&ba};
+        const unsigned grid = (unsigned)(grid_blocks > 0 ? grid_blocks : b->max_blocks[j]);
+        FS_HIP(hipModuleLaunchKernel(b->fn_sweep[j], grid, 1, 1, 64 * b->wpb, 1, 1, (unsigned)b->smem, st, args,
+                                     nullptr));
+    }
     return FS_OK;
 }
 
@@ -766,11 +809,32 @@ void batch_free(fs_batch *b)
 extern "C" {
 
 int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32_t ntypes, int32_t waves_per_block,
-                    int compile_only, fs_batch **out)
+                    const int32_t *kernel_of_type, const int32_t *waves_of_kernel, int compile_only, fs_batch **out)
 {
     if (!out || !cfgs || !system_srcs) return fail(FS_ERR_INVALID, "cfgs, system_srcs and out are required");
     *out = nullptr;
-    if (ntypes < 1 || ntypes > 64) return fail(FS_ERR_INVALID, "ntypes must be in [1, 64]");
+    if (ntypes < 1 || ntypes > kBatchMaxTypes) return fail(FS_ERR_INVALID, "ntypes must be in [1, %d]", kBatchMaxTypes);
+    int nk = 1;
+    std::vector<int> kot(ntypes, 0), per(kBatchMaxKernels, 0);
+    if (kernel_of_type) {
+        for (int t = 0; t < ntypes; t++) {
+            if (kernel_of_type[t] < 0 || kernel_of_type[t] >= kBatchMaxKernels)
+                return fail(FS_ERR_INVALID, "kernel_of_type[%d] must be in [0, %d)", t, kBatchMaxKernels);
+            kot[t] = kernel_of_type[t];
+            nk = std::max(nk, kot[t] + 1);
+        }
+    }
+    for (int t = 0; t < ntypes; t++) per[kot[t]]++;
+    for (int j = 0; j < nk; j++)
+        if (!per[j]) return fail(FS_ERR_INVALID, "sweep kernel %d has no type (kernel_of_type must number its kernels "
+                                                 "0, 1, ... without gaps)", j);
+    std::vector<int> wok(nk, 0);
+    if (waves_of_kernel)
+        for (int j = 0; j < nk; j++) {
+            if (waves_of_kernel[j] < 0 || waves_of_kernel[j] > 8)
+                return fail(FS_ERR_INVALID, "waves_of_kernel[%d] must be in [0, 8]", j);
+            wok[j] = waves_of_kernel[j];
+        }
     for (int t = 0; t < ntypes; t++) {
         if (int rc = validate(&cfgs[t])) return rc;
         const fs_config &c = cfgs[t], &c0 = cfgs[0];
@@ -785,6 +849,9 @@ int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32
     if (!compile_only && ndev == 0) return fail(FS_ERR_NO_DEVICE, "no GPU visible");
     fs_batch *b = new fs_batch();
     b->cfgs.assign(cfgs, cfgs + ntypes);
+    b->nk = nk;
+    b->kernel_of_type = kot;
+    b->types_in_kernel.assign(per.begin(), per.begin() + nk);
     size_t lds = 0;
     for (auto &c : b->cfgs) lds = std::max(lds, wave_lds_bytes(&c));
     lds = (lds + 15) & ~(size_t)15;
@@ -796,7 +863,7 @@ int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32
     }
     b->wpb = wpb;
     b->smem = lds * wpb;
-    b->source = batch_source(b->cfgs, system_srcs, wpb, lds);
+    b->source = batch_source(b->cfgs, b->kernel_of_type, nk, wok, system_srcs, wpb, lds);
     hiprtcProgram prog;
     const char *hdrs[] = {kRngHeader, kArgsHeader, kWaveHeader, kStreamHeader, kLaneHeader};
     const char *names[] = {"fs_rng.h", "fs_abi_args.h", "fs_wave.h", "fs_stream.h", "fs_lane.h"};
@@ -828,19 +895,25 @@ int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32
     hiprtcDestroyProgram(&prog);
     if (!compile_only) {
         hipError_t e = hipModuleLoadData(&b->module, b->code.data());
-        if (e == hipSuccess) e = hipModuleGetFunction(&b->fn_sweep, b->module, "fs_batch_sweep");
         if (e == hipSuccess) e = hipModuleGetFunction(&b->fn_reset, b->module, "fs_batch_reset");
-        if (e == hipSuccess) e = hipMalloc(&b->d_tables, fs_dispatch_layout(ntypes).bytes);
-        int per_cu = 0, dev = 0, cus = 0;
-        if (e == hipSuccess)
-            e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn_sweep, 64 * wpb, b->smem);
+        if (e == hipSuccess) e = hipMalloc(&b->d_tables, fs_dispatch_layout(ntypes, nk).bytes());
+        int dev = 0, cus = 0;
         if (e == hipSuccess) e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        b->fn_sweep.assign(nk, nullptr);
+        b->max_blocks.assign(nk, 1);
+        for (int j = 0; j < nk && e == hipSuccess; j++) {
+            const std::string name = "fs_batch_sweep_" + std::to_string(j);
+            e = hipModuleGetFunction(&b->fn_sweep[j], b->module, name.c_str());
+            int per_cu = 0;
+            if (e == hipSuccess)
+                e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn_sweep[j], 64 * wpb, b->smem);
+            b->max_blocks[j] = std::max(1, per_cu) * std::max(1, cus);
+        }
         if (e != hipSuccess) {
             batch_free(b);
             return fail(FS_ERR_HIP, "batch module setup failed: %s", hipGetErrorString(e));
         }
-        b->max_blocks = std::max(1, per_cu) * std::max(1, cus);
         b->device = dev;
     }
     *out = b;
@@ -923,9 +996,10 @@ int fs_batch_run_noalloc(fs_batch *b, const fs_buffers *bufs, const int32_t *ord
     if (!b->module) return fail(FS_ERR_INVALID, "batch was created compile-only");
     if (operator_runs < 0 || metro_steps < 0) return fail(FS_ERR_INVALID, "negative run length");
     const int nt = (int)b->cfgs.size();
-    if (nt > kBatchMaxTypes || fs_dispatch_layout(nt).bytes > kScratchBytes)
-        return fail(FS_ERR_INVALID, "%d types need %zu bytes of dispatch tables; the static scratch holds %zu "
-                    "(at most %d types)", nt, fs_dispatch_layout(nt).bytes, kScratchBytes, kBatchMaxTypes);
+    if (nt > kBatchMaxTypes || b->nk > kBatchMaxKernels || fs_dispatch_layout(nt, b->nk).bytes() > kScratchBytes)
+        return fail(FS_ERR_INVALID, "%d types in %d sweep kernels need %zu bytes of dispatch tables; the static "
+                    "scratch holds %zu (at most %d types, %d kernels)", nt, b->nk,
+                    fs_dispatch_layout(nt, b->nk).bytes(), kScratchBytes, kBatchMaxTypes, kBatchMaxKernels);
     std::vector<char> image;
     if (int rc = batch_tables(b, bufs, order, operator_runs, metro_steps, series_offset, image)) return rc;
     char *tables = nullptr;

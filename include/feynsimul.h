@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FS_ABI_VERSION 3
+#define FS_ABI_VERSION 4
 
 enum {
     FS_OK = 0,
@@ -140,6 +140,10 @@ int fs_create(const fs_config *cfg, const char *system_src, int compile_only, fs
 int fs_destroy(fs_engine *e);
 /* The path storage the module was built with (FS_STORAGE_REGISTERS or _MEMORY). */
 int fs_path_storage(const fs_engine *e, int *storage);
+/* The path storage a module for this configuration would use, without building it (the batched
+ * scheduler groups types by it).  No reference counterpart: the reference always stages paths the
+ * way its flags say (kernel.c:582-603). */
+int fs_storage_of(const fs_config *cfg, int *storage);
 /* Copies the build log (NUL-terminated, truncated to len) and returns its full length in *needed. */
 int fs_build_log(const fs_engine *e, char *buf, size_t len, size_t *needed);
 /* Copies the generated module source (for verbose dumps). */
@@ -191,8 +195,15 @@ int fs_rng_bench(int32_t kind, int32_t lux, int32_t double_precision, void *stat
  * (NULL: as given); `grid_blocks` 0 = enough blocks to fill the device. */
 typedef struct fs_batch fs_batch;
 typedef struct fs_graph fs_graph;
+/* kernel_of_type[t] in [0, 4) puts type t into one of up to four persistent
+ * sweep kernels (numbered without gaps), so types of different register
+ * demand run at their own occupancy; the kernels run one after another on
+ * the launch stream.  NULL: one kernel for every type.  waves_of_kernel[j]
+ * in [0, 8] asks the compiler to fit kernel j's registers to that many
+ * wavefronts per SIMD (0 or NULL: its own choice). */
 int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32_t ntypes,
-                    int32_t waves_per_block, int compile_only, fs_batch **out);
+                    int32_t waves_per_block, const int32_t *kernel_of_type, const int32_t *waves_of_kernel,
+                    int compile_only, fs_batch **out);
 int fs_batch_destroy(fs_batch *b);
 int fs_batch_path_storage(const fs_batch *b, int32_t type, int *storage);
 int fs_batch_build_log(const fs_batch *b, char *buf, size_t len, size_t *needed);
