@@ -380,7 +380,7 @@ def fixed_sigma(workload, sigma, max_seconds=40.0, train_steps=100, S=None):
 RAGGED_PROFILE = os.path.join(ROOT, "profiles", "fs_batch_sweep_ragged.md")
 
 
-def ragged_bench(steps, warmup, cpu=True, levels=None):
+def ragged_bench(steps, warmup, cpu=True, levels=None, shape=None):
     """Batched ragged LM2M2 systems in one persistent launch (32 sweeps per launch), autotuned."""
     import numpy as np
     import torch
@@ -409,8 +409,11 @@ def ragged_bench(steps, warmup, cpu=True, levels=None):
         table = []
     for ka, S in zip(kas, levels):
         ka.S = S
-    progress("ragged: launch shape (waves per workgroup, order, register classes; cached per device)")
-    best, trials = autotune_batch(kas, initialPaths=starts, use_cache=True, verbose=True)
+    if shape is None:
+        progress("ragged: launch shape (waves per workgroup, order, register classes; cached per device)")
+        best, trials = autotune_batch(kas, initialPaths=starts, use_cache=True, verbose=True)
+    else:
+        best, trials = dict(shape, ms_per_launch=None), []
     batch = BatchedPIMC(kas, wavesPerBlock=best["wavesPerBlock"], order=best["order"], kernels=best["kernels"],
                         kernelWaves=best["kernelWaves"])
     for k, p in zip(batch.kernels, starts):
@@ -649,6 +652,7 @@ def cpu_baseline_aho():
 HO_N, HO_BETA, HO_W, HO_S = 256, 10.0, 4096, 4
 HO_SEG = 1 << HO_S
 HO_UPDATES_PER_SWEEP = HO_W * (HO_N // HO_SEG) * (HO_SEG - 1)
+HO_PROFILE = os.path.join(ROOT, "profiles", "fs_wave_sweep_ho_n256.md")
 
 
 def ho_exact_x2(N, beta):
@@ -660,7 +664,7 @@ def ho_exact_x2(N, beta):
     return float(np.mean(1.0 / a))
 
 
-def ho_bench(steps, warmup, cpu=True):
+def ho_bench(steps, warmup, cpu=True, ess=True):
     """1D harmonic oscillator, N = 256, beta = 10, 4096 chains, S = 4, xorshift128, paths uniform on [-1, 1],
     2000 burn-in sweeps, then steps of 32 sweeps + one measurement of the virial energy; ESS/s of E0 over
     256 measurements against the exact discretised value 0.499950082044."""
@@ -700,9 +704,11 @@ def ho_bench(steps, warmup, cpu=True):
                          "unit": "GB/s", "frac": bytes_upd * updates / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                          "traffic": None, "kernel": "fs_wave_sweep", "avg_launch_ms": ms,
                          "bytes_per_update": bytes_upd, "updates_per_launch": updates,
-                         "note": "the 4 MiB of paths stay in the caches; the VALU bound applies"},
-            "acceptance": k.getAcceptanceRate(),
-            "ess_at_fixed_sigma": fixed_sigma("ho", 1e-4)}
+                         "note": "the 4 MiB of paths stay in registers for the whole launch; HBM is not a bound"},
+            "valu": dict(valu_fields(32 * updates / (ms * 1e-3), HO_PROFILE), ops_per_update=32),
+            "acceptance": k.getAcceptanceRate()}
+    if ess:
+        line["ess_at_fixed_sigma"] = fixed_sigma("ho", 1e-4)
     if cpu and int(os.environ.get("RANK", "0")) == 0:
         line["cpu_baseline"] = cpu_baseline_ho()
     return line
@@ -837,6 +843,8 @@ def main():
     ap.add_argument("--learned-measurements", type=int, default=32, help="learned measurements (32 sweeps each)")
     ap.add_argument("--ragged-levels", default=None,
                     help="comma-separated S of the 16 ragged types (skips the level autotuner)")
+    ap.add_argument("--ragged-shape", default=None,
+                    help="waves per workgroup, order, kernels (e.g. 4,cost,one): skips the launch-shape autotuner")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="run only the xorshift128 stream kernel on 1,048,576 streams (3 launches), the VALU "
                          "issue-rate reference for rocprofv3 SQ counters")
@@ -874,10 +882,15 @@ def main():
         return
     if args.workload == "ragged":
         lv = [int(v) for v in args.ragged_levels.split(",")] if args.ragged_levels else None
-        print(json.dumps(ragged_bench(args.steps, args.warmup, cpu=not args.no_cpu, levels=lv)), flush=True)
+        shape = None
+        if args.ragged_shape:
+            wpb, order, kern = args.ragged_shape.split(",")
+            shape = {"wavesPerBlock": int(wpb), "order": order, "kernels": kern, "kernelWaves": None}
+        print(json.dumps(ragged_bench(args.steps, args.warmup, cpu=not args.no_cpu, levels=lv, shape=shape)),
+              flush=True)
         return
     if args.workload == "ho":
-        print(json.dumps(ho_bench(args.steps, args.warmup, cpu=not args.no_cpu)), flush=True)
+        print(json.dumps(ho_bench(args.steps, args.warmup, cpu=not args.no_cpu, ess=not args.no_ess)), flush=True)
         return
     if args.workload == "rng":
         print(json.dumps(rng_bench()), flush=True)

@@ -556,6 +556,41 @@ int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t met
     return FS_OK;
 }
 
+int fs_launch_info(const fs_engine *e, fs_launch_info_t *info)
+{
+    if (!e || !info) return fail(FS_ERR_INVALID, "engine and info are required");
+    if (!e->module) return fail(FS_ERR_INVALID, "engine was created compile-only");
+    const fs_config &c = e->cfg;
+    hipFunction_t fn = c.sampler == FS_SAMPLER_EXACT ? e->fn_exact : e->fn_wave;
+    fs_launch_info_t r{};
+    int v = 0;
+    FS_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, fn));
+    r.max_threads_per_block = v;
+    FS_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, fn));
+    r.lds_bytes = v;
+    FS_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, fn));
+    r.registers = v;
+    if (c.sampler == FS_SAMPLER_EXACT) {
+        r.threads_per_block = c.parallelize_path ? (c.N >> c.S) * c.walkers_per_group : 64;
+    } else {
+        r.threads_per_block = 64 * e->wpb;
+        r.lds_bytes += (int32_t)e->wave_smem;
+    }
+    int dev = e->device >= 0 ? e->device : 0;
+    FS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    r.lds_limit = v;
+    FS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxThreadsPerBlock, dev));
+    r.device_max_threads_per_block = v;
+    FS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxBlockDimX, dev));
+    r.device_max_block_dim[0] = v;
+    FS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxBlockDimY, dev));
+    r.device_max_block_dim[1] = v;
+    FS_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxBlockDimZ, dev));
+    r.device_max_block_dim[2] = v;
+    *info = r;
+    return FS_OK;
+}
+
 int fs_set_waves_per_simd(fs_engine *e, int32_t cap)
 {
     if (!e) return fail(FS_ERR_INVALID, "engine is NULL");

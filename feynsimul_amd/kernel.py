@@ -306,10 +306,27 @@ class PIMCKernel:
             return  # a BatchedPIMC compiles every member in one module, then calls _attach
         self._engine = _native.Engine(cfg, self._systemSource)
         if verbose:
-            with open("cleaned.hip", "w") as f:
-                f.write(self._engine.source)
-            print("Saved the generated module source in cleaned.hip")
+            self._dumpSource(self._engine.source)
         self._attach(self._engine.path_storage)
+
+    @staticmethod
+    def _dumpSource(source):
+        """The generated module source in cleaned.hip and, run through the C preprocessor with
+        comments and line markers dropped, in preprocessed.hip (kernel.py:351-364 writes cleaned.c and
+        preprocessed.c)."""
+        import subprocess
+        with open("cleaned.hip", "w") as f:
+            f.write(source)
+        dev = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "device")
+        try:
+            res = subprocess.run(["cpp", "-P", "-I", dev, "-D__HIPCC_RTC__"], input=source, capture_output=True,
+                                 text=True, timeout=60)
+            text = res.stdout if res.returncode == 0 else "// cpp failed:\n" + res.stderr
+        except (OSError, subprocess.SubprocessError) as e:
+            text = "// cpp could not run: %s\n" % e
+        with open("preprocessed.hip", "w") as f:
+            f.write("".join(line + "\n" for line in text.splitlines() if line.strip()))
+        print("Saved cleaned and preprocessed code in cleaned.hip and preprocessed.hip")
 
     def _attach(self, storage):
         """Allocates the device buffers (kernel.py:386-457) for the given path storage."""
@@ -607,16 +624,34 @@ class PIMCKernel:
             ret += "Number of workgroups (used): " + str(self._nbrOfWalkers // self._nbrOfWalkersPerWorkGroup)
         return ret
 
+    def _launchInfo(self):
+        if self._engine is None:
+            raise Exception("this kernel is a member of a BatchedPIMC; the batch owns its module")
+        return self._engine.launch_info()
+
     def getMaxKernelWGSize(self):
-        return 1024
+        """Largest work-group the compiled sampler kernel can run (kernel.py:732-734)."""
+        return int(self._launchInfo().max_threads_per_block)
 
     def exceedsLimits(self):
-        """True if this configuration exceeds the device's limits (kernel.py:736-775)."""
+        """True if this configuration exceeds the device's limits (kernel.py:736-775): global memory,
+        the kernel's LDS against the device's per-work-group LDS, the work-group size against the
+        device's and the kernel's limits, and each work-group dimension against the device's."""
         props = self._torch.cuda.get_device_properties(self._device)
         if self.getGlobalMemory() > props.total_memory:
             return True
-        if self._sampler == "exact" and self._enableParallelizePath:
-            if (self._N // 2 ** self._S) * self._nbrOfWalkersPerWorkGroup > 1024:
+        info = self._launchInfo()
+        if info.lds_bytes > info.lds_limit:
+            return True
+        if self._enableParallelizePath and self._sampler == "exact":
+            wg = (self._N // 2 ** self._S) * self._nbrOfWalkersPerWorkGroup
+            if wg > info.device_max_threads_per_block or wg > info.max_threads_per_block:
+                return True
+        if info.threads_per_block > min(info.device_max_threads_per_block, info.max_threads_per_block):
+            return True
+        local = self._localSize if (self._localSize and self._sampler == "exact") else (info.threads_per_block,)
+        for i, n in enumerate(local):
+            if n > info.device_max_block_dim[i]:
                 return True
         return False
 

@@ -156,6 +156,21 @@ int fs_source(const fs_engine *e, char *buf, size_t len, size_t *needed);
  * autotuner uses it. */
 int fs_set_waves_per_simd(fs_engine *e, int32_t cap);
 
+/* Launch shape and resource use of the module's sampler kernel against the
+ * device limits: what the reference's getMaxKernelWGSize and exceedsLimits
+ * read from OpenCL (kernel.py:732-775: WORK_GROUP_SIZE, LOCAL_MEM_SIZE,
+ * MAX_WORK_GROUP_SIZE, MAX_WORK_ITEM_SIZES). */
+typedef struct fs_launch_info_t {
+    int32_t threads_per_block;          /* work-items of one launch's work-group */
+    int32_t max_threads_per_block;      /* the kernel's limit (its WORK_GROUP_SIZE) */
+    int32_t lds_bytes;                  /* static + dynamic LDS of one work-group */
+    int32_t registers;                  /* VGPRs per work-item */
+    int32_t lds_limit;                  /* device LDS per work-group */
+    int32_t device_max_threads_per_block;
+    int32_t device_max_block_dim[3];
+} fs_launch_info_t;
+int fs_launch_info(const fs_engine *e, fs_launch_info_t *info);
+
 /* One launch of operator_runs x metro_steps sweeps (the reference `run`). */
 int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t metro_steps,
            int32_t series_offset, void *stream);

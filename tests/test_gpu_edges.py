@@ -108,3 +108,23 @@ def test_exact_edges_match_oracle(gpu, case):
     assert same_w.any()
     err = np.abs(k.getPaths()[same_w] - paths[same_w]).max()
     assert err < 2e-4, "path error %g" % err
+
+
+@pytest.mark.parametrize("sampler", ["exact", "wave"])
+def test_limits_and_verbose_dump(gpu, sampler, tmp_path, monkeypatch):
+    """getMaxKernelWGSize reads the compiled kernel's work-group limit, exceedsLimits checks the launch
+    against it and the device (kernel.py:732-775), and verbose=True writes the generated source and its
+    preprocessed form (kernel.py:351-364)."""
+    _case("limits and verbose dump, " + sampler)
+    monkeypatch.chdir(tmp_path)
+    system = HarmOsc()
+    ka = _ka(system, 8, 1024, 1, 10.0, sampler)   # exact: 512 work-items per work-group
+    k = PIMCKernel(ka, verbose=True)
+    info = k._launchInfo()
+    assert k.getMaxKernelWGSize() == info.max_threads_per_block >= info.threads_per_block
+    assert info.threads_per_block == (512 if sampler == "exact" else 64 * (info.threads_per_block // 64))
+    assert 0 < info.lds_limit and info.lds_bytes <= info.lds_limit
+    assert k.exceedsLimits() is False
+    assert (tmp_path / "cleaned.hip").read_text() == k._engine.source
+    pre = (tmp_path / "preprocessed.hip").read_text()
+    assert "potential" in pre and "#define" not in pre
