@@ -112,7 +112,7 @@ def reap_children():
     psutil.wait_procs(kids, timeout=5)
 
 
-def make_kernel(steps_per_launch, seed, storage=None, operator_runs=1):
+def make_kernel(steps_per_launch, seed, storage=None, operator_runs=1, ranlux=False, waves_per_simd=None):
     from feynsimul_amd.kernel_args import KernelArgs
     from feynsimul_amd.kernel import PIMCKernel
     from feynsimul_amd.physical_systems import Lm2m2Cluster
@@ -126,8 +126,14 @@ def make_kernel(steps_per_launch, seed, storage=None, operator_runs=1):
                     operators=(system.energyOp, system.meanPairDistanceOp), seed=seed, sampler="wave",
                     pathStorage=storage,
                     histogramObservables=tuple(system.pairDistance(i, j) for i, j in system.pairs),
-                    histogramRange=(0.0, HIST_MAX_A), histogramBins=256)
-    return system, PIMCKernel(ka)
+                    histogramRange=(0.0, HIST_MAX_A), histogramBins=256,
+                    enableRanlux=ranlux, luxuaryFactor=1 if ranlux else None,
+                    ranluxIntMax=2 ** 31 - 1 if ranlux else None, wavesPerSimd=waves_per_simd)
+    k = PIMCKernel(ka)
+    if ranlux:
+        k.initRanlux()
+        k.groupInitRanlux()
+    return system, k
 
 
 def initial_paths(system, rng, walkers=W, sigma=3.0):
@@ -262,6 +268,69 @@ def learned_phase(paths, train_steps, measurements):
             "mfma": {"bound": "mfma", "kernel": "fs_lp_sweep", "achieved": flop / (ms_sweep * 1e-3),
                      "peak": MFMA_BF16_PEAK, "unit": "FLOP/s", "frac": flop / (ms_sweep * 1e-3) / MFMA_BF16_PEAK,
                      "flop_per_update": LP_FLOP_PER_UPDATE, "updates_per_launch": UPDATES_PER_SWEEP}}
+
+
+def hbm_ceiling(gib=2, repeats=7):
+    """Measured HBM ceilings: fs_hbm_probe streams of `gib` GiB (16-byte vectors, grid-stride) that copy,
+    only read or only write, at several grid sizes; median of `repeats` launches, best grid."""
+    import torch
+    from feynsimul_amd import _native
+    L = _native.lib()
+    nbytes = gib << 30
+    a = torch.zeros(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.zeros_like(a)
+    out = {}
+    for kind, name, moved in ((0, "copy", 2 * nbytes), (1, "read", nbytes), (2, "write", nbytes)):
+        best = None
+        for blocks in (1024, 2048, 4096, 8192, 16384):
+            t = []
+            for _ in range(repeats):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                _native.check(L.fs_hbm_probe(kind, a.data_ptr(), b.data_ptr(), nbytes // 16, blocks, None))
+                e1.record()
+                torch.cuda.synchronize()
+                t.append(e0.elapsed_time(e1))
+            ms = sorted(t)[repeats // 2]
+            gbs = moved / (ms * 1e-3) / 1e9
+            if best is None or gbs > best["GBps"]:
+                best = {"GBps": gbs, "blocks": blocks, "median_ms": ms}
+        out[name] = best
+    return {"metric": "HBM ceilings (fs_hbm_probe)", "bytes": nbytes, "ceilings": out, "peak_GBps": HBM_PEAK_GBS}
+
+
+def lp_bench(steps, warmup, spw=0):
+    """The learned sweep alone on the He3 workload (fs_lp_sweep): `steps` sweeps timed with HIP events on
+    the kernel's stream after `warmup`, from paths 512 plain sweeps into the run; a network with a
+    non-zero output layer (the trained case's cost)."""
+    import numpy as np
+    import torch
+    from feynsimul_amd.learned import LearnedProposal
+    system, k = make_kernel(512, seed=12345, storage="memory")
+    k.setPaths(initial_paths(system, np.random.default_rng(0)))
+    k.run()
+    lp = LearnedProposal(k, batch=65536, lr=1e-3, seed=779, segsPerWave=spw)
+    lp.train(4)
+    lp.sweep(max(warmup, 1))
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(k._stream)
+    lp.sweep(steps, wait=False)
+    ev1.record(k._stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    ar = lp.lastAcceptanceOfQueued(steps)
+    flop = LP_FLOP_PER_UPDATE * UPDATES_PER_SWEEP
+    return {"metric": "learned sweeps: slice-updates/s (fs_lp_sweep, BF16 MFMA MLP)",
+            "value": UPDATES_PER_SWEEP / (ms * 1e-3), "unit": "slice-updates/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16/f32", "data": "synthetic: He3 paths after 512 plain sweeps",
+            "config": {"workload": "LM2M2 He3, N=1024, 16384 chains, S=6, one learned sweep per step"},
+            "acceptance": ar,
+            "roofline": {"bound": "mfma", "achieved": flop / (ms * 1e-3) / 1e12, "peak": MFMA_BF16_PEAK / 1e12,
+                         "unit": "TFLOP/s", "frac": flop / (ms * 1e-3) / MFMA_BF16_PEAK, "kernel": "fs_lp_sweep",
+                         "avg_launch_ms": ms, "flop_per_update": LP_FLOP_PER_UPDATE,
+                         "updates_per_launch": UPDATES_PER_SWEEP, "traffic": None}}
 
 
 # ---- ESS/s of E0 at fixed sigma, learned vs plain, on the stationary workloads ----
@@ -828,12 +897,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the ESS/s phase")
-    ap.add_argument("--workload", choices=["he3", "ragged", "aho", "rng", "ho"], default="he3",
+    ap.add_argument("--workload", choices=["he3", "ragged", "aho", "rng", "ho", "lp"], default="he3",
                     help="he3: the headline workload; ragged: LM2M2 n in {3,4,6,8} x N in {256..2048}, "
                          "2048 chains each, in one autotuned persistent launch; aho: anharmonic oscillator "
                          "N=4096, beta=40, 65536 chains, 512 correlator lags, xorshift128 and RANLUX; "
                          "rng: stream throughput of every generator, as the reference's RNG benchmark; "
-                         "ho: harmonic oscillator N=256, beta=10, 4096 chains")
+                         "ho: harmonic oscillator N=256, beta=10, 4096 chains; lp: the learned sweep alone on He3")
     ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the E0 series")
     ap.add_argument("--ess-launches", type=int, default=64, help="E0 series launches of 16 x 32 sweeps")
     ap.add_argument("--aho-sigma", type=float, default=2e-4, help="standard error target of E0 on the anharmonic "
@@ -845,11 +914,17 @@ def main():
                     help="comma-separated S of the 16 ragged types (skips the level autotuner)")
     ap.add_argument("--ragged-shape", default=None,
                     help="waves per workgroup, order, kernels (e.g. 4,cost,one): skips the launch-shape autotuner")
+    ap.add_argument("--lp-spw", type=int, default=0, help="segments per wavefront of the learned sweep (0: automatic)")
+    ap.add_argument("--hbm-ceiling", action="store_true",
+                    help="measure the HBM ceilings (copy, read-only, write-only streams of 2 GiB) and exit")
     ap.add_argument("--valu-calibration", action="store_true",
                     help="run only the xorshift128 stream kernel on 1,048,576 streams (3 launches), the VALU "
                          "issue-rate reference for rocprofv3 SQ counters")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (5 times), to scale HBM counters")
+    ap.add_argument("--waves-per-simd", type=int, default=None,
+                    help="occupancy cap of the He3 launches (wavefronts per SIMD; default: the kernel's own)")
+    ap.add_argument("--ranlux", action="store_true", help="He3 workload with RANLUX luxury 2 streams instead of xorshift128")
     ap.add_argument("--storage", choices=["registers", "memory"], default=None,
                     help="where the wave sampler keeps each path during a launch (default: automatic)")
     args = ap.parse_args()
@@ -863,10 +938,19 @@ def main():
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
-        for _ in range(5):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(5)]
+        for a, b in ev:
+            a.record()
             dst.copy_(src)
+            b.record()
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5}), flush=True)
+        ms = sorted(a.elapsed_time(b) for a, b in ev)[2]
+        # a copy reads and writes each byte once
+        print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 5, "median_ms": ms,
+                          "hbm_GBps": 2 * src.numel() * 4 / (ms * 1e-3) / 1e9}), flush=True)
+        return
+    if args.hbm_ceiling:
+        print(json.dumps(hbm_ceiling()), flush=True)
         return
     if args.valu_calibration:
         from feynsimul_amd import _native
@@ -892,6 +976,9 @@ def main():
     if args.workload == "ho":
         print(json.dumps(ho_bench(args.steps, args.warmup, cpu=not args.no_cpu, ess=not args.no_ess)), flush=True)
         return
+    if args.workload == "lp":
+        print(json.dumps(lp_bench(args.steps, args.warmup, args.lp_spw)), flush=True)
+        return
     if args.workload == "rng":
         print(json.dumps(rng_bench()), flush=True)
         return
@@ -901,7 +988,8 @@ def main():
     # untimed check against the FP64 CPU reference: 23 launches (736 sweeps) from the initial paths the
     # reference run starts from, whatever --warmup and --steps are
     progress("oracle leg: %d launches from the FP64 reference's initial paths" % ORACLE_LAUNCHES)
-    system, ko = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
+    system, ko = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage, ranlux=args.ranlux,
+                             waves_per_simd=args.waves_per_simd)
     ko.setPaths(initial_paths(system, np.random.default_rng(0)))
     for _ in range(ORACLE_LAUNCHES):
         ko.run(wait=False)
@@ -910,7 +998,8 @@ def main():
     del ko
     torch.cuda.empty_cache()
     progress("timed: %d warmup + %d steps" % (args.warmup, args.steps))
-    system, k = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage)
+    system, k = make_kernel(SWEEPS_PER_STEP, seed=12345, storage=args.storage, ranlux=args.ranlux,
+                            waves_per_simd=args.waves_per_simd)
     rng = np.random.default_rng(0)
     k.setPaths(initial_paths(system, rng))
     for _ in range(args.warmup):
@@ -945,7 +1034,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded Gaussian blobs (sigma 3 A) around a random triangle of side 3.5 A",
-        "config": {"workload": "LM2M2 He3, 3D Cartesian, N=1024, beta=10/K, 16384 chains, S=6, xorshift128, "
+        "config": {"workload": "LM2M2 He3, 3D Cartesian, N=1024, beta=10/K, 16384 chains, S=6, %s, "
+                               % ("RANLUX luxury 2" if args.ranlux else "xorshift128") +
                                "32 sweeps + 1 measurement (energy, mean pair distance, 256-bin pair-distance "
                                "histogram) per step", "sampler": "wave",
                    "path_storage": args.storage or "auto",
@@ -962,6 +1052,14 @@ def main():
     }
     del k
     torch.cuda.empty_cache()
+    # the kernel's measured HBM traffic rate against the box's measured stream ceilings
+    progress("HBM ceilings")
+    ceil = hbm_ceiling()
+    line["hbm_ceiling"] = ceil
+    if line["roofline"]["traffic"]:
+        rate = line["roofline"]["traffic"] / kern_s / 1e9
+        line["roofline"]["traffic_GBps"] = rate
+        line["roofline"]["frac_of_measured_copy_ceiling"] = rate / ceil["ceilings"]["copy"]["GBps"]
     if not args.no_ess:
         progress("He3 E0 series (stationarity)")
         line["ess"] = ess_phase(initial_paths(system, np.random.default_rng(0)), args.storage, args.ess_burn,

@@ -18,7 +18,7 @@ FS_POTCACHE_AUTO, FS_POTCACHE_ON, FS_POTCACHE_OFF = 0, 1, 2
 
 EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "fs_destroy", "fs_path_storage",
            "fs_batch_create", "fs_batch_destroy", "fs_batch_path_storage", "fs_batch_build_log", "fs_batch_run",
-           "fs_storage_of", "fs_launch_info",
+           "fs_storage_of", "fs_launch_info", "fs_hbm_probe",
            "fs_batch_run_noalloc",
            "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
            "fs_source", "fs_run", "fs_set_waves_per_simd", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
@@ -97,6 +97,7 @@ def lib():
     L.fs_batch_destroy.argtypes = [vp]
     L.fs_storage_of.argtypes = [ctypes.POINTER(FsConfig), ctypes.POINTER(ctypes.c_int)]
     L.fs_launch_info.argtypes = [vp, ctypes.POINTER(FsLaunchInfo)]
+    L.fs_hbm_probe.argtypes = [i32, vp, vp, ctypes.c_int64, i32, vp]
     L.fs_batch_path_storage.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int)]
     L.fs_batch_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_batch_run.argtypes = [vp, ctypes.POINTER(FsBuffers), ctypes.POINTER(ctypes.c_int32), i32, i32, i32, i32, vp]

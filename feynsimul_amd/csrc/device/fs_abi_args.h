@@ -45,6 +45,16 @@ constexpr int fs_cor_stage_len(int N, int lags)
     return N <= FS_COR_FULL_N ? N : 2 * FS_COR_CH + 64 * fs_cor_lanelags(lags);
 }
 
+// LDS of one wave-sampler wavefront's measurements: correlator staging
+// (cor_bytes), then the private histogram of the scalar observables
+// (hist_bytes), then the FP64 running operator sums and their Kahan
+// compensations (nops x 16 bytes, 8-byte aligned), rounded up to 16 bytes.
+constexpr int fs_wave_ops_off(int cor_bytes, int hist_bytes) { return (cor_bytes + hist_bytes + 7) & ~7; }
+constexpr int fs_wave_meas_total(int cor_bytes, int hist_bytes, int nops)
+{
+    return (fs_wave_ops_off(cor_bytes, hist_bytes) + nops * 16 + 15) & ~15;
+}
+
 // Persistent batched launch over several system types (one wave per walker).
 // Walkers are numbered in dispatch order: type typeOf[d] owns numbers
 // [start[d], start[d + 1]); wavefronts take numbers from *counter until

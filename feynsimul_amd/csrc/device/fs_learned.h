@@ -282,20 +282,45 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
             // midpoint of the row's ends for this lane's coordinates d = 2q + g, kept
             // across the network instead of reading the ends again
             float mid[FS_LP_CB][8];
+            // what the row needs that does not depend on the network, computed before it
+            // so that it overlaps the MFMA chain: the proposal's normals and the old
+            // coordinates that the density of the old path needs
+            float zr[FS_LP_CB][8], xo[FS_LP_CB][8];
 #pragma unroll
             for (int cb = 0; cb < FS_LP_CB; cb++) {
                 const int R = t0 + cb * 32 + c32;
                 float f[32];
 #pragma unroll
-                for (int q = 0; q < 8; q++) mid[cb][q] = 0.0f;
+                for (int q = 0; q < 8; q++) {
+                    mid[cb][q] = 0.0f;
+                    zr[cb][q] = 0.0f;
+                    xo[cb][q] = 0.0f;
+                }
                 if (R < total) {
                     const bool old = R >= NR;
                     const int Rl = old ? R - NR : R, k = Rl >> (n - 1), m = (2 * (Rl & (nn - 1)) + 1) * h;
+                    const int s = slice_of(k, m);
+                    // the ends' loads are issued first; the proposal's normals (integer and
+                    // transcendental work that needs no memory) are computed while they are in flight
                     float ea[D], eb[D];
 #pragma unroll
                     for (int d = 0; d < D; d++) {
                         ea[d] = value(k, m - h, old, d);
                         eb[d] = value(k, m + h, old, d);
+                    }
+                    uint4 px = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int d = 2 * q + g;
+                        if (2 * q < D) {
+                            if (q % 2 == 0 && !old)
+                                px = fs_philox(make_uint4((fs_u32)s, wl, a.sweep, FS_LP_TAG_NORMAL | (q / 2)), a.key0,
+                                               a.key1);
+                            if (d < D) {
+                                if (!old) zr[cb][q] = fs_lp_normal(px, q % 2, g);
+                                if (old || n == 1) xo[cb][q] = path[d * N + s];
+                            }
+                        }
                     }
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
@@ -318,26 +343,22 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
                 const int R = t0 + cb * 32 + c32;
                 const bool valid = R < total, old = R >= NR;
                 const int Rl = old ? R - NR : R, k = Rl >> (n - 1), m = (2 * (Rl & (nn - 1)) + 1) * h;
-                const int s = slice_of(k, m);
                 float contrib = 0.0f;
                 if (valid) {
-                    uint4 px = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll
                     for (int q = 0; q < 8; q++) {
                         const int d = 2 * q + g;
-                        if (q % 2 == 0 && !old) px = fs_philox(make_uint4((fs_u32)s, wl, a.sweep, FS_LP_TAG_NORMAL | (q / 2)), a.key0, a.key1);
                         if (d < D) {
                             const float mu = mid[cb][q] + sig * out[cb][2 * q];
                             const float sv = fminf(fmaxf(out[cb][2 * q + 1], -FS_LP_SMAX), FS_LP_SMAX);
                             const float sd = sig * expf(sv);
-                            const float xo = path[d * N + s];
                             if (!old) {
-                                const float z = fs_lp_normal(px, q % 2, g);
+                                const float z = zr[cb][q];
                                 stage[(k * (SEG - 1) + m - 1) * D + d] = mu + sd * z;
                                 contrib -= -0.5f * z * z - sv;
                             }
                             if (old || n == 1) {
-                                const float zo = (xo - mu) / sd;
+                                const float zo = (xo[cb][q] - mu) / sd;
                                 contrib += -0.5f * zo * zo - sv;
                             }
                         }

@@ -33,13 +33,17 @@
 #include "fs_rng.h"
 #include "fs_stream.h"
 
-// Bytes of LDS one wavefront uses for a system: correlator staging, then a
-// private histogram of the scalar observables, rounded up to 16 bytes; then
-// the group stream's RANLUX state.
+// Bytes of LDS one wavefront uses for a system: correlator staging, a
+// private histogram of the scalar observables and the FP64 operator sums
+// (fs_wave_meas_total); then the group stream's RANLUX state.
+template <class Sys> constexpr int fs_wave_cor_bytes()
+{
+    return Sys::NCORR > 0 ? fs_cor_stage_len(Sys::N, Sys::CORLAGS) * (int)sizeof(FLOAT_TYPE) : 0;
+}
+template <class Sys> constexpr int fs_wave_hist_bytes() { return Sys::NHOBS > 0 ? Sys::HBINS * 4 : 0; }
 template <class Sys> constexpr int fs_wave_meas_bytes()
 {
-    return ((Sys::NCORR > 0 ? fs_cor_stage_len(Sys::N, Sys::CORLAGS) * (int)sizeof(FLOAT_TYPE) : 0) +
-            (Sys::NHOBS > 0 ? Sys::HBINS * 4 : 0) + 15) & ~15;
+    return fs_wave_meas_total(fs_wave_cor_bytes<Sys>(), fs_wave_hist_bytes<Sys>(), Sys::NOPS);
 }
 template <class Sys> constexpr int fs_wave_lds_bytes()
 {
@@ -143,9 +147,14 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
 #endif
 
     fs_u32 nacc = 0;
-    double opS[NOPS1], opC[NOPS1];
+    // the running FP64 operator sums and their Kahan compensations live in LDS
+    // (lane 0 updates them once per measurement), not in registers that would
+    // stay allocated through the whole sweep loop
+    double *opAcc = reinterpret_cast<double *>(lds + fs_wave_ops_off(fs_wave_cor_bytes<Sys>(), fs_wave_hist_bytes<Sys>()));
+    if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < NOPS1; k++) opS[k] = opC[k] = 0.0;
+        for (int k = 0; k < 2 * Sys::NOPS; k++) opAcc[k] = 0.0;
+    }
 
     F *cbuf = reinterpret_cast<F *>(lds);
     unsigned *hbuf = reinterpret_cast<unsigned *>(
@@ -402,10 +411,13 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
 #pragma unroll
                 for (int k = 0; k < Sys::NOPS; k++) {
                     const F m = fs_group_sum<B>(part[k]);
-                    const double y = (double)m - opC[k];
-                    const double t = opS[k] + y;
-                    opC[k] = (t - opS[k]) - y;
-                    opS[k] = t;
+                    if (lane == 0) {
+                        const double sk = opAcc[2 * k], ck = opAcc[2 * k + 1];
+                        const double y = (double)m - ck;
+                        const double t = sk + y;
+                        opAcc[2 * k + 1] = (t - sk) - y;
+                        opAcc[2 * k] = t;
+                    }
                     if (a.series != nullptr && lane == 0)
                         a.series[((long)(a.seriesOffset + i) * a.nWalkers + w) * Sys::NOPS + k] = (float)(m / (F)N);
                 }
@@ -539,10 +551,11 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
             double *opm = reinterpret_cast<double *>(a.opMeans);
 #pragma unroll
             for (int k = 0; k < Sys::NOPS; k++) {
-                opm[w * Sys::NOPS + k] = opS[k] / ((double)a.operatorRuns * (double)N);
+                const double sk = opAcc[2 * k], ck = opAcc[2 * k + 1];
+                opm[w * Sys::NOPS + k] = sk / ((double)a.operatorRuns * (double)N);
                 if (a.opTotals != nullptr) {
                     double *tk = a.opTotals + (w * Sys::NOPS + k) * 2;
-                    const double y = (opS[k] - opC[k]) - tk[1];
+                    const double y = (sk - ck) - tk[1];
                     const double t = tk[0] + y;
                     tk[1] = (t - tk[0]) - y;
                     tk[0] = t;

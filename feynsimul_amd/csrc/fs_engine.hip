@@ -258,11 +258,11 @@ std::string wave_trait(const fs_config *c)
 size_t wave_lds_bytes(const fs_config *c)
 {
     // the same figure as fs_wave_lds_bytes<Sys>() in device/fs_wave.h
-    const size_t fsize = c->double_precision ? 8 : 4;
+    const int fsize = c->double_precision ? 8 : 4;
     const int lags = c->cor_lags > 0 ? c->cor_lags : c->N / 2;
-    return (((c->n_corr > 0 ? (size_t)fs_cor_stage_len(c->N, lags) * fsize : 0) +
-             (c->n_hist_obs > 0 ? (size_t)c->hist_bins * 4 : 0) + 15) & ~(size_t)15) +
-           (c->ranlux ? sizeof(fs_ranlux) : 0);
+    const int cor = c->n_corr > 0 ? fs_cor_stage_len(c->N, lags) * fsize : 0;
+    const int hist = c->n_hist_obs > 0 ? c->hist_bins * 4 : 0;
+    return (size_t)fs_wave_meas_total(cor, hist, c->n_ops) + (c->ranlux ? sizeof(fs_ranlux) : 0);
 }
 
 std::vector<std::string> compile_options(const fs_config *c, const std::string &arch)
@@ -1160,6 +1160,18 @@ int fs_ranlux_draw(void *tab, void *out_f32, void *out_f64, int64_t n, int32_t n
                        reinterpret_cast<hipStream_t>(stream), static_cast<fs_ranlux *>(tab),
                        static_cast<float *>(out_f32), static_cast<double *>(out_f64), (long long)n, (int)nquad,
                        (int)lux);
+    FS_HIP(hipGetLastError());
+    return FS_OK;
+}
+
+int fs_hbm_probe(int32_t kind, const void *src, void *dst, int64_t n_vec16, int32_t blocks, void *stream)
+{
+    if (kind < 0 || kind > 2 || n_vec16 < 0 || blocks < 1 || (kind != 2 && !src) || !dst)
+        return fail(FS_ERR_INVALID, "bad bandwidth probe request");
+    if (int rc = need_device()) return rc;
+    if (n_vec16 == 0) return FS_OK;
+    hipLaunchKernelGGL(fs_k_hbm_probe, dim3((unsigned)blocks), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       static_cast<const float4 *>(src), static_cast<float4 *>(dst), (long long)n_vec16, (int)kind);
     FS_HIP(hipGetLastError());
     return FS_OK;
 }

@@ -150,3 +150,26 @@ extern "C" __global__ void fs_k_rl_bench(fs_ranlux *__restrict__ tab, fs_u32 *__
     if (dbl) fs_rl_bench_lux<double>(tab, check, i, count, lux);
     else fs_rl_bench_lux<float>(tab, check, i, count, lux);
 }
+
+// HBM ceilings the samplers' rooflines are read against: a grid-stride stream
+// of 16-byte vectors that copies (kind 0), only reads (kind 1, the sum of the
+// values is written only if it equals an impossible sentinel) or only writes
+// (kind 2) n vectors.
+extern "C" __global__ void fs_k_hbm_probe(const float4 *__restrict__ src, float4 *__restrict__ dst, long long n,
+                                          int kind)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (kind == 0) {
+        for (; i < n; i += stride) dst[i] = src[i];
+    } else if (kind == 1) {
+        float s = 0.0f;
+        for (; i < n; i += stride) {
+            const float4 v = src[i];
+            s += (v.x + v.y) + (v.z + v.w);
+        }
+        if (s == -1.2345e-38f) dst[0] = make_float4(s, s, s, s);
+    } else {
+        for (; i < n; i += stride) dst[i] = make_float4(1.0f, 2.0f, 3.0f, 4.0f);
+    }
+}

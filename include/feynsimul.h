@@ -201,6 +201,12 @@ enum { FS_RNG_XORSHIFT = 0, FS_RNG_RANLUX = 1 };
 int fs_rng_bench(int32_t kind, int32_t lux, int32_t double_precision, void *states, void *check, int64_t n,
                  int64_t count, void *stream);
 
+/* HBM ceilings, no reference counterpart: the measured bandwidths the
+ * samplers' rooflines are read against.  A grid-stride stream of n_vec16
+ * 16-byte vectors in `blocks` workgroups of 256 work-items that copies src to
+ * dst (kind 0), only reads src (kind 1) or only writes dst (kind 2). */
+int fs_hbm_probe(int32_t kind, const void *src, void *dst, int64_t n_vec16, int32_t blocks, void *stream);
+
 /* ---- batched systems: one persistent launch over several configurations ----
  * No reference counterpart as a call: the reference compiles one program per
  * (system, N, S) and its run loop relaunches them one after another
