@@ -457,6 +457,251 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
     if (lane == 0 && nacc) atomicAdd(a.accepts + w, nacc);
 }
 
+// ---- whole-path coarse move (FS_LP_COARSE) ----
+// One wavefront moves FS_LP_SPW walkers, one MH decision per path.  With L = FS_LPC_L slices per
+// fine interval and M = N / L, the coarse nodes are the slices c_j = r + j L (j = 0 .. M - 1,
+// node M = node 0) for a random anchor r.  Levels 1 .. LC (M = 2^LC) of the bisection of the whole
+// cyclic path (2^S = N) draw the new coarse nodes from the network, exactly as a segment's levels
+// are drawn; each interval between neighbouring new nodes is then filled with a free-particle
+// (Levy) bridge, built in closed form from a lane prefix sum of Gaussian increments of variance
+// eps: x(jL + i) = c'_j + (c'_{j+1} - c'_j) i / L + W_i - (i / L) W_L.  The bridge samples the
+// kinetic action of the fine links exactly given the nodes, so the Metropolis-Hastings ratio keeps
+// only the coarse part of the kinetic action:
+//   log alpha = -eps (V_new - V_old) - (Keff_new - Keff_old) + log q(old nodes) - log q(new nodes),
+//   Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps L).
+// Random numbers are counted by (node or lane and block, walker, sweep, purpose) as in the segment
+// move; the bridge's normals are drawn twice (to score the path, and to write it once accepted).
+#define FS_LP_TAG_FINE 0x500u
+#ifndef FS_LPC_L
+#define FS_LPC_L 64
+#endif
+template <class Sys> struct fs_lpc_geom {
+    static constexpr int N = Sys::N, S = Sys::S, D = Sys::DOF, L = FS_LPC_L, M = N / L, WPW = FS_LP_SPW;
+    static constexpr int NODES = WPW * M * D;                                // floats per node set
+    static constexpr int WAVE_BYTES = ((2 * NODES + 2 * WPW) * 4 + 15) / 16 * 16;
+    static_assert(L == 64, "coarse move: one lane per fine slice");
+    static_assert((1 << S) == N && M >= 2, "coarse move: N = 64 x 2^LC, LC >= 1");
+    static_assert(2 * D + 1 <= 32 && D <= 15, "learned proposal: DOF <= 15");
+};
+
+// Normal k (k = interval * D + d) of lane `lane` of one walker's bridge, from Philox block k / 4.
+FS_DEV float fs_lpc_fine_normal(uint4 &px, int k, int lane, fs_u32 wl, const fs_lp_args &a)
+{
+    if ((k & 3) == 0)
+        px = fs_philox(make_uint4((fs_u32)((k >> 2) * 64 + lane), wl, a.sweep, FS_LP_TAG_FINE), a.key0, a.key1);
+    return fs_lp_normal(px, (k >> 1) & 1, k & 1);
+}
+
+// The new fine slices of interval j on every lane (lane 63: node j + 1).
+template <int D>
+FS_DEV void fs_lpc_bridge(const float *cn, int j, int M, int lane, uint4 &px, fs_u32 wl, const fs_lp_args &a,
+                          float eps, float (&x)[D])
+{
+    const float fi = (float)(lane + 1) * (1.0f / (float)FS_LPC_L);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        float v = sqrtf(eps) * fs_lpc_fine_normal(px, j * D + d, lane, wl, a);
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const float t = __shfl_up(v, o, 64);
+            if (lane >= o) v += t;
+        }
+        const float tot = __shfl(v, 63, 64);
+        const float ca = cn[j * D + d], cb = cn[((j + 1) & (M - 1)) * D + d];
+        x[d] = lane == 63 ? cb : ca + (cb - ca) * fi + v - fi * tot;
+    }
+}
+
+template <class Sys>
+FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long w0, int lane)
+{
+    typedef fs_lpc_geom<Sys> G;
+    constexpr int N = G::N, S = G::S, D = G::D, L = G::L, M = G::M, WPW = G::WPW, LC = S - 6;
+    const int g = lane >> 5, c32 = lane & 31;
+    float *cnew = stage, *cold = stage + G::NODES, *lq = stage + 2 * G::NODES;
+    auto anchor = [&](long w) {
+        return (int)(fs_philox(make_uint4(0u, (fs_u32)w, a.sweep, FS_LP_TAG_OFFSET), a.key0, a.key1).x & (N - 1));
+    };
+    // old coarse nodes; the anchor (node 0) is also the first new node
+#pragma unroll 1
+    for (int k = 0; k < WPW; k++) {
+        const float *path = a.paths + (w0 + k) * (long)(N * D);
+        const int rk = anchor(w0 + k);
+        for (int i = lane; i < M * D; i += 64) {
+            const int j = i / D, d = i % D;
+            const float v = path[d * N + ((rk + j * L) & (N - 1))];
+            cold[k * M * D + i] = v;
+            if (j == 0) cnew[k * M * D + i] = v;
+        }
+    }
+    if (lane < 2 * WPW) lq[lane] = 0.0f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- coarse levels: network proposals and reverse rows, as in the segment move ----
+#pragma unroll 1
+    for (int n = 1; n <= LC; n++) {
+        const int h = M >> n, nn = 1 << (n - 1);
+        const int NR = WPW * nn, total = n == 1 ? NR : 2 * NR;
+        const float sig = sqrtf((float)N * (float)Sys::EPS / (2.0f * (float)(1 << n)));
+        const float level = (float)n / (float)S;
+#pragma unroll 1
+        for (int t0 = 0; t0 < total; t0 += 32 * FS_LP_CB) {
+            fs_bf8 xin[2][FS_LP_CB];
+            float mid[FS_LP_CB][8], zr[FS_LP_CB][8], xo[FS_LP_CB][8];
+#pragma unroll
+            for (int cb = 0; cb < FS_LP_CB; cb++) {
+                const int R = t0 + cb * 32 + c32;
+                float f[32];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    mid[cb][q] = 0.0f;
+                    zr[cb][q] = 0.0f;
+                    xo[cb][q] = 0.0f;
+                }
+                if (R < total) {
+                    const bool old = R >= NR;
+                    const int Rl = old ? R - NR : R, k = Rl >> (n - 1), j = (2 * (Rl & (nn - 1)) + 1) * h;
+                    const float *src = (old ? cold : cnew) + k * M * D;
+                    float ea[D], eb[D];
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        ea[d] = src[(j - h) * D + d];
+                        eb[d] = src[((j + h) & (M - 1)) * D + d];
+                    }
+                    constexpr int NB = (D + 3) / 4;
+                    uint4 pb[NB];
+                    if (!old) {
+#pragma unroll
+                        for (int b0 = 0; b0 < NB; b0 += 2) {
+                            const bool pair = b0 + 1 < NB;
+                            const uint4 x = fs_philox(make_uint4((fs_u32)j, (fs_u32)(w0 + k), a.sweep,
+                                                                 FS_LP_TAG_NORMAL | (fs_u32)(pair ? b0 + g : b0)),
+                                                      a.key0, a.key1);
+                            if (pair) {
+                                const auto sx = __builtin_amdgcn_permlane32_swap(x.x, x.x, false, false);
+                                const auto sy = __builtin_amdgcn_permlane32_swap(x.y, x.y, false, false);
+                                const auto sz = __builtin_amdgcn_permlane32_swap(x.z, x.z, false, false);
+                                const auto sw = __builtin_amdgcn_permlane32_swap(x.w, x.w, false, false);
+                                pb[b0] = make_uint4(sx[0], sy[0], sz[0], sw[0]);
+                                pb[b0 + 1] = make_uint4(sx[1], sy[1], sz[1], sw[1]);
+                            } else {
+                                pb[b0] = x;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int d = 2 * q + g;
+                        if (2 * q < D && d < D) {
+                            if (!old) zr[cb][q] = fs_lp_normal(pb[q / 2], q % 2, g);
+                            if (old || n == 1) xo[cb][q] = cold[k * M * D + j * D + d];
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const float m0 = 2 * q < D ? 0.5f * (ea[2 * q < D ? 2 * q : 0] + eb[2 * q < D ? 2 * q : 0]) : 0.0f;
+                        const float m1 =
+                            2 * q + 1 < D ? 0.5f * (ea[2 * q + 1 < D ? 2 * q + 1 : 0] + eb[2 * q + 1 < D ? 2 * q + 1 : 0]) : 0.0f;
+                        mid[cb][q] = g ? m1 : m0;
+                    }
+                    fs_lp_features<D>(ea, eb, level, a.inScale, f);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 32; i++) f[i] = 0.0f;
+                }
+                fs_lp_xin(f, g, xin[0][cb], xin[1][cb]);
+            }
+            fs_f16v out[FS_LP_CB];
+            fs_lp_mlp(img, xin, out, lane);
+#pragma unroll
+            for (int cb = 0; cb < FS_LP_CB; cb++) {
+                const int R = t0 + cb * 32 + c32;
+                const bool valid = R < total, old = R >= NR;
+                const int Rl = old ? R - NR : R, k = Rl >> (n - 1), j = (2 * (Rl & (nn - 1)) + 1) * h;
+                float contrib = 0.0f;
+                if (valid) {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const int d = 2 * q + g;
+                        if (d < D) {
+                            const float mu = mid[cb][q] + sig * out[cb][2 * q];
+                            const float sv = fminf(fmaxf(out[cb][2 * q + 1], -FS_LP_SMAX), FS_LP_SMAX);
+                            const float sd = sig * expf(sv);
+                            if (!old) {
+                                const float z = zr[cb][q];
+                                cnew[k * M * D + j * D + d] = mu + sd * z;
+                                contrib -= -0.5f * z * z - sv;
+                            }
+                            if (old || n == 1) {
+                                const float zo = (xo[cb][q] - mu) / sd;
+                                contrib += -0.5f * zo * zo - sv;
+                            }
+                        }
+                    }
+                }
+                contrib += __shfl_xor(contrib, 32, 64);
+                for (int o = 1; o < nn; o <<= 1) contrib += __shfl_xor(contrib, o, 64);
+                if (valid && g == 0 && (c32 & (nn - 1)) == 0) lq[(old ? WPW : 0) + k] += contrib;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+
+    // ---- fine levels (Levy bridge), the decision, and the write of an accepted path ----
+    const float eps = (float)Sys::EPS, inv2eL = 0.5f / ((float)Sys::EPS * (float)L);
+#pragma unroll 1
+    for (int k = 0; k < WPW; k++) {
+        const long w = w0 + k;
+        const fs_u32 wl = (fs_u32)w;
+        float *path = a.paths + w * (long)(N * D);
+        const float *cn = cnew + k * M * D, *co = cold + k * M * D;
+        const int rk = anchor(w);
+        float dv = 0.0f, dk = 0.0f;
+        uint4 px = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll 1
+        for (int j = 0; j < M; j++) {
+            float xn[D], xold[D];
+            fs_lpc_bridge<D>(cn, j, M, lane, px, wl, a, eps, xn);
+            const int s = (rk + j * L + lane + 1) & (N - 1);
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                xold[d] = path[d * N + s];
+                const float en = cn[((j + 1) & (M - 1)) * D + d] - cn[j * D + d];
+                const float eo = co[((j + 1) & (M - 1)) * D + d] - co[j * D + d];
+                dk += en * en - eo * eo;
+            }
+            dv += (float)Sys::V(xn) - (float)Sys::V(xold);
+        }
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) dv += __shfl_xor(dv, o, 64);
+        dk *= inv2eL;  // the same on every lane
+        const float lqk = lq[k] + lq[WPW + k];
+        const float logA = -dk - eps * dv + lqk;
+        if (a.trace && lane == 0) {
+            float *tr = a.trace + w * 4;
+            tr[0] = lqk;
+            tr[1] = dk;
+            tr[2] = dv;
+            tr[3] = logA;
+        }
+        const float u = fs_lp_u24(fs_philox(make_uint4(0u, wl, a.sweep, FS_LP_TAG_ACCEPT), a.key0, a.key1).x);
+        if (!(expf(logA) < u)) {
+            px = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll 1
+            for (int j = 0; j < M; j++) {
+                float xn[D];
+                fs_lpc_bridge<D>(cn, j, M, lane, px, wl, a, eps, xn);
+                const int s = (rk + j * L + lane + 1) & (N - 1);
+#pragma unroll
+                for (int d = 0; d < D; d++) path[d * N + s] = xn[d];
+            }
+            if (lane == 0) a.accepts[w] += 1u;
+        }
+    }
+}
+
 // ---- pack: FP32 master parameters (logical) -> inference image and training matrices ----
 // Logical parameter order: W1 [128][2D+1], b1 [128], W2 [128][128], b2, W3 [128][128], b3,
 // W4 [2D][128], b4 [2D].  Training matrices (BF16): W1b [128][32], W2b, W3b [128][128],
@@ -544,7 +789,12 @@ template <class Sys> FS_DEV void fs_lp_gather_row(const fs_lp_train_args &a, int
     const uint4 x = fs_philox(make_uint4((fs_u32)b, a.stepLo, a.stepHi, FS_LP_TAG_TRAIN), a.key0, a.key1);
     const long w = (long)(((unsigned long long)x.x * (unsigned long long)a.nWalkers) >> 32);
     const int s0 = (int)(x.y & (N - 1));
+#if FS_LP_COARSE
+    // the whole-path move learns only the coarse nodes, every FS_LPC_L slices
+    const int p = FS_LPC_L * (1 + (int)(((unsigned long long)x.z * (unsigned long long)(SEG / FS_LPC_L - 1)) >> 32));
+#else
     const int p = 1 + (int)(((unsigned long long)x.z * (unsigned long long)(SEG - 1)) >> 32);
+#endif
     const int h = p & -p;
     const int n = S - (31 - __builtin_clz((unsigned)h));
     const float *path = a.paths + w * (long)(N * D);
@@ -670,6 +920,25 @@ template <int D> FS_DEV void fs_lp_adam_body(const fs_lp_adam_args &a, int i)
 
 // ---- kernels of one module (instantiated for the trait `sys`) ----
 #ifndef FS_LP_NO_KERNELS
+#if FS_LP_COARSE
+// One launch = one whole-path coarse move of every walker (FS_LP_SPW walkers per wavefront).
+extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp_args a)
+{
+    typedef fs_lpc_geom<sys> G;
+    extern __shared__ __attribute__((aligned(16))) char fs_lp_smem[];
+    {
+        const uint4 *src = reinterpret_cast<const uint4 *>(a.img);
+        uint4 *dst = reinterpret_cast<uint4 *>(fs_lp_smem);
+        for (int i = threadIdx.x; i < FS_LP_IMG_BYTES / 16; i += blockDim.x) dst[i] = src[i];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float *stage = reinterpret_cast<float *>(fs_lp_smem + FS_LP_IMG_BYTES + wave * G::WAVE_BYTES);
+    for (long long u = (long long)blockIdx.x * FS_LP_WAVES + wave; u < a.nUnits;
+         u += (long long)gridDim.x * FS_LP_WAVES)
+        fs_lpc_unit<sys>(a, fs_lp_smem, stage, (long)u * G::WPW, lane);
+}
+#else
 extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp_args a)
 {
     typedef fs_lp_geom<sys> G;
@@ -686,6 +955,7 @@ extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp
          u += (long long)gridDim.x * FS_LP_WAVES)
         fs_lp_unit<sys>(a, fs_lp_smem, stage, (long)(u / G::UPW), (int)(u % G::UPW) * G::SPW, lane);
 }
+#endif
 
 // The network alone on given feature rows (FP32 [rows][32]) -> logical outputs [rows][32].
 extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_eval(const char *img_g, const float *X,

@@ -40,6 +40,10 @@ class LearnedProposal:
 
     kernel:   a PIMCKernel (FP32, any sampler; its paths are updated in place)
     batch:    training rows (segment midpoints) per Adam step, multiple of 2048
+    coarse:   the whole-path move instead of segment sweeps (N = 64 x 2^k): the network proposes
+              the nodes every 64 slices of the whole cyclic path (bisection levels 1..k), a Levy
+              bridge fills the slices between them, one Metropolis-Hastings decision per path;
+              training uses those nodes only
     lr:       Adam learning rate (beta1 0.9, beta2 0.999, eps 1e-8)
     seed:     Philox key of the proposal's random numbers (default: the kernel's seed + 1)
     atoms3d:  centre the inputs on the atoms' mean (default: system.translationInvariant)
@@ -47,7 +51,7 @@ class LearnedProposal:
     """
 
     def __init__(self, kernel, batch=65536, lr=1e-3, seed=None, atoms3d=None, inScale=1.0, segsPerWave=0,
-                 initSeed=0, fallbackAcceptance=0.2):
+                 initSeed=0, fallbackAcceptance=0.2, coarse=False):
         if kernel._enableDouble:
             raise Exception("the learned proposal runs on FP32 paths")
         if kernel._engine is None:
@@ -68,6 +72,10 @@ class LearnedProposal:
         lp.in_scale = float(inScale)
         lp.lr, lp.beta1, lp.beta2, lp.adam_eps = float(lr), 0.9, 0.999, 1e-8
         lp.seed = int(seed) & ((1 << 64) - 1)
+        lp.coarse = int(bool(coarse))
+        self.coarse = bool(coarse)
+        # moves proposed per sweep: every segment, or (whole-path move) every path
+        self._movesPerSweep = kernel._nbrOfWalkers * (1 if coarse else kernel._N >> kernel._S)
         self._P = _native.Proposal(kernel._cfg, kernel._systemSource, lp)
         info = self._P.info
         z = lambda n, dt: t.zeros((int(n),), dtype=dt, device=kernel._device)
@@ -124,7 +132,8 @@ class LearnedProposal:
 
     # ---- sampling ----
     def sweep(self, nSweeps=1, wait=True):
-        """nSweeps learned sweeps; returns their acceptance rate (segments accepted / proposed)."""
+        """nSweeps learned sweeps; returns their acceptance rate (segments, or paths for the
+        whole-path move, accepted / proposed)."""
         k = self._k
         self._acc.zero_()
         self._P.sweep(k._paths.data_ptr(), self._acc.data_ptr(), self._bufs(), self._sweepCount, int(nSweeps),
@@ -132,15 +141,14 @@ class LearnedProposal:
         self._sweepCount += int(nSweeps)
         if not wait:
             return None
-        segs = k._nbrOfWalkers * (k._N >> k._S) * int(nSweeps)
-        self.lastAcceptance = float(self._acc.sum().item()) / segs
+        self.lastAcceptance = float(self._acc.sum().item()) / (self._movesPerSweep * int(nSweeps))
         return self.lastAcceptance
 
     def lastAcceptanceOfQueued(self, nSweeps):
         """Acceptance rate of the last sweep(nSweeps, wait=False) call, after it has finished."""
         k = self._k
         k._stream.synchronize()
-        self.lastAcceptance = float(self._acc.sum().item()) / (k._nbrOfWalkers * (k._N >> k._S) * int(nSweeps))
+        self.lastAcceptance = float(self._acc.sum().item()) / (self._movesPerSweep * int(nSweeps))
         return self.lastAcceptance
 
     def run(self, operatorRuns=None, sweepsPerMeasurement=None):
@@ -184,7 +192,8 @@ class LearnedProposal:
     def traceSweeps(self, on=True):
         """Keep (log q ratio, dK, dV, log alpha) of every segment of the last sweep (test support)."""
         k = self._k
-        self._trace = (k._torch.zeros((k._nbrOfWalkers, k._N >> k._S, 4), dtype=k._torch.float32,
+        per = 1 if self.coarse else k._N >> k._S
+        self._trace = (k._torch.zeros((k._nbrOfWalkers, per, 4), dtype=k._torch.float32,
                                       device=k._device) if on else None)
 
     def getTrace(self):

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FS_ABI_VERSION 4
+#define FS_ABI_VERSION 5
 
 enum {
     FS_OK = 0,
@@ -254,7 +254,8 @@ int fs_graph_destroy(fs_graph *g);
  * kernel.c:338-360 with a conditional Gaussian from an MLP
  * (2 DOF + 1 -> 128 -> 128 -> 128 -> 2 DOF) and extends the accept test of
  * kernel.c:376-377 with the proposal density ratio, so any weights keep
- * detailed balance.  FP32 paths only, 1 <= S <= 6, DOF <= 15.  Random numbers
+ * detailed balance.  FP32 paths only, 1 <= S <= 6 (or the whole-path move of
+ * fs_lp_config.coarse), DOF <= 15.  Random numbers
  * come from Philox4x32-10 keyed by `seed`.  The caller allocates the buffers
  * with the sizes of fs_lp_info. */
 typedef struct fs_lp fs_lp;
@@ -266,6 +267,10 @@ typedef struct fs_lp_config {
     float in_scale;           /* input scale of the coordinates */
     float lr, beta1, beta2, adam_eps;
     uint64_t seed;            /* Philox key */
+    int32_t coarse;           /* 0: segment proposal (levels 1 .. S of 2^S-slice segments);
+                                 1: whole-path move, N = 64 x 2^k: the network proposes the nodes every
+                                 64 slices (levels 1 .. k of the whole cyclic path), a Levy bridge the
+                                 slices between them, one Metropolis-Hastings decision per path */
 } fs_lp_config;
 typedef struct fs_lp_info {
     int64_t n_params;         /* FP32 master parameters (and each Adam moment) */
@@ -299,7 +304,7 @@ int fs_lp_build_log(const fs_lp *p, char *buf, size_t len, size_t *needed);
 /* Rebuild the BF16 inference image and training matrices from params. */
 int fs_lp_pack(fs_lp *p, const fs_lp_buffers *b, void *stream);
 /* n_sweeps learned sweeps (sweep counters sweep0 ...) over paths [walkers][DOF][N];
- * accepted segments are added to accepts[walkers]. */
+ * accepted segments (whole-path move: accepted paths) are added to accepts[walkers]. */
 int fs_lp_sweep(fs_lp *p, void *paths, void *accepts, const fs_lp_buffers *b, uint64_t sweep0, int32_t n_sweeps,
                 void *stream);
 /* One Adam step (step counts from 1) on `batch` rows drawn from paths, then a repack. */

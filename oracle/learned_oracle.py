@@ -38,7 +38,8 @@ HIDDEN = 128
 SMAX = 3.0
 M0, M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
 W0, W1 = np.uint32(0x9E3779B9), np.uint32(0xBB67AE85)
-TAG_NORMAL, TAG_ACCEPT, TAG_OFFSET, TAG_TRAIN = 0x100, 0x200, 0x300, 0x400
+TAG_NORMAL, TAG_ACCEPT, TAG_OFFSET, TAG_TRAIN, TAG_FINE = 0x100, 0x200, 0x300, 0x400, 0x500
+FINE_L = 64  # slices between the coarse nodes of the whole-path move
 
 
 # ---------------------------------------------------------------------------
@@ -109,13 +110,14 @@ def accept_uniforms(seed, walkers, sweep, segs):
     return u24(x[..., 0])
 
 
-def train_rows(seed, step, batch, W, N, SEG):
-    """(walker, segment start slice, node position p in [1, SEG)) of each training row."""
+def train_rows(seed, step, batch, W, N, SEG, fine=1):
+    """(walker, segment start slice, node position p in [1, SEG)) of each training row; with
+    fine = FINE_L (the whole-path move, SEG = N) only the coarse nodes, p a multiple of FINE_L."""
     b = np.arange(batch, dtype=np.uint64)
     x = philox4x32(_ctr(b, step, np.uint64(int(step) >> 32), TAG_TRAIN), seed_key(seed))
     w = ((x[..., 0].astype(np.uint64) * np.uint64(W)) >> np.uint64(32)).astype(np.int64)
     s0 = (x[..., 1] & np.uint32(N - 1)).astype(np.int64)
-    p = 1 + ((x[..., 2].astype(np.uint64) * np.uint64(SEG - 1)) >> np.uint64(32)).astype(np.int64)
+    p = fine * (1 + ((x[..., 2].astype(np.uint64) * np.uint64(SEG // fine - 1)) >> np.uint64(32)).astype(np.int64))
     return w, s0, p
 
 
@@ -297,6 +299,114 @@ def learned_sweep(paths, p, V, *, S, eps, seed, sweep, atoms3d=False, in_scale=1
         return acc.sum(axis=1), dict(logA=logA, u=u, accept=acc, new=new, old=old, slices=slices, lq=lq, dK=dK,
                                      dV=dV, r=r)
     return acc.sum(axis=1)
+
+
+# ---------------------------------------------------------------------------
+# One whole-path coarse move per walker
+# ---------------------------------------------------------------------------
+def fine_normals(seed, walker, sweep, M, dof):
+    """z[w, j, lane, d]: the bridge normals of interval j, lane 0..63; normal k = j dof + d of a lane
+    is Box-Muller on Philox block (64 (k // 4) + lane, walker, sweep, TAG_FINE), pair (k // 2) % 2,
+    cosine for even k, sine for odd k."""
+    walker = np.asarray(walker)
+    K = M * dof
+    nb = (K + 3) // 4
+    lane = np.arange(FINE_L)
+    blk = np.arange(nb)
+    x = philox4x32(_ctr(blk[None, :, None] * FINE_L + lane[None, None, :], walker[:, None, None], sweep, TAG_FINE),
+                   seed_key(seed))                                     # (W, nb, 64, 4)
+    k = np.arange(K)
+    pair = (k >> 1) & 1
+    xa = x[:, k >> 2, :, :][..., 0:4]
+    w1 = np.take_along_axis(xa, (2 * pair)[None, :, None, None].repeat(FINE_L, 2), axis=3)[..., 0]
+    w2 = np.take_along_axis(xa, (2 * pair + 1)[None, :, None, None].repeat(FINE_L, 2), axis=3)[..., 0]
+    rho = np.sqrt(np.float32(-2.0) * np.log(u24(w1, half=True))).astype(np.float32)
+    ang = (np.float32(2.0 * np.pi) * u24(w2)).astype(np.float32)
+    z = np.where((k & 1)[None, :, None] == 1, rho * np.sin(ang), rho * np.cos(ang)).astype(np.float32)  # (W, K, 64)
+    return z.reshape(walker.shape[0], M, dof, FINE_L).transpose(0, 1, 3, 2)    # (W, M, 64, D)
+
+
+def bridge(cn, z, eps):
+    """Fine slices of every interval: cn (W, M + 1, D) nodes, z (W, M, 64, D) -> (W, M, 64, D), lane 63
+    the node j + 1.  The lane prefix sum adds in the order of a log-step (Hillis-Steele) scan."""
+    v = (np.float32(np.sqrt(np.float32(eps))) * z).astype(np.float32)
+    o = 1
+    while o < FINE_L:
+        t = v.copy()
+        t[:, :, o:, :] = v[:, :, o:, :] + v[:, :, :-o, :]
+        v = t.astype(np.float32)
+        o *= 2
+    tot = v[:, :, FINE_L - 1:FINE_L, :]
+    fi = (np.arange(1, FINE_L + 1, dtype=np.float32) / np.float32(FINE_L))[None, None, :, None]
+    ca, cb = cn[:, :-1, None, :], cn[:, 1:, None, :]
+    x = (ca + (cb - ca) * fi + v - fi * tot).astype(np.float32)
+    x[:, :, FINE_L - 1, :] = cb[:, :, 0, :]
+    return x
+
+
+def coarse_move(paths, p, V, *, eps, seed, sweep, atoms3d=False, in_scale=1.0, emulate_bf16=True,
+                return_detail=False, walker0=0):
+    """The whole-path move: paths float32 (W, D, N), N = 64 M, M = 2^LC, updated in place.
+
+    Anchor r (Philox (0, walker, sweep, TAG_OFFSET) mod N); coarse nodes c_j = x(r + 64 j), node M =
+    node 0.  Levels n = 1..LC of the bisection of the whole path (S = log2 N: sigma_n^2 = N eps /
+    (2 * 2^n), level feature n / S) draw the new nodes from the network with normals counted by
+    (node j, walker, sweep, TAG_NORMAL | block), and score the old ones (the reverse rows); a Levy
+    bridge (`bridge`, normals `fine_normals`) fills each interval.  The bridge samples the fine
+    links' kinetic action exactly given the nodes, so
+        log alpha = -eps (V_new - V_old) - (Keff_new - Keff_old) + log q_old - log q_new,
+        Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps 64).
+    Returns accepted moves per walker (and, with return_detail, the parts of log alpha)."""
+    W, D, N = paths.shape
+    S = int(np.log2(N))
+    M = N // FINE_L
+    LC = S - int(np.log2(FINE_L))
+    assert (1 << S) == N and M >= 2
+    wi = np.arange(W)
+    wg = wi + int(walker0)
+    r = offsets(seed, wg, sweep, N)
+    nodes = (r[:, None] + FINE_L * np.arange(M + 1)[None, :]) % N                 # (W, M + 1)
+    cold = paths[wi[:, None, None], np.arange(D)[None, None, :], nodes[..., None]].astype(np.float32)
+    cnew = cold.copy()
+    lq = np.zeros(W, dtype=np.float32)
+    for n in range(1, LC + 1):
+        h = M >> n
+        m = np.arange(h, M, 2 * h)
+        sig = np.float32(np.sqrt(N * eps / (2.0 * (1 << n))))
+        for which in ("new", "old") if n > 1 else ("new",):
+            src = cnew if which == "new" else cold
+            a, b = src[:, m - h, :], src[:, m + h, :]
+            f = features(a, b, n, S, atoms3d, in_scale)
+            o, _ = mlp_forward(p, f.reshape(-1, f.shape[-1]), emulate_bf16)
+            o = o.reshape(f.shape[:-1] + (o.shape[-1],))
+            mu, s, sd = proposal_params(o, a, b, sig, D)
+            if which == "new":
+                z = normals(seed, wg[:, None], sweep, m[None, :], D)
+                cnew[:, m, :] = (mu + sd * z).astype(np.float32)
+                lq -= np.sum(-0.5 * z * z - s, axis=(1, 2)).astype(np.float32)
+                if n == 1:
+                    zo = ((cold[:, m, :] - mu) / sd).astype(np.float32)
+                    lq += np.sum(-0.5 * zo * zo - s, axis=(1, 2)).astype(np.float32)
+            else:
+                zo = ((cold[:, m, :] - mu) / sd).astype(np.float32)
+                lq += np.sum(-0.5 * zo * zo - s, axis=(1, 2)).astype(np.float32)
+        cnew[:, M, :] = cnew[:, 0, :]
+    xn = bridge(cnew, fine_normals(seed, wg, sweep, M, D), eps)                   # (W, M, 64, D)
+    fine = (r[:, None, None] + FINE_L * np.arange(M)[None, :, None] + 1 + np.arange(FINE_L)[None, None, :]) % N
+    xo = paths[wi[:, None, None, None], np.arange(D)[None, None, None, :], fine[..., None]].astype(np.float32)
+    dV = (V(xn.astype(np.float64)).sum(axis=(1, 2)) - V(xo.astype(np.float64)).sum(axis=(1, 2))).astype(np.float32)
+    keff = lambda c: np.sum((c[:, 1:, :] - c[:, :-1, :]).astype(np.float64) ** 2, axis=(1, 2)) / (2.0 * eps * FINE_L)
+    dK = (keff(cnew) - keff(cold)).astype(np.float32)
+    logA = (-dK - np.float32(eps) * dV + lq).astype(np.float32)
+    u = accept_uniforms(seed, wg, sweep, 0)
+    with np.errstate(over="ignore"):
+        acc = ~(np.exp(logA) < u)
+    for w in np.nonzero(acc)[0]:
+        paths[w][:, fine[w].reshape(-1)] = xn[w].reshape(-1, D).T
+    if return_detail:
+        return acc.astype(np.int64), dict(logA=logA, u=u, accept=acc, new_nodes=cnew, old_nodes=cold, fine=fine,
+                                          new_fine=xn, lq=lq, dK=dK, dV=dV, r=r)
+    return acc.astype(np.int64)
 
 
 def init_params(dof, seed, hidden=HIDDEN, zero_last=True, scale_last=0.0):

@@ -1,0 +1,191 @@
+"""The whole-path coarse move (LearnedProposal(coarse=True)) against the oracle's coarse_move
+(oracle/learned_oracle.py).
+
+Tolerances (floating point): the network runs in BF16 with FP32 accumulation and the GPU sums in
+other orders than numpy, so
+* the parts of log alpha (log q ratio, Keff change, V change) agree to 2e-2 + 1e-3 relative;
+* accept decisions agree wherever |log alpha - log u| > 0.05 (at least 80 % of the paths), and the
+  paths of walkers both accept agree to 5e-3;
+* with a zero output layer the coarse proposal is the Levy bridge, and log q_old - log q_new must
+  equal the change of the coarse kinetic action exactly in the mathematics: 1e-3 relative on the GPU.
+Statistics: on the C1 harmonic oscillator the move, alone and trained online, keeps <x^2> at the
+exact discretised value within 4 standard errors.
+"""
+import numpy as np
+import pytest
+
+from oracle import learned_oracle as L
+from oracle import oracle as O
+from feynsimul_amd.kernel_args import KernelArgs
+from feynsimul_amd.kernel import PIMCKernel
+from feynsimul_amd.learned import LearnedProposal
+from feynsimul_amd.physical_systems import HarmOsc, AnharmOsc, Lm2m2Cluster
+from feynsimul_amd.physical_systems.lm2m2 import lm2m2_kelvin
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print("[case] " + msg, flush=True)
+
+
+def _kernel(system, W, N, beta, opRuns=1, M=1, seed=5, S=6):
+    ka = KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=S, enableOperator=True,
+                    enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                    enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                    enableGlobalOldPath=False, enableBins=False, enableDouble=False, enableRanlux=False,
+                    metroStepsPerOperatorRun=M, operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=1,
+                    operators=(system.energyOp,), seed=seed, sampler="wave", pathStorage="memory")
+    return PIMCKernel(ka)
+
+
+def _ho_cov(N, beta, k=1.0):
+    eps = beta / N
+    A = np.zeros((N, N))
+    for i in range(N):
+        A[i, i] = 2 / eps + eps * k
+        A[i, (i + 1) % N] -= 1 / eps
+        A[i, (i - 1) % N] -= 1 / eps
+    return np.linalg.inv(A)
+
+
+def _poly_V(lam):
+    return lambda x: 0.5 * x[..., 0] * x[..., 0] + lam * x[..., 0] ** 4
+
+
+def _lm2m2_V(system):
+    vk = np.vectorize(lm2m2_kelvin)
+
+    def V(x):
+        x = x.astype(np.float64)
+        tot = 0.0
+        for i, j in system.pairs:
+            r = np.sqrt(((x[..., 3 * i:3 * i + 3] - x[..., 3 * j:3 * j + 3]) ** 2).sum(-1)) / system.massScale
+            tot = tot + vk(r)
+        return tot
+    return V
+
+
+def _setup(case, rng):
+    if case in ("ho128", "ho256", "aho512"):
+        N, beta = {"ho128": (128, 2.0), "ho256": (256, 10.0), "aho512": (512, 8.0)}[case]
+        system = HarmOsc() if case.startswith("ho") else AnharmOsc(0.1)
+        W, D = 64, 1
+        # Gaussian paths of the discretised harmonic action
+        chol = np.linalg.cholesky(_ho_cov(N, beta))
+        paths0 = (rng.standard_normal((W, N)) @ chol.T).astype(np.float32)[:, None, :]
+        V = _poly_V(0.0 if case.startswith("ho") else 0.1)
+        return system, W, N, beta, D, paths0, V, 0.03
+    system, W, N, beta, D = Lm2m2Cluster(3), 16, 128, 1.0, 9
+    base = system.initialPositions(W, rng)
+    paths0 = (base[:, :, None] + 0.02 * rng.standard_normal((W, D, N))).astype(np.float32)
+    return system, W, N, beta, D, paths0, _lm2m2_V(system), 0.05
+
+
+def _run_case(case, theta_scale=None, seed=777, sweep=3):
+    rng = np.random.default_rng(11)
+    system, W, N, beta, D, paths0, V, scale = _setup(case, rng)
+    if theta_scale is not None:
+        scale = theta_scale
+    k = _kernel(system, W, N, beta)
+    k.setPaths(paths0.reshape(W, D * N))
+    lp = LearnedProposal(k, batch=2048, seed=seed, coarse=True)
+    theta = L.init_params(D, 5, scale_last=scale)
+    lp.setParameters(theta)
+    lp._sweepCount = sweep
+    lp.traceSweeps()
+    lp.sweep(1)
+    got = k.getPaths().reshape(W, D, N)
+    tr = lp.getTrace()[:, 0, :]
+    ref = paths0.copy()
+    _, d = L.coarse_move(ref, L.unflatten(theta, D), V, eps=beta / N, seed=seed, sweep=sweep, atoms3d=(D == 9),
+                         return_detail=True)
+    return paths0, got, ref, tr, d
+
+
+@pytest.mark.parametrize("case", ["ho128", "ho256", "aho512", "he3"])
+def test_coarse_move_matches_oracle(gpu, case):
+    _case("whole-path move %s" % case)
+    paths0, got, ref, tr, d = _run_case(case)
+    W = paths0.shape[0]
+    for name, col, tol in (("lq", 0, 1e-3), ("dK", 1, 1e-3), ("dV", 2, 1e-3)):
+        err = np.abs(tr[:, col] - d[name])
+        lim = 2e-2 + tol * np.abs(d[name])
+        print("%s max err %.3e (worst allowed %.3e)" % (name, err.max(), lim[np.argmax(err)]))
+        assert (err < lim).all(), (name, tr[:, col][:4], d[name][:4])
+    changed = np.array([not np.array_equal(paths0[w], got[w]) for w in range(W)])
+    clear = np.abs(d["logA"] - np.log(np.maximum(d["u"], 1e-30))) > 0.05
+    print("accepted %d/%d, clear %d" % (d["accept"].sum(), W, clear.sum()))
+    assert clear.mean() > 0.8
+    assert np.array_equal(changed[clear], d["accept"][clear])
+    for w in np.nonzero(changed & d["accept"])[0]:
+        assert np.abs(got[w] - ref[w]).max() < 5e-3
+    if case in ("ho128", "aho512"):
+        assert 0 < d["accept"].sum() < W
+
+
+def test_zero_output_is_the_levy_bridge(gpu):
+    """With the output layer at zero the coarse proposal is the free-particle bridge of the whole
+    path, whose density ratio is exactly the change of the coarse kinetic action: log alpha reduces
+    to -eps dV, plain bisection of the whole path."""
+    _case("whole-path move, zero output layer")
+    paths0, got, ref, tr, d = _run_case("ho256", theta_scale=0.0)
+    err = np.abs(tr[:, 0] - tr[:, 1])
+    print("max |lq - dK| %.3e, max |dK| %.3e" % (err.max(), np.abs(tr[:, 1]).max()))
+    assert (err < 1e-3 * np.abs(tr[:, 1]) + 1e-3).all()
+    np.testing.assert_allclose(d["lq"], d["dK"], rtol=1e-4, atol=1e-4)
+
+
+def test_coarse_move_keeps_ho_energy(gpu):
+    """C1 sizes (N = 256, beta = 10, 4096 chains): whole-path moves alone, with the network trained
+    online on the coarse nodes, sample the exact discretised <x^2> within 4 standard errors."""
+    _case("whole-path move, C1 energy")
+    system = HarmOsc()
+    W, N, beta = 4096, 256, 10.0
+    k = _kernel(system, W, N, beta, opRuns=1, M=1500, S=4)
+    k.setPaths(np.random.default_rng(0).uniform(-1, 1, (W, N)).astype(np.float32))
+    k.run()
+    lp = LearnedProposal(k, batch=16384, seed=31, coarse=True, fallbackAcceptance=0.0)
+    first = lp.trainOnline(1)
+    last = lp.trainOnline(99)
+    ar = lp.sweep(64)
+    k.setSeriesCapacity(200)
+    lp.run(200, 8)
+    s = k.getSeries()[:, :, 0].astype(np.float64)
+    per = s.mean(axis=0)
+    E, se = per.mean(), per.std() / np.sqrt(W)
+    exact = O.ho_discrete_x2(N, beta)
+    print("NLL %.4f -> %.4f, whole-path acceptance %.3f, E %.6f +- %.6f exact %.9f" % (first, last, ar, E, se,
+                                                                                     exact))
+    assert last < first
+    assert 0.05 < ar < 1.0
+    assert abs(E - exact) < 4 * se
+
+
+def test_coarse_training_rows_match_oracle(gpu):
+    """Training draws only coarse nodes (positions that are multiples of 64 of the whole path):
+    the loss of one Adam step equals the oracle's on the same rows."""
+    _case("whole-path move, training rows")
+    rng = np.random.default_rng(4)
+    system, W, N, beta, D = HarmOsc(), 128, 512, 8.0, 1
+    paths = rng.standard_normal((W, D, N)).astype(np.float32)
+    k = _kernel(system, W, N, beta)
+    k.setPaths(paths.reshape(W, D * N))
+    B = 4096
+    lp = LearnedProposal(k, batch=B, seed=78, coarse=True)
+    theta = L.init_params(D, 1, scale_last=0.3)
+    lp.setParameters(theta)
+    loss = lp.train(1)
+    S = 9
+    w, s0, p = L.train_rows(78, 1, B, W, N, N, fine=L.FINE_L)
+    assert (p % L.FINE_L == 0).all() and p.min() >= L.FINE_L and p.max() <= N - L.FINE_L
+    h = p & -p
+    n = S - np.log2(h).astype(np.int64)
+    a = paths[w, :, (s0 + p - h) % N]
+    b = paths[w, :, (s0 + p + h) % N]
+    x = paths[w, :, (s0 + p) % N]
+    f = L.features(a, b, n, S, False, 1.0)
+    sig = np.sqrt(N * (beta / N) / (2.0 * 2.0 ** n)).astype(np.float32)
+    loss_o, _ = L.nll_and_grad(L.unflatten(theta, D), f, a, b, x, sig, D)
+    print("loss %.6f oracle %.6f" % (loss, loss_o))
+    assert abs(loss - loss_o) < 1e-3 * abs(loss_o) + 1e-4
