@@ -52,7 +52,8 @@ class FsLpConfig(ctypes.Structure):
     _fields_ = [("abi_version", ctypes.c_int32), ("atoms3d", ctypes.c_int32), ("batch", ctypes.c_int32),
                 ("segs_per_wave", ctypes.c_int32), ("in_scale", ctypes.c_float), ("lr", ctypes.c_float),
                 ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("adam_eps", ctypes.c_float),
-                ("seed", ctypes.c_uint64), ("coarse", ctypes.c_int32)]
+                ("seed", ctypes.c_uint64), ("coarse", ctypes.c_int32),
+                ("width", ctypes.c_int32)]
 
 
 class FsLpInfo(ctypes.Structure):

@@ -23,6 +23,25 @@ template <typename T> FS_DEV T fs_lane_next(T v)
     return a.v;
 }
 
+// Inclusive prefix sum over the 64 lanes with six DPP additions and no LDS:
+// a log-step scan inside each 16-lane row (row_shr 1, 2, 4, 8; lanes shifted
+// in from outside the row read 0), then row 0's total (lane 15) into row 1 and
+// row 2's (lane 47) into row 3 (row_bcast:15), and the first half's total
+// (lane 31) into rows 2 and 3 (row_bcast:31).
+#define FS_DPP_F(x, ctrl, rows) \
+    __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), (ctrl), (rows), 0xF, false))
+FS_DEV float fs_lane_scan(float v)
+{
+    v += FS_DPP_F(v, 0x111, 0xF);  // row_shr:1
+    v += FS_DPP_F(v, 0x112, 0xF);  // row_shr:2
+    v += FS_DPP_F(v, 0x114, 0xF);  // row_shr:4
+    v += FS_DPP_F(v, 0x118, 0xF);  // row_shr:8
+    v += FS_DPP_F(v, 0x142, 0xA);  // row_bcast:15 into rows 1 and 3
+    v += FS_DPP_F(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
+#undef FS_DPP_F
+
 // The value of lane l (wave-uniform l) in scalar registers, one v_readlane_b32
 // per 32-bit word.
 template <typename T> FS_DEV T fs_readlane(T v, int l)

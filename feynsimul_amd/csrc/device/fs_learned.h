@@ -53,6 +53,15 @@ typedef __attribute__((ext_vector_type(8))) __bf16 fs_bf8;
 typedef __attribute__((ext_vector_type(16))) float fs_f16v;
 
 #define FS_LP_KIN 32
+// Active hidden width of inference (a multiple of 32, at most FS_LP_HID): the image keeps the
+// FS_LP_HID-wide layout, and a network whose units beyond the active width are zero (weights in and
+// out, and biases; training leaves them zero: their ReLU masks stop every gradient) is evaluated on
+// the active units only.
+#ifndef FS_LP_ACT
+#define FS_LP_ACT FS_LP_HID
+#endif
+#define FS_LP_OB (FS_LP_ACT / 32)  // 32-unit output blocks of a hidden layer
+#define FS_LP_KS (FS_LP_ACT / 16)  // 16-unit k-steps of the layer after it
 #define FS_LP_SMAX 3.0f
 #define FS_LP_TAG_NORMAL 0x100u
 #define FS_LP_TAG_ACCEPT 0x200u
@@ -126,7 +135,7 @@ FS_DEV void fs_lp_bias_init(const float *b, int ob, int g, fs_f16v (&c)[FS_LP_CB
 FS_DEV void fs_lp_relu_pack(const fs_f16v (&acc)[4][FS_LP_CB], fs_bf8 (&h)[8][FS_LP_CB])
 {
 #pragma unroll
-    for (int ob = 0; ob < 4; ob++)
+    for (int ob = 0; ob < FS_LP_OB; ob++)
 #pragma unroll
         for (int cb = 0; cb < FS_LP_CB; cb++)
 #pragma unroll
@@ -154,11 +163,11 @@ FS_DEV void fs_lp_hidden(const char *img, int woff, int boff, const fs_bf8 (&hin
     const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + woff);
     const float *b = reinterpret_cast<const float *>(img + boff);
 #pragma unroll
-    for (int ob = 0; ob < 4; ob++) fs_lp_bias_init(b, ob, g, acc[ob]);
+    for (int ob = 0; ob < FS_LP_OB; ob++) fs_lp_bias_init(b, ob, g, acc[ob]);
 #pragma unroll
-    for (int ks = 0; ks < 8; ks++)
+    for (int ks = 0; ks < FS_LP_KS; ks++)
 #pragma unroll
-        for (int ob = 0; ob < 4; ob++) {
+        for (int ob = 0; ob < FS_LP_OB; ob++) {
             const fs_bf8 A = W[(ob * 8 + ks) * 64 + lane];
 #pragma unroll
             for (int cb = 0; cb < FS_LP_CB; cb++) acc[ob][cb] = fs_lp_mfma(A, hin[ks][cb], acc[ob][cb]);
@@ -173,11 +182,11 @@ FS_DEV void fs_lp_mlp(const char *img, const fs_bf8 (&xin)[2][FS_LP_CB], fs_f16v
         const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_OFF_W1);
         const float *b = reinterpret_cast<const float *>(img + FS_LP_OFF_B1);
 #pragma unroll
-        for (int ob = 0; ob < 4; ob++) fs_lp_bias_init(b, ob, g, acc[ob]);
+        for (int ob = 0; ob < FS_LP_OB; ob++) fs_lp_bias_init(b, ob, g, acc[ob]);
 #pragma unroll
         for (int ks = 0; ks < 2; ks++)
 #pragma unroll
-            for (int ob = 0; ob < 4; ob++) {
+            for (int ob = 0; ob < FS_LP_OB; ob++) {
                 const fs_bf8 A = W[(ob * 2 + ks) * 64 + lane];
 #pragma unroll
                 for (int cb = 0; cb < FS_LP_CB; cb++) acc[ob][cb] = fs_lp_mfma(A, xin[ks][cb], acc[ob][cb]);
@@ -201,7 +210,7 @@ FS_DEV void fs_lp_mlp(const char *img, const fs_bf8 (&xin)[2][FS_LP_CB], fs_f16v
         const float *b = reinterpret_cast<const float *>(img + FS_LP_OFF_B4);
         fs_lp_bias_init(b, 0, g, out);
 #pragma unroll
-        for (int ks = 0; ks < 8; ks++) {
+        for (int ks = 0; ks < FS_LP_KS; ks++) {
             const fs_bf8 A = W[ks * 64 + lane];
 #pragma unroll
             for (int cb = 0; cb < FS_LP_CB; cb++) out[cb] = fs_lp_mfma(A, h[ks][cb], out[cb]);
@@ -470,7 +479,12 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
 //   log alpha = -eps (V_new - V_old) - (Keff_new - Keff_old) + log q(old nodes) - log q(new nodes),
 //   Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps L).
 // Random numbers are counted by (node or lane and block, walker, sweep, purpose) as in the segment
-// move; the bridge's normals are drawn twice (to score the path, and to write it once accepted).
+// move.  A lane keeps its new slices in registers until the decision when they fit (M DOF <= 64);
+// otherwise the bridge's normals are drawn twice (to score the path, and to write it once accepted).
+// One move is two launches: fs_lp_sweep runs the network on the coarse levels (FS_LP_SPW walkers per
+// wavefront, sharing row tiles, at the network's register budget) and leaves each walker's new nodes
+// and log q ratio in the workspace; fs_lpc_fine, one wavefront per walker and few registers, builds the
+// bridges, decides and writes.
 #define FS_LP_TAG_FINE 0x500u
 #ifndef FS_LPC_L
 #define FS_LPC_L 64
@@ -478,6 +492,7 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
 template <class Sys> struct fs_lpc_geom {
     static constexpr int N = Sys::N, S = Sys::S, D = Sys::DOF, L = FS_LPC_L, M = N / L, WPW = FS_LP_SPW;
     static constexpr int NODES = WPW * M * D;                                // floats per node set
+    static constexpr int SCR = M * D + 2;                                    // workspace floats per walker
     static constexpr int WAVE_BYTES = ((2 * NODES + 2 * WPW) * 4 + 15) / 16 * 16;
     static_assert(L == 64, "coarse move: one lane per fine slice");
     static_assert((1 << S) == N && M >= 2, "coarse move: N = 64 x 2^LC, LC >= 1");
@@ -500,13 +515,8 @@ FS_DEV void fs_lpc_bridge(const float *cn, int j, int M, int lane, uint4 &px, fs
     const float fi = (float)(lane + 1) * (1.0f / (float)FS_LPC_L);
 #pragma unroll
     for (int d = 0; d < D; d++) {
-        float v = sqrtf(eps) * fs_lpc_fine_normal(px, j * D + d, lane, wl, a);
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const float t = __shfl_up(v, o, 64);
-            if (lane >= o) v += t;
-        }
-        const float tot = __shfl(v, 63, 64);
+        const float v = fs_lane_scan(sqrtf(eps) * fs_lpc_fine_normal(px, j * D + d, lane, wl, a));
+        const float tot = fs_readlane(v, 63);
         const float ca = cn[j * D + d], cb = cn[((j + 1) & (M - 1)) * D + d];
         x[d] = lane == 63 ? cb : ca + (cb - ca) * fi + v - fi * tot;
     }
@@ -649,47 +659,118 @@ FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long
         }
     }
 
-    // ---- fine levels (Levy bridge), the decision, and the write of an accepted path ----
-    const float eps = (float)Sys::EPS, inv2eL = 0.5f / ((float)Sys::EPS * (float)L);
+    // the new nodes, the log q ratio and the change of the coarse kinetic action of every walker,
+    // for the fine pass
+    const float inv2eL = 0.5f / ((float)Sys::EPS * (float)L);
 #pragma unroll 1
     for (int k = 0; k < WPW; k++) {
-        const long w = w0 + k;
-        const fs_u32 wl = (fs_u32)w;
-        float *path = a.paths + w * (long)(N * D);
-        const float *cn = cnew + k * M * D, *co = cold + k * M * D;
-        const int rk = anchor(w);
-        float dv = 0.0f, dk = 0.0f;
-        uint4 px = make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll 1
-        for (int j = 0; j < M; j++) {
-            float xn[D], xold[D];
-            fs_lpc_bridge<D>(cn, j, M, lane, px, wl, a, eps, xn);
-            const int s = (rk + j * L + lane + 1) & (N - 1);
-#pragma unroll
-            for (int d = 0; d < D; d++) {
-                xold[d] = path[d * N + s];
-                const float en = cn[((j + 1) & (M - 1)) * D + d] - cn[j * D + d];
-                const float eo = co[((j + 1) & (M - 1)) * D + d] - co[j * D + d];
-                dk += en * en - eo * eo;
-            }
-            dv += (float)Sys::V(xn) - (float)Sys::V(xold);
+        float *dst = a.nodes + (w0 + k) * (long)G::SCR;
+        float dk = 0.0f;
+        for (int i = lane; i < M * D; i += 64) {
+            const int j = i / D, d = i % D, i1 = ((j + 1) & (M - 1)) * D + d;
+            const float en = cnew[k * M * D + i1] - cnew[k * M * D + i], eo = cold[k * M * D + i1] - cold[k * M * D + i];
+            dk += en * en - eo * eo;
+            dst[i] = cnew[k * M * D + i];
         }
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) dv += __shfl_xor(dv, o, 64);
-        dk *= inv2eL;  // the same on every lane
-        const float lqk = lq[k] + lq[WPW + k];
-        const float logA = -dk - eps * dv + lqk;
+        for (int o = 1; o < 64; o <<= 1) dk += __shfl_xor(dk, o, 64);
+        if (lane == 0) {
+            dst[M * D] = lq[k] + lq[WPW + k];
+            dst[M * D + 1] = dk * inv2eL;
+        }
+    }
+}
+
+// The fine pass of one walker (one wavefront): bridges between the new nodes, log alpha, the
+// decision, and the write of an accepted path.  cn: LDS for the walker's new nodes.
+template <class Sys>
+FS_DEV void fs_lpc_fine_walker(const fs_lp_args &a, float *cn, long w, int lane)
+{
+    typedef fs_lpc_geom<Sys> G;
+    constexpr int N = G::N, D = G::D, L = G::L, M = G::M;
+    const fs_u32 wl = (fs_u32)w;
+    float *path = a.paths + w * (long)(N * D);
+    const int rk = (int)(fs_philox(make_uint4(0u, wl, a.sweep, FS_LP_TAG_OFFSET), a.key0, a.key1).x & (N - 1));
+    const float *src = a.nodes + w * (long)G::SCR;
+    for (int i = lane; i < M * D; i += 64) cn[i] = src[i];
+    const float lqk = src[M * D], dk = src[M * D + 1];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const float eps = (float)Sys::EPS;
+    const float u = fs_lp_u24(fs_philox(make_uint4(0u, wl, a.sweep, FS_LP_TAG_ACCEPT), a.key0, a.key1).x);
+    float dv = 0.0f;
+    uint4 px = make_uint4(0u, 0u, 0u, 0u);
+    // the old slices of the next interval are loaded while this one is computed
+    float xnext[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) xnext[d] = path[d * N + ((rk + lane + 1) & (N - 1))];
+    auto decide = [&](float dvs) {
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) dvs += __shfl_xor(dvs, o, 64);
+        const float logA = -dk - eps * dvs + lqk;
         if (a.trace && lane == 0) {
             float *tr = a.trace + w * 4;
             tr[0] = lqk;
             tr[1] = dk;
-            tr[2] = dv;
+            tr[2] = dvs;
             tr[3] = logA;
         }
-        const float u = fs_lp_u24(fs_philox(make_uint4(0u, wl, a.sweep, FS_LP_TAG_ACCEPT), a.key0, a.key1).x);
-        if (!(expf(logA) < u)) {
+        return !(expf(logA) < u);
+    };
+    if constexpr (M * D <= 64) {
+        // a lane's new slices fit in registers: kept there until the decision.  The old slices only
+        // enter through -V; the bridges are drawn one Philox block (four normals) at a time.
+#pragma unroll 4
+        for (int j = 0; j < M; j++) {
+            float xold[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) xold[d] = path[d * N + ((rk + j * L + lane + 1) & (N - 1))];
+            dv -= (float)Sys::V(xold);
+        }
+        float xs[M][D];
+        const float fi = (float)(lane + 1) * (1.0f / (float)L), se = sqrtf(eps);
+#pragma unroll
+        for (int b = 0; b < (M * D + 3) / 4; b++) {
+            const uint4 x =
+                fs_philox(make_uint4((fs_u32)(b * 64 + lane), wl, a.sweep, FS_LP_TAG_FINE), a.key0, a.key1);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int k = 4 * b + t, j = k / D, d = k % D;
+                if (k < M * D) {
+                    const float v = fs_lane_scan(se * fs_lp_normal(x, t >> 1, t & 1));
+                    const float tot = fs_readlane(v, 63);
+                    const float ca = cn[j * D + d], cb = cn[((j + 1) & (M - 1)) * D + d];
+                    xs[j][d] = lane == 63 ? cb : ca + (cb - ca) * fi + v - fi * tot;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < M; j++) dv += (float)Sys::V(xs[j]);
+        if (decide(dv)) {
+#pragma unroll
+            for (int j = 0; j < M; j++) {
+                const int s = (rk + j * L + lane + 1) & (N - 1);
+#pragma unroll
+                for (int d = 0; d < D; d++) path[d * N + s] = xs[j][d];
+            }
+            if (lane == 0) a.accepts[w] += 1u;
+        }
+    } else {
+        // otherwise the bridges are drawn again (the same normals) to write an accepted path
+#pragma unroll 2
+        for (int j = 0; j < M; j++) {
+            float xn[D], xold[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                xold[d] = xnext[d];
+                if (j + 1 < M) xnext[d] = path[d * N + ((rk + (j + 1) * L + lane + 1) & (N - 1))];
+            }
+            fs_lpc_bridge<D>(cn, j, M, lane, px, wl, a, eps, xn);
+            dv += (float)Sys::V(xn) - (float)Sys::V(xold);
+        }
+        if (decide(dv)) {
             px = make_uint4(0u, 0u, 0u, 0u);
-#pragma unroll 1
+#pragma unroll 2
             for (int j = 0; j < M; j++) {
                 float xn[D];
                 fs_lpc_bridge<D>(cn, j, M, lane, px, wl, a, eps, xn);
@@ -937,6 +1018,16 @@ extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp
     for (long long u = (long long)blockIdx.x * FS_LP_WAVES + wave; u < a.nUnits;
          u += (long long)gridDim.x * FS_LP_WAVES)
         fs_lpc_unit<sys>(a, fs_lp_smem, stage, (long)u * G::WPW, lane);
+}
+// The fine pass of the whole-path move: one wavefront per walker, FS_LPC_FINE_WAVES per block.
+#define FS_LPC_FINE_WAVES 4
+extern "C" __global__ __launch_bounds__(64 * FS_LPC_FINE_WAVES) void fs_lpc_fine(fs_lp_args a)
+{
+    typedef fs_lpc_geom<sys> G;
+    __shared__ float nodes[FS_LPC_FINE_WAVES][G::M * G::D];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long w = (long)blockIdx.x * FS_LPC_FINE_WAVES + wave;
+    if (w < a.nWalkers) fs_lpc_fine_walker<sys>(a, nodes[wave], w, lane);
 }
 #else
 extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp_args a)

@@ -25,6 +25,7 @@ struct fs_lp_args {
     unsigned sweep;         // sweep counter (low 32 bits)
     float inScale;
     float *trace;           // optional [walkers][N / 2^S][4]: log q ratio, dK, dV, log alpha (test support)
+    float *nodes;           // whole-path move: [walkers][M DOF + 2] new coarse nodes, log q ratio, dKeff
 };
 
 struct fs_lp_train_args {

@@ -24,15 +24,42 @@ import numpy as np
 from . import _native
 
 
-def _he_init(dof, seed, hidden=128):
-    """He-normal hidden layers, zero biases, zero output layer (the Levy bridge)."""
+def _mask_width(theta, dof, active, hidden=128):
+    """Zero the parameters of the hidden units beyond `active` (weights in and out, biases)."""
+    theta = np.array(theta, dtype=np.float32, copy=True)
+    if active >= hidden:
+        return theta
+    nin, nout, H = 2 * dof + 1, 2 * dof, hidden
+    o = 0
+    W1 = theta[o:o + H * nin].reshape(H, nin); o += H * nin
+    b1 = theta[o:o + H]; o += H
+    W2 = theta[o:o + H * H].reshape(H, H); o += H * H
+    b2 = theta[o:o + H]; o += H
+    W3 = theta[o:o + H * H].reshape(H, H); o += H * H
+    b3 = theta[o:o + H]; o += H
+    W4 = theta[o:o + nout * H].reshape(nout, H)
+    W1[active:] = 0.0
+    b1[active:] = 0.0
+    W2[active:] = 0.0
+    W2[:, active:] = 0.0
+    b2[active:] = 0.0
+    W3[active:] = 0.0
+    W3[:, active:] = 0.0
+    b3[active:] = 0.0
+    W4[:, active:] = 0.0
+    return theta
+
+
+def _he_init(dof, seed, hidden=128, active=128):
+    """He-normal hidden layers, zero biases, zero output layer (the Levy bridge); units beyond
+    `active` zero."""
     rng = np.random.default_rng(seed)
     nin, nout = 2 * dof + 1, 2 * dof
     parts = [rng.standard_normal((hidden, nin)) * np.sqrt(2.0 / nin), np.zeros(hidden),
-             rng.standard_normal((hidden, hidden)) * np.sqrt(2.0 / hidden), np.zeros(hidden),
-             rng.standard_normal((hidden, hidden)) * np.sqrt(2.0 / hidden), np.zeros(hidden),
+             rng.standard_normal((hidden, hidden)) * np.sqrt(2.0 / active), np.zeros(hidden),
+             rng.standard_normal((hidden, hidden)) * np.sqrt(2.0 / active), np.zeros(hidden),
              np.zeros((nout, hidden)), np.zeros(nout)]
-    return np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32)
+    return _mask_width(np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32), dof, active, hidden)
 
 
 class LearnedProposal:
@@ -44,6 +71,9 @@ class LearnedProposal:
               the nodes every 64 slices of the whole cyclic path (bisection levels 1..k), a Levy
               bridge fills the slices between them, one Metropolis-Hastings decision per path;
               training uses those nodes only
+    width:    active hidden units (multiple of 32, at most 128; default 64 for the whole-path move,
+              128 otherwise): the parameters keep the 128-wide layout with the other units at zero,
+              which training preserves (their ReLU masks stop every gradient)
     lr:       Adam learning rate (beta1 0.9, beta2 0.999, eps 1e-8)
     seed:     Philox key of the proposal's random numbers (default: the kernel's seed + 1)
     atoms3d:  centre the inputs on the atoms' mean (default: system.translationInvariant)
@@ -51,7 +81,7 @@ class LearnedProposal:
     """
 
     def __init__(self, kernel, batch=65536, lr=1e-3, seed=None, atoms3d=None, inScale=1.0, segsPerWave=0,
-                 initSeed=0, fallbackAcceptance=0.2, coarse=False):
+                 initSeed=0, fallbackAcceptance=0.2, coarse=False, width=None):
         if kernel._enableDouble:
             raise Exception("the learned proposal runs on FP32 paths")
         if kernel._engine is None:
@@ -74,6 +104,8 @@ class LearnedProposal:
         lp.seed = int(seed) & ((1 << 64) - 1)
         lp.coarse = int(bool(coarse))
         self.coarse = bool(coarse)
+        self.width = int(width if width is not None else (64 if coarse else 128))
+        lp.width = self.width
         # moves proposed per sweep: every segment, or (whole-path move) every path
         self._movesPerSweep = kernel._nbrOfWalkers * (1 if coarse else kernel._N >> kernel._S)
         self._P = _native.Proposal(kernel._cfg, kernel._systemSource, lp)
@@ -95,7 +127,7 @@ class LearnedProposal:
         self.fallbackAcceptance = float(fallbackAcceptance)
         self.lastAcceptance = None
         self.fallbackBlocks = 0
-        self.setParameters(_he_init(self._dof, initSeed))
+        self.setParameters(_he_init(self._dof, initSeed, active=self.width))
 
     # ---- buffers and parameters ----
     def _bufs(self):
@@ -117,11 +149,13 @@ class LearnedProposal:
         return self._params.cpu().numpy()
 
     def setParameters(self, theta, resetAdam=True):
-        """FP32 parameters in logical order W1 [128][2D+1], b1, W2, b2, W3, b3, W4 [2D][128], b4."""
+        """FP32 parameters in logical order W1 [128][2D+1], b1, W2, b2, W3, b3, W4 [2D][128], b4.
+        The parameters of hidden units beyond the active width are set to zero."""
         t = self._k._torch
         theta = np.ascontiguousarray(theta, dtype=np.float32).reshape(-1)
         if theta.size != self.parameterCount():
             raise ValueError("expected %d parameters, got %d" % (self.parameterCount(), theta.size))
+        theta = _mask_width(theta, self._dof, self.width)
         self._params.copy_(t.from_numpy(theta))
         if resetAdam:
             self._m.zero_()

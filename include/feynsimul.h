@@ -271,6 +271,9 @@ typedef struct fs_lp_config {
                                  1: whole-path move, N = 64 x 2^k: the network proposes the nodes every
                                  64 slices (levels 1 .. k of the whole cyclic path), a Levy bridge the
                                  slices between them, one Metropolis-Hastings decision per path */
+    int32_t width;            /* active hidden units of inference, a multiple of 32 up to 128 (0 = 128):
+                                 the parameters keep the 128-wide layout, and units beyond the active
+                                 width must be zero (training keeps them zero) */
 } fs_lp_config;
 typedef struct fs_lp_info {
     int64_t n_params;         /* FP32 master parameters (and each Adam moment) */

@@ -326,16 +326,31 @@ def fine_normals(seed, walker, sweep, M, dof):
     return z.reshape(walker.shape[0], M, dof, FINE_L).transpose(0, 1, 3, 2)    # (W, M, 64, D)
 
 
+def lane_scan(v):
+    """Inclusive prefix sum over axis 2 (64 lanes) in the GPU's order of additions: a log-step scan
+    inside each 16-lane row (shifts 1, 2, 4, 8), then row 0's total into row 1 and row 2's into row 3,
+    then the first half's total into rows 2 and 3."""
+    v = v.astype(np.float32).copy()
+    lane = np.arange(FINE_L)
+    for o in (1, 2, 4, 8):
+        t = np.zeros_like(v)
+        t[:, :, o:, :] = v[:, :, :-o, :]
+        t[:, :, (lane % 16) < o, :] = 0.0
+        v = (v + t).astype(np.float32)
+    t = np.zeros_like(v)
+    t[:, :, 16:32, :] = v[:, :, 15:16, :]
+    t[:, :, 48:64, :] = v[:, :, 47:48, :]
+    v = (v + t).astype(np.float32)
+    t = np.zeros_like(v)
+    t[:, :, 32:64, :] = v[:, :, 31:32, :]
+    return (v + t).astype(np.float32)
+
+
 def bridge(cn, z, eps):
     """Fine slices of every interval: cn (W, M + 1, D) nodes, z (W, M, 64, D) -> (W, M, 64, D), lane 63
-    the node j + 1.  The lane prefix sum adds in the order of a log-step (Hillis-Steele) scan."""
-    v = (np.float32(np.sqrt(np.float32(eps))) * z).astype(np.float32)
-    o = 1
-    while o < FINE_L:
-        t = v.copy()
-        t[:, :, o:, :] = v[:, :, o:, :] + v[:, :, :-o, :]
-        v = t.astype(np.float32)
-        o *= 2
+    the node j + 1: x_i = c_j + (c_{j+1} - c_j) i / 64 + W_i - (i / 64) W_64, W the prefix sums
+    (`lane_scan`) of the increments sqrt(eps) z."""
+    v = lane_scan((np.float32(np.sqrt(np.float32(eps))) * z).astype(np.float32))
     tot = v[:, :, FINE_L - 1:FINE_L, :]
     fi = (np.arange(1, FINE_L + 1, dtype=np.float32) / np.float32(FINE_L))[None, None, :, None]
     ca, cb = cn[:, :-1, None, :], cn[:, 1:, None, :]
@@ -409,16 +424,30 @@ def coarse_move(paths, p, V, *, eps, seed, sweep, atoms3d=False, in_scale=1.0, e
     return acc.astype(np.int64)
 
 
-def init_params(dof, seed, hidden=HIDDEN, zero_last=True, scale_last=0.0):
+def mask_width(theta, dof, active, hidden=HIDDEN):
+    """Zero every parameter of the hidden units beyond `active` (their incoming weights and biases and
+    their outgoing weights): the network then computes with `active` units exactly."""
+    p = {k: v.copy() for k, v in unflatten(np.asarray(theta, np.float32), dof, hidden).items()}
+    for L in ("1", "2", "3"):
+        p["W" + L][active:, :] = 0.0
+        p["b" + L][active:] = 0.0
+    for L in ("2", "3", "4"):
+        p["W" + L][:, active:] = 0.0
+    return np.concatenate([p[k].reshape(-1) for k, _ in param_shapes(dof, hidden)]).astype(np.float32)
+
+
+def init_params(dof, seed, hidden=HIDDEN, zero_last=True, scale_last=0.0, active=None):
     """He-normal hidden layers, zero biases; the last layer is zero (the Levy bridge) unless
-    scale_last is given."""
+    scale_last is given.  `active`: only the first `active` hidden units are used (the rest zero)."""
     rng = np.random.default_rng(seed)
     p = {}
+    fan = {"W1": 2 * dof + 1, "W2": active or hidden, "W3": active or hidden, "W4": active or hidden}
     for name, shape in param_shapes(dof, hidden):
         if name.startswith("b"):
             p[name] = np.zeros(shape, np.float32)
         else:
-            p[name] = (rng.standard_normal(shape) * np.sqrt(2.0 / shape[1])).astype(np.float32)
+            p[name] = (rng.standard_normal(shape) * np.sqrt(2.0 / fan[name])).astype(np.float32)
     if zero_last:
         p["W4"] = (rng.standard_normal(p["W4"].shape) * scale_last).astype(np.float32)
-    return np.concatenate([p[k].reshape(-1) for k, _ in param_shapes(dof, hidden)]).astype(np.float32)
+    theta = np.concatenate([p[k].reshape(-1) for k, _ in param_shapes(dof, hidden)]).astype(np.float32)
+    return mask_width(theta, dof, active, hidden) if active else theta

@@ -9,7 +9,8 @@ other orders than numpy, so
 * with a zero output layer the coarse proposal is the Levy bridge, and log q_old - log q_new must
   equal the change of the coarse kinetic action exactly in the mathematics: 1e-3 relative on the GPU.
 Statistics: on the C1 harmonic oscillator the move, alone and trained online, keeps <x^2> at the
-exact discretised value within 4 standard errors.
+exact discretised value within 4 standard errors.  The move's network runs 64 of the 128 hidden units
+(the others zero, which training must preserve bit for bit).
 """
 import numpy as np
 import pytest
@@ -90,7 +91,7 @@ def _run_case(case, theta_scale=None, seed=777, sweep=3):
     k = _kernel(system, W, N, beta)
     k.setPaths(paths0.reshape(W, D * N))
     lp = LearnedProposal(k, batch=2048, seed=seed, coarse=True)
-    theta = L.init_params(D, 5, scale_last=scale)
+    theta = L.init_params(D, 5, scale_last=scale, active=lp.width)
     lp.setParameters(theta)
     lp._sweepCount = sweep
     lp.traceSweeps()
@@ -160,6 +161,9 @@ def test_coarse_move_keeps_ho_energy(gpu):
     assert last < first
     assert 0.05 < ar < 1.0
     assert abs(E - exact) < 4 * se
+    # training kept the hidden units beyond the active width (64) at exactly zero
+    theta = lp.getParameters()
+    assert np.array_equal(theta, L.mask_width(theta, 1, lp.width)) and lp.width == 64
 
 
 def test_coarse_training_rows_match_oracle(gpu):
@@ -173,7 +177,7 @@ def test_coarse_training_rows_match_oracle(gpu):
     k.setPaths(paths.reshape(W, D * N))
     B = 4096
     lp = LearnedProposal(k, batch=B, seed=78, coarse=True)
-    theta = L.init_params(D, 1, scale_last=0.3)
+    theta = L.init_params(D, 1, scale_last=0.3, active=lp.width)
     lp.setParameters(theta)
     loss = lp.train(1)
     S = 9

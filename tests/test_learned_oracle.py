@@ -215,3 +215,29 @@ def test_coarse_poor_network_keeps_the_exact_law():
     assert 0.05 < ar < 0.95, ar
     assert np.abs(C - Cs).max() < 0.03 * Cs[0, 0]
     assert np.abs(X.mean(axis=0)).max() < 0.03 * np.sqrt(Cs[0, 0])
+
+
+def test_lane_scan_is_a_prefix_sum():
+    v = np.random.default_rng(1).standard_normal((3, 2, 64, 2)).astype(np.float32)
+    np.testing.assert_allclose(L.lane_scan(v), np.cumsum(v.astype(np.float64), axis=2), rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("active", [32, 64, 96])
+def test_inactive_units_get_zero_gradient(active):
+    """Units beyond the active width, zero in and out, stay exactly zero under training: their ReLU
+    masks stop the gradient of their incoming weights and biases, and their zero activations stop
+    that of their outgoing weights; the outputs equal those of the network cut to `active` units."""
+    D = 1
+    theta = L.init_params(D, 3, scale_last=0.3, active=active)
+    p = L.unflatten(theta, D)
+    rng = np.random.default_rng(0)
+    a, b, x = (rng.standard_normal((256, D)).astype(np.float32) for _ in range(3))
+    f = L.features(a, b, 2, 6, False, 1.0)
+    _, g = L.nll_and_grad(p, f, a, b, x, np.float32(0.3), D)
+    g = np.concatenate([g[k].reshape(-1) for k, _ in L.param_shapes(D)])
+    assert np.array_equal(L.mask_width(g, D, active), g)
+    cut = {k: (v[:active] if k in ("W1", "b1") else v[:active, :active] if k in ("W2", "W3")
+               else v[:active] if k in ("b2", "b3") else v[:, :active] if k == "W4" else v) for k, v in p.items()}
+    o_full, _ = L.mlp_forward(p, f)
+    o_cut, _ = L.mlp_forward(cut, f)
+    assert np.array_equal(o_full, o_cut)
