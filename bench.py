@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 N, S, W, BETA, DOF = 1024, 6, 16384, 10.0, 9
 HIST_MAX_A = 40.0  # pair-distance histogram: 256 bins on [0, 40) angstrom
 SWEEPS_PER_STEP = 32
+TAU_MEASUREMENTS = 2048  # E0 after every sweep, for the plain sampler's tau_int in sweeps
+# standard-error targets of the fixed-sigma comparison: small enough that sampling, not the learned
+# proposal's training, takes most of the time (the plain sampler needs about 3 s and 10 s)
+HO_SIGMA, AHO_SIGMA = 2e-5, 2e-5
 SEG = 1 << S
 UPDATES_PER_SWEEP = W * (N // SEG) * (SEG - 1)
 # algorithmic HBM bytes per slice-update: a segment reads 2^S + 1 slices and
@@ -334,30 +338,49 @@ def lp_bench(steps, warmup, spw=0):
 
 
 # ---- ESS/s of E0 at fixed sigma, learned vs plain, on the stationary workloads ----
+TAU_CHAINS = 512  # chains whose autocorrelation estimates tau_int in the stopping rule
+
+
 def _series_stats(series, exact):
+    """Statistics of an E0 series (torch tensor (measurements, chains) on the GPU): the standard error
+    of the grand mean from the spread of the chain means (chains are independent), ESS = variance /
+    SE^2, and Sokal's tau_int from the first TAU_CHAINS chains."""
     import numpy as np
-    from feynsimul_amd.stats import effective_sample_size
-    r = effective_sample_size(series)
-    x = np.asarray(series, dtype=np.float64)
-    return {"E0": r["mean"], "sigma": r["se"], "tau_int_measurements": r["tau_int"],
-            "measurements": int(x.shape[0]), "ess": float(x.var() / r["se"] ** 2),
-            "z": abs(r["mean"] - exact) / r["se"]}
+    from feynsimul_amd.stats import integrated_autocorr_time
+    x = series.double()
+    M, C = x.shape
+    means = x.mean(dim=0)
+    se = float(means.std() / np.sqrt(C))
+    mean = float(means.mean())
+    tau, _ = integrated_autocorr_time(series[:, :TAU_CHAINS].cpu().numpy())
+    return {"E0": mean, "sigma": se, "tau_int_measurements": tau, "measurements": int(M),
+            "ess": float(x.var() / se ** 2), "z": abs(mean - exact) / se}
 
 
-def run_to_sigma(block, get_series, sigma, exact, max_seconds, measurements_per_block):
+def run_to_sigma(block, get_series, sigma, exact, max_seconds, measurements_per_block, capacity):
     """Calls block() (one block of measurements on the GPU) until the chain-mean standard error of E0
     is at most sigma and Sokal's tau_int is at most 1/5 of the series, or max_seconds of sampling have
-    passed.  Wall time counts the blocks only (each ends synchronised), not the host analysis."""
+    passed, or the series buffer (capacity measurements) has no room for another block.  Wall time
+    counts the blocks only (each ends synchronised), not the host analysis."""
     import torch
     t = 0.0
+    n = 0
+    check = measurements_per_block
     while True:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         block()
         torch.cuda.synchronize()
         t += time.perf_counter() - t0
+        n += measurements_per_block
+        full = n + measurements_per_block > capacity
+        # the rule is checked each time the series has doubled (the host analysis stays a small share)
+        if n < check and not full and t <= max_seconds:
+            continue
+        check = 2 * n
         st = _series_stats(get_series(), exact)
-        if (st["sigma"] <= sigma and st["tau_int_measurements"] <= st["measurements"] / 5.0) or t > max_seconds:
+        if (st["sigma"] <= sigma and st["tau_int_measurements"] <= st["measurements"] / 5.0) or t > max_seconds \
+                or full:
             break
     st["seconds"] = t
     st["sigma_target"] = sigma
@@ -368,16 +391,27 @@ def run_to_sigma(block, get_series, sigma, exact, max_seconds, measurements_per_
     return st
 
 
-def fixed_sigma(workload, sigma, max_seconds=40.0, train_steps=100, S=None):
-    """Plain and learned samplers from the same equilibrated paths, each run to the same standard
-    error sigma of E0 (virial energy, slice mean, one measurement every 32 sweeps).  ESS/s counts the
-    learned proposal's online training.  ratio = ESS/s learned / ESS/s plain at that sigma."""
+# the learned leg of the fixed-sigma comparison: the whole-path move, its network on 32 hidden units,
+# trained online (Adam steps, each after two plain sweeps), then MOVES_PER_MEASUREMENT moves per E0
+FIXED_SIGMA_LEARNED = {"ho": {"train_steps": 300, "batch": 16384}, "aho": {"train_steps": 500, "batch": 65536}}
+LEARNED_WIDTH, MOVES_PER_MEASUREMENT = 32, 4
+
+
+def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
+    """Plain bisection and the learned whole-path move from the same equilibrated paths, each run to
+    the same standard error sigma of E0 (virial energy, slice mean; plain: one measurement every 32
+    sweeps, learned: one every MOVES_PER_MEASUREMENT moves).  ESS/s counts the learned proposal's
+    online training.  ratio = ESS/s learned / ESS/s plain at that sigma.
+    `ceiling`: the plain sampler's tau_int of E0 in sweeps (a measurement after every sweep) and, given
+    the measured HBM copy ceiling in GB/s, the largest ESS/s ratio over plain that any move replacing a
+    sweep could reach."""
     import numpy as np
     import torch
     from feynsimul_amd.kernel_args import KernelArgs
     from feynsimul_amd.kernel import PIMCKernel
     from feynsimul_amd.learned import LearnedProposal
     from feynsimul_amd.physical_systems import HarmOsc, AnharmOsc
+    from feynsimul_amd.stats import integrated_autocorr_time
     if workload == "ho":
         system, W, NN, beta = HarmOsc(), HO_W, HO_N, HO_BETA
         S = S or HO_S
@@ -403,47 +437,85 @@ def fixed_sigma(workload, sigma, max_seconds=40.0, train_steps=100, S=None):
     kb.run()
     start = kb.getPaths()
     del kb
-    cap = 4096 if W * 4096 * 4 <= (1 << 30) else 1024
+    # series buffers of at most 1 GiB
+    cap = min(65536, (1 << 30) // (W * 4)) // runs * runs
     # plain
     kp = kernel(SWEEPS_PER_STEP, runs, 42)
     kp.setPaths(start)
     kp.setSeriesCapacity(cap)
-    plain = run_to_sigma(lambda: kp.run(wait=False), lambda: kp.getSeries()[:, :, 0], sigma, exact,
-                         max_seconds, runs)
+    plain = run_to_sigma(lambda: kp.run(wait=False), lambda: kp.getSeriesTensor()[:, :, 0], sigma, exact,
+                         max_seconds, runs, cap)
     plain["acceptance"] = kp.getAcceptanceRate()
-    plain["ms_per_sweep"] = None
+    plain["ms_per_sweep"] = 1e3 * plain["seconds"] / (plain["measurements"] * SWEEPS_PER_STEP)
     del kp
     torch.cuda.empty_cache()
-    # learned: online training from the same paths, then learned sweeps
-    kl = kernel(SWEEPS_PER_STEP, runs, 43)
-    kl.setPaths(start)
-    lp = LearnedProposal(kl, batch=65536, lr=1e-3, seed=44)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    first = lp.train(1)
-    last = lp.trainOnline(train_steps - 1, plainSweepsPerStep=2)
-    torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
-    kl.setSeriesCapacity(cap)
-    learned = run_to_sigma(lambda: lp.run(runs, SWEEPS_PER_STEP), lambda: kl.getSeries()[:, :, 0], sigma, exact,
-                           max(max_seconds - t_train, 1.0), runs)
-    learned.update({"seconds_train": t_train, "adam_steps": train_steps, "nll_first": first, "nll_last": last,
-                    "acceptance": lp.lastAcceptance, "fallback_blocks": lp.fallbackBlocks})
-    del kl, lp
+    # E0 after every plain sweep: tau_int in sweeps is what a move that replaced the sweep and gave an
+    # independent E0 value every sweep (tau_int = 1) could gain per sweep
+    kt = kernel(1, runs, 45)
+    kt.setPaths(start)
+    kt.setSeriesCapacity(TAU_MEASUREMENTS)
+    for _ in range(TAU_MEASUREMENTS // runs):
+        kt.run(wait=False)
+    tau_sweeps, window = integrated_autocorr_time(kt.getSeries()[:, :, 0])
+    del kt
     torch.cuda.empty_cache()
-    t_plain = plain["seconds_to_sigma"]
-    t_learned = learned["seconds_to_sigma"] + t_train
-    for r, t in ((plain, t_plain), (learned, t_learned)):
-        r["ess_per_s"] = r["ess"] / t * (r["sigma"] / sigma) ** 2
-    # the ceiling if the network cost nothing: effective samples per sweep, learned over plain
-    per_sweep = [r["ess"] / (r["measurements"] * SWEEPS_PER_STEP) for r in (plain, learned)]
-    return {"workload": {"ho": "harmonic oscillator N=256, beta=10, 4096 chains",
+    ceiling = {"plain_tau_int_sweeps": tau_sweeps, "window_sweeps": window, "measurements": TAU_MEASUREMENTS,
+               "plain_ms_per_sweep": plain["ms_per_sweep"]}
+    if copy_GBps:
+        # such a move still reads and writes every slice once: at the measured HBM copy ceiling that takes
+        # path_ms, so its ESS/s can exceed the plain sampler's by at most tau_sweeps * plain_ms / path_ms
+        path_ms = 1e3 * 2 * W * NN * 4 / (copy_GBps * 1e9)
+        ceiling.update({"path_read_write_ms_at_copy_ceiling": path_ms, "copy_ceiling_GBps": copy_GBps,
+                        "max_ess_per_s_ratio_any_sweep_move": tau_sweeps * plain["ms_per_sweep"] / path_ms})
+    # learned: the whole-path move, trained online from the same paths
+    cfg = FIXED_SIGMA_LEARNED[workload]
+    MPM = MOVES_PER_MEASUREMENT
+
+    def learned_leg(width):
+        kl = kernel(SWEEPS_PER_STEP, runs, 43)
+        kl.setPaths(start)
+        lp = LearnedProposal(kl, batch=cfg["batch"], lr=1e-3, seed=44, coarse=True, width=width)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        first = lp.train(1)
+        last = lp.trainOnline(cfg["train_steps"] - 1, plainSweepsPerStep=2)
+        torch.cuda.synchronize()
+        t_train = time.perf_counter() - t0
+        kl.setSeriesCapacity(cap)
+        r = run_to_sigma(lambda: lp.run(runs, MPM), lambda: kl.getSeriesTensor()[:, :, 0], sigma, exact,
+                         max(max_seconds - t_train, 1.0), runs, cap)
+        r.update({"seconds_train": t_train, "adam_steps": cfg["train_steps"], "training_batch": cfg["batch"],
+                  "nll_first": first, "nll_last": last, "acceptance": lp.lastAcceptance,
+                  "fallback_blocks": lp.fallbackBlocks, "hidden_units": width, "moves_per_measurement": MPM,
+                  "ms_per_move": 1e3 * r["seconds"] / (r["measurements"] * MPM)})
+        r["ess_per_s"] = r["ess"] / (r["seconds_to_sigma"] + t_train) * (r["sigma"] / sigma) ** 2
+        r["ess_per_s_without_training"] = r["ess"] / r["seconds_to_sigma"] * (r["sigma"] / sigma) ** 2
+        del kl, lp
+        torch.cuda.empty_cache()
+        return r
+
+    learned = learned_leg(LEARNED_WIDTH)
+    plain["ess_per_s"] = plain["ess"] / plain["seconds_to_sigma"] * (plain["sigma"] / sigma) ** 2
+    # the full 128-unit network of the segment proposal, for comparison (the anharmonic workload)
+    full = learned_leg(128) if workload == "aho" else None
+    # per unit of work: effective samples per learned move over effective samples per plain sweep
+    per = [plain["ess"] / (plain["measurements"] * SWEEPS_PER_STEP), learned["ess"] / (learned["measurements"] * MPM)]
+    out = {"workload": {"ho": "harmonic oscillator N=256, beta=10, 4096 chains",
                          "aho": "anharmonic oscillator x^2/2 + 0.1 x^4, N=4096, beta=40, 65536 chains"}[workload]
-                        + ", S=%d, E0 every 32 sweeps" % S,
+                        + ", plain S=%d, E0 every 32 sweeps" % S,
+            "learned_form": "whole-path move: network on the nodes every 64 slices (%d hidden units), Levy "
+                            "bridges between them, one MH decision per path, E0 every %d moves" % (LEARNED_WIDTH, MPM),
             "sigma": sigma, "E0_exact": exact, "plain": plain, "learned": learned,
             "ratio_learned_over_plain": learned["ess_per_s"] / plain["ess_per_s"],
-            "ess_per_sweep_ratio_learned_over_plain": per_sweep[1] / per_sweep[0],
+            "ratio_without_training": learned["ess_per_s_without_training"] / plain["ess_per_s"],
+            "ess_per_move_over_ess_per_plain_sweep": per[1] / per[0],
+            "ceiling": ceiling,
             "bias_within_2_sigma": {"plain": plain["z"] <= 2.0, "learned": learned["z"] <= 2.0}}
+    if full is not None:
+        out["learned_128_units"] = full
+        out["ratio_learned_128_units_over_plain"] = full["ess_per_s"] / plain["ess_per_s"]
+        out["bias_within_2_sigma"]["learned_128_units"] = full["z"] <= 2.0
+    return out
 
 
 RAGGED_PROFILE = os.path.join(ROOT, "profiles", "fs_batch_sweep_ragged.md")
@@ -642,7 +714,9 @@ def aho_bench(steps, warmup, cpu=True, ess=True):
         return line
     # ESS/s of E0 at fixed sigma, plain vs learned, at the configured S = 4 and at S = 6
     progress("ESS/s at fixed sigma: S=4 and S=6")
-    line["ess_at_fixed_sigma"] = {"S4": fixed_sigma("aho", 2e-4, S=4), "S6": fixed_sigma("aho", 2e-4, S=6)}
+    copy = hbm_ceiling(1, 3)["ceilings"]["copy"]["GBps"]
+    # plain at S = 6, the level that decorrelates E0 fastest (tau_int 17 sweeps against 210 at S = 4)
+    line["ess_at_fixed_sigma"] = {"S6": fixed_sigma("aho", AHO_SIGMA, S=6, copy_GBps=copy)}
     if cpu and int(os.environ.get("RANK", "0")) == 0:
         line["cpu_baseline"] = cpu_baseline_aho()
     return line
@@ -777,7 +851,7 @@ def ho_bench(steps, warmup, cpu=True, ess=True):
             "valu": dict(valu_fields(32 * updates / (ms * 1e-3), HO_PROFILE), ops_per_update=32),
             "acceptance": k.getAcceptanceRate()}
     if ess:
-        line["ess_at_fixed_sigma"] = fixed_sigma("ho", 1e-4)
+        line["ess_at_fixed_sigma"] = fixed_sigma("ho", HO_SIGMA)
     if cpu and int(os.environ.get("RANK", "0")) == 0:
         line["cpu_baseline"] = cpu_baseline_ho()
     return line
@@ -905,7 +979,7 @@ def main():
                          "ho: harmonic oscillator N=256, beta=10, 4096 chains; lp: the learned sweep alone on He3")
     ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the E0 series")
     ap.add_argument("--ess-launches", type=int, default=64, help="E0 series launches of 16 x 32 sweeps")
-    ap.add_argument("--aho-sigma", type=float, default=2e-4, help="standard error target of E0 on the anharmonic "
+    ap.add_argument("--aho-sigma", type=float, default=AHO_SIGMA, help="standard error target of E0 on the anharmonic "
                                                                    "oscillator in the fixed-sigma comparison")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal phase")
     ap.add_argument("--learned-train", type=int, default=200, help="online Adam steps before learned sampling")
@@ -1068,9 +1142,10 @@ def main():
             progress("He3 learned proposal")
             line["learned"] = learned_phase(ess_phase.start_paths, args.learned_train, args.learned_measurements)
         progress("ESS/s at fixed sigma: harmonic oscillator")
-        fs = {"ho": fixed_sigma("ho", 1e-4)}
+        # the harmonic paths (4 MB) stay in cache, so the HBM copy ceiling bounds only the anharmonic case
+        fs = {"ho": fixed_sigma("ho", HO_SIGMA)}
         progress("ESS/s at fixed sigma: anharmonic oscillator, S=6")
-        fs["aho_S6"] = fixed_sigma("aho", args.aho_sigma, S=6)
+        fs["aho_S6"] = fixed_sigma("aho", args.aho_sigma, S=6, copy_GBps=ceil["ceilings"]["copy"]["GBps"])
         line["ess_at_fixed_sigma"] = fs
     if not args.no_cpu and int(os.environ.get("RANK", "0")) == 0:
         progress("CPU baseline")

@@ -527,6 +527,12 @@ class PIMCKernel:
             raise Exception("call setSeriesCapacity first")
         return self._series[: self._seriesCount].cpu().numpy()
 
+    def getSeriesTensor(self):
+        """The stored measurements as a device tensor (measurements, walkers, nOps), without a copy."""
+        if self._series is None:
+            raise Exception("call setSeriesCapacity first")
+        return self._series[: self._seriesCount]
+
     def getCorrelator(self):
         """(mean, standard error) of the correlators, each (nCorr, lags) (kernel.py:608-617)."""
         if not self._enableCorrelator:
