@@ -337,6 +337,54 @@ def lp_bench(steps, warmup, spw=0):
                          "updates_per_launch": UPDATES_PER_SWEEP, "traffic": None}}
 
 
+def lpc_bench(steps, warmup, width=None, train_steps=500):
+    """The learned whole-path move alone on the anharmonic workload (x^2/2 + 0.1 x^4, N = 4096,
+    beta = 40, 65536 chains): `steps` moves timed with HIP events on the kernel's stream after
+    `warmup`, from paths 2048 plain sweeps (S = 6) into the run, with the network trained online for
+    `train_steps` Adam steps.  One move = the network kernel on the coarse nodes (fs_lp_sweep) and
+    the fine pass (fs_lpc_fine, Levy bridges, decision, write)."""
+    import numpy as np
+    import torch
+    from feynsimul_amd.kernel_args import KernelArgs
+    from feynsimul_amd.kernel import PIMCKernel
+    from feynsimul_amd.learned import LearnedProposal
+    from feynsimul_amd.physical_systems import AnharmOsc
+    width = width or LEARNED_WIDTH
+    system = AnharmOsc(AHO_LAM)
+    k = PIMCKernel(KernelArgs(system=system, nbrOfWalkers=AHO_W, N=AHO_N, beta=AHO_BETA, S=6, enableOperator=True,
+                              enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                              enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                              enableGlobalOldPath=False, enableBins=False, metroStepsPerOperatorRun=2048,
+                              operatorRuns=1, nbrOfWalkersPerWorkGroup=1, operators=(system.energyOp,), seed=41,
+                              sampler="wave", pathStorage="memory"))
+    k.setPaths(np.random.default_rng(40).uniform(-1, 1, (AHO_W, AHO_N)).astype(np.float32))
+    k.run()
+    lp = LearnedProposal(k, batch=65536, lr=1e-3, seed=44, coarse=True, width=width)
+    lp.trainOnline(train_steps, plainSweepsPerStep=2)
+    lp.sweep(max(warmup, 1))
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(k._stream)
+    lp.sweep(steps, wait=False)
+    ev1.record(k._stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / steps
+    ar = lp.lastAcceptanceOfQueued(steps)
+    slices = AHO_W * AHO_N
+    # HBM bytes the move needs: every slice read, the accepted paths written
+    algo = 4.0 * slices * (1.0 + ar)
+    return {"metric": "learned whole-path moves: slice-updates/s (fs_lp_sweep + fs_lpc_fine)",
+            "value": slices / (ms * 1e-3), "unit": "slice-updates/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "bf16/f32", "data": "synthetic: anharmonic-oscillator paths after 2048 plain sweeps",
+            "config": {"workload": "anharmonic oscillator x^2/2 + 0.1 x^4, N=4096, beta=40, 65536 chains, "
+                                   "one whole-path move per step, %d hidden units, %d Adam steps" % (width, train_steps)},
+            "acceptance": ar,
+            "roofline": {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": "fs_lp_sweep + fs_lpc_fine",
+                         "avg_launch_ms": ms, "bytes_per_move": algo, "traffic": None}}
+
+
 # ---- ESS/s of E0 at fixed sigma, learned vs plain, on the stationary workloads ----
 TAU_CHAINS = 512  # chains whose autocorrelation estimates tau_int in the stopping rule
 
@@ -971,12 +1019,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ess", action="store_true", help="skip the ESS/s phase")
-    ap.add_argument("--workload", choices=["he3", "ragged", "aho", "rng", "ho", "lp"], default="he3",
+    ap.add_argument("--workload", choices=["he3", "ragged", "aho", "rng", "ho", "lp", "lpc"], default="he3",
                     help="he3: the headline workload; ragged: LM2M2 n in {3,4,6,8} x N in {256..2048}, "
                          "2048 chains each, in one autotuned persistent launch; aho: anharmonic oscillator "
                          "N=4096, beta=40, 65536 chains, 512 correlator lags, xorshift128 and RANLUX; "
                          "rng: stream throughput of every generator, as the reference's RNG benchmark; "
-                         "ho: harmonic oscillator N=256, beta=10, 4096 chains; lp: the learned sweep alone on He3")
+                         "ho: harmonic oscillator N=256, beta=10, 4096 chains; lp: the learned sweep alone on He3; "
+                         "lpc: the learned whole-path move alone on the anharmonic oscillator")
     ap.add_argument("--ess-burn", type=int, default=24, help="burn-in launches of 512 sweeps before the E0 series")
     ap.add_argument("--ess-launches", type=int, default=64, help="E0 series launches of 16 x 32 sweeps")
     ap.add_argument("--aho-sigma", type=float, default=AHO_SIGMA, help="standard error target of E0 on the anharmonic "
@@ -1052,6 +1101,9 @@ def main():
         return
     if args.workload == "lp":
         print(json.dumps(lp_bench(args.steps, args.warmup, args.lp_spw)), flush=True)
+        return
+    if args.workload == "lpc":
+        print(json.dumps(lpc_bench(args.steps, args.warmup)), flush=True)
         return
     if args.workload == "rng":
         print(json.dumps(rng_bench()), flush=True)
