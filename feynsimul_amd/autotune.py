@@ -1,15 +1,18 @@
 """Launch-shape autotuner.
 
 autotune_batch: every candidate (wavefronts per workgroup) x (dispatch order)
-for a list of KernelArgs in one persistent launch.  autotune_levels: the
-bisection level S of every member of a ragged batch, from acceptance rates
-measured on the device (the reference's run loop adapts S to keep the
-acceptance between 20 and 50 %, FeynSimul/pimc_utils.py:270-279).  autotune_kernel: every
-(wavefronts per workgroup) x (wavefronts per SIMD cap) for one wave-sampler
-kernel.  Waves per workgroup fixes the LDS layout and the launch bounds, so
-each candidate value is a separate compile; the dispatch order and the
-occupancy cap are run-time choices.  The fastest is kept; results are cached
-per configuration and device in a JSON file ($FEYNSIMUL_AMD_CACHE, default
+x (register classes) for a list of KernelArgs in one persistent launch.
+autotune_levels: the bisection level S of every member of a ragged batch, from
+acceptance rates measured on the device (the reference's run loop adapts S to
+keep the acceptance between 20 and 50 %, FeynSimul/pimc_utils.py:270-279).
+autotune_kernel: lanes per chain for one kernel (a wavefront per chain, one
+work-item per segment, or one per chain) and, for the wave sampler, every
+(wavefronts per workgroup) x (wavefronts per SIMD cap).  A batch's members are
+wave-sampler specialisations of one module, so their lanes per chain is 64.
+Waves per workgroup fixes the LDS layout and the launch bounds, so each
+candidate value is a separate compile; the dispatch order and the occupancy
+cap are run-time choices.  The fastest is kept; results are cached per
+configuration and device in a JSON file ($FEYNSIMUL_AMD_CACHE, default
 ~/.cache/feynsimul_amd/autotune.json).
 """
 import hashlib
@@ -61,38 +64,60 @@ def _cache_store(cache, key, best, trials):
 
 
 def autotune_kernel(kernelArgs, initialPaths=None, wavesPerBlock=(2, 4, 8), wavesPerSimd=(0, 8, 6, 5, 4, 3),
-                    launches=3, use_cache=True, verbose=False):
-    """Returns (best, trials) for one wave-sampler KernelArgs: best = {"wavesPerBlock", "wavesPerSimd",
-    "ms_per_launch"}.  Each trial is one warm-up launch and `launches` timed launches from
-    `initialPaths` (or the kernel's default paths); the occupancy cap does not change results."""
+                    launches=3, use_cache=True, verbose=False, lanesPerChain=(64,)):
+    """Returns (best, trials) for one KernelArgs: best = {"lanesPerChain", "wavesPerBlock", "wavesPerSimd",
+    "ms_per_launch"}.  lanesPerChain candidates: 64, the wave sampler (a wavefront per chain, a slice per
+    lane), timed for every (waves per workgroup, occupancy cap) pair; "segments", the reference-order
+    sampler with one work-item per segment (N / 2^S lanes per chain, the reference's ParallelizePath);
+    1, the reference-order sampler with one work-item per chain.  Each trial is one warm-up launch and
+    `launches` timed launches from `initialPaths` (or the kernel's default paths); neither the occupancy
+    cap nor the choice of sampler changes the sampled distribution."""
     import copy
     import torch
     from .kernel import PIMCKernel
     dev = torch.cuda.get_device_properties(torch.cuda.current_device()).name
-    key = "kernel:" + config_key([kernelArgs], dev)
+    key = "kernel:" + config_key([kernelArgs], dev) + ":" + ",".join(str(x) for x in lanesPerChain)
     cache, hit = _cache_load(key, use_cache)
     if hit is not None:
         return hit["best"], hit["trials"]
     trials = []
-    for wpb in wavesPerBlock:
-        ka = copy.copy(kernelArgs)
-        ka.wavesPerBlock, ka.wavesPerSimd = wpb, None
-        k = PIMCKernel(ka)
-        for cap in wavesPerSimd:
-            if initialPaths is not None:
-                k.setPaths(initialPaths)
-            k.setWavesPerSimd(cap)
-            k.run()
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(launches):
-                k.run(wait=False)
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t0) / launches * 1e3
-            trials.append({"wavesPerBlock": wpb, "wavesPerSimd": cap, "ms_per_launch": ms})
+
+    def timed(k):
+        if initialPaths is not None:
+            k.setPaths(initialPaths)
+        k.run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(launches):
+            k.run(wait=False)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / launches * 1e3
+
+    for lanes in lanesPerChain:
+        if lanes != 64:
+            ka = copy.copy(kernelArgs)
+            ka.sampler = "exact"
+            ka.enableParallelizePath = lanes == "segments"
+            k = PIMCKernel(ka)
+            ms = timed(k)
+            trials.append({"lanesPerChain": lanes if lanes == 1 else int(ka.N) >> int(ka.S), "sampler": "exact",
+                           "wavesPerBlock": None, "wavesPerSimd": None, "ms_per_launch": ms})
             if verbose:
-                print("autotune: waves/workgroup %d, waves/SIMD cap %d: %.3f ms" % (wpb, cap, ms), flush=True)
-        del k
+                print("autotune: reference-order sampler, %s lanes per chain: %.3f ms" % (lanes, ms), flush=True)
+            del k
+            continue
+        for wpb in wavesPerBlock:
+            ka = copy.copy(kernelArgs)
+            ka.wavesPerBlock, ka.wavesPerSimd, ka.sampler = wpb, None, "wave"
+            k = PIMCKernel(ka)
+            for cap in wavesPerSimd:
+                k.setWavesPerSimd(cap)
+                ms = timed(k)
+                trials.append({"lanesPerChain": 64, "sampler": "wave", "wavesPerBlock": wpb, "wavesPerSimd": cap,
+                               "ms_per_launch": ms})
+                if verbose:
+                    print("autotune: waves/workgroup %d, waves/SIMD cap %d: %.3f ms" % (wpb, cap, ms), flush=True)
+            del k
     best = min(trials, key=lambda t: t["ms_per_launch"])
     if use_cache:
         _cache_store(cache, key, best, trials)

@@ -650,9 +650,12 @@ FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long
                         }
                     }
                 }
+                // a walker's rows of this level within one 32-row group: nn, or 32 when its rows span groups
+                // (levels 7 and deeper, more than 64 coarse nodes); each group adds its own part
+                const int span = nn < 32 ? nn : 32;
                 contrib += __shfl_xor(contrib, 32, 64);
-                for (int o = 1; o < nn; o <<= 1) contrib += __shfl_xor(contrib, o, 64);
-                if (valid && g == 0 && (c32 & (nn - 1)) == 0) lq[(old ? WPW : 0) + k] += contrib;
+                for (int o = 1; o < span; o <<= 1) contrib += __shfl_xor(contrib, o, 64);
+                if (valid && g == 0 && (c32 & (span - 1)) == 0) lq[(old ? WPW : 0) + k] += contrib;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();

@@ -238,7 +238,8 @@ def test_ragged_lm2m2_batch_full_size(gpu):
 
 def test_occupancy_cap_keeps_results_and_autotune_kernel(gpu):
     """The occupancy cap only pads the launch's LDS request: paths, operators and streams are bit-identical
-    with and without it; autotune_kernel times every (waves per workgroup, cap) pair."""
+    with and without it; autotune_kernel times every (waves per workgroup, cap) pair and the reference-order
+    sampler at one work-item per segment and per chain (the lanes-per-chain choice)."""
     from feynsimul_amd.autotune import autotune_kernel
     _case("occupancy cap + autotune_kernel")
     system = AnharmOsc(0.1)
@@ -255,9 +256,13 @@ def test_occupancy_cap_keeps_results_and_autotune_kernel(gpu):
     np.testing.assert_array_equal(out[0][1], out[1][1])
     np.testing.assert_array_equal(out[0][2], out[1][2])
     best, trials = autotune_kernel(_ka(system, W, N, S, 20.0, 1, 8, seed=5), initialPaths=p0, wavesPerBlock=(2, 4),
-                                   wavesPerSimd=(0, 4), launches=1, use_cache=False)
-    assert len(trials) == 4 and best in trials
+                                   wavesPerSimd=(0, 4), launches=1, use_cache=False, lanesPerChain=(64, "segments", 1))
+    assert len(trials) == 6 and best in trials
     assert all(t["ms_per_launch"] > 0 for t in trials)
+    # lanes per chain: a wavefront (the wave sampler), one work-item per segment (N / 2^S = 64), one per chain
+    assert sorted({t["lanesPerChain"] for t in trials}) == [1, 64]
+    assert [t["sampler"] for t in trials].count("exact") == 2
+    print("lanes per chain: %s" % [(t["lanesPerChain"], t["sampler"], round(t["ms_per_launch"], 3)) for t in trials])
 
 
 def test_autotune_levels_follows_the_acceptance_window(gpu):

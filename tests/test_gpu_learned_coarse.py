@@ -40,14 +40,13 @@ def _kernel(system, W, N, beta, opRuns=1, M=1, seed=5, S=6):
     return PIMCKernel(ka)
 
 
-def _ho_cov(N, beta, k=1.0):
+def _ho_paths(W, N, beta, rng):
+    """Paths from the exact Gaussian of the discretised harmonic action: its matrix is circulant with
+    eigenvalues (2 - 2 cos(2 pi k / N)) / eps + eps, so x = F^-1 (F z / sqrt(lambda)) for white z."""
     eps = beta / N
-    A = np.zeros((N, N))
-    for i in range(N):
-        A[i, i] = 2 / eps + eps * k
-        A[i, (i + 1) % N] -= 1 / eps
-        A[i, (i - 1) % N] -= 1 / eps
-    return np.linalg.inv(A)
+    lam = (2.0 - 2.0 * np.cos(2.0 * np.pi * np.arange(N) / N)) / eps + eps
+    z = rng.standard_normal((W, N))
+    return np.fft.ifft(np.fft.fft(z, axis=1) / np.sqrt(lam), axis=1).real.astype(np.float32)
 
 
 def _poly_V(lam):
@@ -68,16 +67,15 @@ def _lm2m2_V(system):
 
 
 def _setup(case, rng):
-    if case in ("ho128", "ho256", "aho512"):
-        N, beta = {"ho128": (128, 2.0), "ho256": (256, 10.0), "aho512": (512, 8.0)}[case]
+    if case in ("ho128", "ho256", "aho512", "ho8192"):
+        N, beta = {"ho128": (128, 2.0), "ho256": (256, 10.0), "aho512": (512, 8.0), "ho8192": (8192, 4.0)}[case]
         system = HarmOsc() if case.startswith("ho") else AnharmOsc(0.1)
         W, D = 64, 1
-        # Gaussian paths of the discretised harmonic action
-        chol = np.linalg.cholesky(_ho_cov(N, beta))
-        paths0 = (rng.standard_normal((W, N)) @ chol.T).astype(np.float32)[:, None, :]
+        paths0 = _ho_paths(W, N, beta, rng)[:, None, :]
         V = _poly_V(0.0 if case.startswith("ho") else 0.1)
         return system, W, N, beta, D, paths0, V, 0.03
-    system, W, N, beta, D = Lm2m2Cluster(3), 16, 128, 1.0, 9
+    N = 512 if case == "he3_512" else 128
+    system, W, beta, D = Lm2m2Cluster(3), 16, 1.0 * N / 128, 9
     base = system.initialPositions(W, rng)
     paths0 = (base[:, :, None] + 0.02 * rng.standard_normal((W, D, N))).astype(np.float32)
     return system, W, N, beta, D, paths0, _lm2m2_V(system), 0.05
@@ -104,7 +102,9 @@ def _run_case(case, theta_scale=None, seed=777, sweep=3):
     return paths0, got, ref, tr, d
 
 
-@pytest.mark.parametrize("case", ["ho128", "ho256", "aho512", "he3"])
+# ho8192 (128 intervals) and he3_512 (8 intervals x 9 DOF) exceed 64 values per lane: their accepted bridges are
+# drawn a second time from the same counters instead of being held in registers
+@pytest.mark.parametrize("case", ["ho128", "ho256", "aho512", "he3", "ho8192", "he3_512"])
 def test_coarse_move_matches_oracle(gpu, case):
     _case("whole-path move %s" % case)
     paths0, got, ref, tr, d = _run_case(case)
@@ -121,7 +121,7 @@ def test_coarse_move_matches_oracle(gpu, case):
     assert np.array_equal(changed[clear], d["accept"][clear])
     for w in np.nonzero(changed & d["accept"])[0]:
         assert np.abs(got[w] - ref[w]).max() < 5e-3
-    if case in ("ho128", "aho512"):
+    if case in ("ho128", "aho512", "ho8192"):
         assert 0 < d["accept"].sum() < W
 
 
