@@ -662,6 +662,11 @@ struct fs_batch {
     // dispatch tables of plain launches and the host image last uploaded
     char *d_tables = nullptr;
     std::vector<char> h_tables;
+    // register classes after the first run on streams of their own (default priority), forked from and joined
+    // back to the launch stream with events, so one class's blocks fill the tail of another's
+    std::vector<hipStream_t> side;
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
 };
 
 struct fs_graph {
@@ -813,7 +818,11 @@ int batch_launch(fs_batch *b, char *tables, int32_t operator_runs, int32_t metro
     int nk = b->nk;
     void *rargs[] = {&counter, &nk};
     FS_HIP(hipModuleLaunchKernel(b->fn_reset, 1, 1, 1, 64, 1, 1, 0, st, rargs, nullptr));
+    const bool concurrent = !b->side.empty();
+    if (concurrent) FS_HIP(hipEventRecord(b->ev_fork, st));
     for (int j = 0; j < b->nk; j++) {
+        hipStream_t sj = concurrent && j > 0 ? b->side[j - 1] : st;
+        if (sj != st) FS_HIP(hipStreamWaitEvent(sj, b->ev_fork, 0));
         fs_batch_args ba{};
         ba.types = reinterpret_cast<const fs_kargs *>(tables + L.types());
         ba.start = reinterpret_cast<const long long *>(tables + L.start(j));
@@ -825,15 +834,21 @@ int batch_launch(fs_batch *b, char *tables, int32_t operator_runs, int32_t metro
         ba.seriesOffset = series_offset;
         void *args[] = {&ba};
         const unsigned grid = (unsigned)(grid_blocks > 0 ? grid_blocks : b->max_blocks[j]);
-        FS_HIP(hipModuleLaunchKernel(b->fn_sweep[j], grid, 1, 1, 64 * b->wpb, 1, 1, (unsigned)b->smem, st, args,
+        FS_HIP(hipModuleLaunchKernel(b->fn_sweep[j], grid, 1, 1, 64 * b->wpb, 1, 1, (unsigned)b->smem, sj, args,
                                      nullptr));
+        if (sj != st) FS_HIP(hipEventRecord(b->ev_join[j - 1], sj));
     }
+    if (concurrent)
+        for (int j = 1; j < b->nk; j++) FS_HIP(hipStreamWaitEvent(st, b->ev_join[j - 1], 0));
     return FS_OK;
 }
 
 void batch_free(fs_batch *b)
 {
     if (!b) return;
+    for (hipStream_t s : b->side) (void)hipStreamDestroy(s);
+    for (hipEvent_t ev : b->ev_join) (void)hipEventDestroy(ev);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
     if (b->d_tables) (void)hipFree(b->d_tables);
     if (b->module) (void)hipModuleUnload(b->module);
     delete b;
@@ -944,6 +959,19 @@ int fs_batch_create(const fs_config *cfgs, const char *const *system_srcs, int32
             if (e == hipSuccess)
                 e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, b->fn_sweep[j], 64 * wpb, b->smem);
             b->max_blocks[j] = std::max(1, per_cu) * std::max(1, cus);
+        }
+        // FS_BATCH_SERIAL=1: the register classes run one after the other on the launch stream
+        const char *serial = getenv("FS_BATCH_SERIAL");
+        if (e == hipSuccess && nk > 1 && !(serial && serial[0] == '1')) {
+            e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+            for (int j = 1; j < nk && e == hipSuccess; j++) {
+                hipStream_t s = nullptr;
+                hipEvent_t ev = nullptr;
+                e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+                if (e == hipSuccess) b->side.push_back(s);
+                if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+                if (e == hipSuccess) b->ev_join.push_back(ev);
+            }
         }
         if (e != hipSuccess) {
             batch_free(b);
