@@ -301,3 +301,19 @@ def test_occupancy_cap_range():
     with pytest.raises(_native.FsError, match="max_waves_per_simd"):
         _native.Engine(_cfg(HarmOsc(), 64, 3, 1, max_waves_per_simd=20), system_section(HarmOsc(), ("x1",)),
                        compile_only=True)
+
+
+@pytest.mark.parametrize("N, D, width, needle", [(96, 1, 32, "power of 2"), (64, 1, 32, "N = 64 x 2^k"),
+                                                 (65536, 9, 32, "coarse values per walker"),
+                                                 (256, 1, 48, "multiple of 32")])
+def test_whole_path_move_rejects_bad_shapes(N, D, width, needle):
+    """The whole-path move refuses, before compiling, paths that are not 64 x 2^k slices, coarse node sets
+    that would not fit the fine pass's LDS, and an active width that is not a multiple of 32."""
+    from feynsimul_amd import _native
+    system = HarmOsc() if D == 1 else Lm2m2Cluster(3)
+    c = _cfg(system, N, 4, _native.FS_SAMPLER_WAVE)
+    lp = _native.FsLpConfig()
+    lp.abi_version, lp.atoms3d, lp.batch, lp.in_scale = _native.FS_ABI_VERSION, int(D == 9), 2048, 1.0
+    lp.lr, lp.beta1, lp.beta2, lp.adam_eps, lp.seed, lp.coarse, lp.width = 1e-3, 0.9, 0.999, 1e-8, 1, 1, width
+    with pytest.raises(Exception, match=needle.replace("^", r"\^")):
+        _native.Proposal(c, "", lp, compile_only=True)
