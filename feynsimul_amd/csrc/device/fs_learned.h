@@ -62,6 +62,20 @@ typedef __attribute__((ext_vector_type(16))) float fs_f16v;
 #endif
 #define FS_LP_OB (FS_LP_ACT / 32)  // 32-unit output blocks of a hidden layer
 #define FS_LP_KS (FS_LP_ACT / 16)  // 16-unit k-steps of the layer after it
+// A block's LDS copy of the image holds only the active units: W1p [OB][2 ks], W2p and W3p
+// [OB][KS], W4p [KS] (64 lanes x 8 BF16 = 1024 bytes per block entry), then the biases b1, b2, b3
+// (FS_LP_ACT floats each) and b4p (32 floats).  At the full width this is the global layout.
+#define FS_LP_L_W1 0
+#define FS_LP_L_W2 (FS_LP_L_W1 + FS_LP_OB * 2 * 1024)
+#define FS_LP_L_W3 (FS_LP_L_W2 + FS_LP_OB * FS_LP_KS * 1024)
+#define FS_LP_L_W4 (FS_LP_L_W3 + FS_LP_OB * FS_LP_KS * 1024)
+#define FS_LP_L_B1 (FS_LP_L_W4 + FS_LP_KS * 1024)
+#define FS_LP_L_B2 (FS_LP_L_B1 + 4 * FS_LP_ACT)
+#define FS_LP_L_B3 (FS_LP_L_B2 + 4 * FS_LP_ACT)
+#define FS_LP_L_B4 (FS_LP_L_B3 + 4 * FS_LP_ACT)
+#define FS_LP_L_BYTES FS_LP_LDS_IMG_BYTES(FS_LP_ACT)
+static_assert(FS_LP_L_B4 + 128 == FS_LP_L_BYTES, "LDS image layout");
+static_assert(FS_LP_ACT % 32 == 0 && FS_LP_ACT <= FS_LP_HID, "active width");
 #define FS_LP_SMAX 3.0f
 #define FS_LP_TAG_NORMAL 0x100u
 #define FS_LP_TAG_ACCEPT 0x200u
@@ -106,6 +120,31 @@ FS_DEV float fs_lp_normal(uint4 x, int pair, int t)
 }
 
 FS_DEV fs_f16v fs_lp_mfma(fs_bf8 a, fs_bf8 b, fs_f16v c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+
+// Copy the active part of the global image (FS_LP_IMG_BYTES, full-width layout) into the block's
+// LDS (FS_LP_L_BYTES, the FS_LP_L_* layout), 16 bytes per step; the caller synchronises.
+FS_DEV void fs_lp_load_image(const char *img_g, char *lds)
+{
+    const uint4 *src = reinterpret_cast<const uint4 *>(img_g);
+    uint4 *dst = reinterpret_cast<uint4 *>(lds);
+    constexpr int W1 = FS_LP_OB * 2 * 64, WH = FS_LP_OB * FS_LP_KS * 64, W4 = FS_LP_KS * 64, BC = FS_LP_ACT / 4;
+    for (int i = threadIdx.x; i < FS_LP_L_BYTES / 16; i += blockDim.x) {
+        int c = i, s;
+        if (FS_LP_ACT == FS_LP_HID || c < W1) {
+            s = c;  // W1p [ob][ks]: the active output blocks come first
+        } else if ((c -= W1) < 2 * WH) {
+            const int l = c >= WH, r = c - l * WH, q = r >> 6;
+            s = (l ? FS_LP_OFF_W3 : FS_LP_OFF_W2) / 16 + ((q / FS_LP_KS) * 8 + q % FS_LP_KS) * 64 + (r & 63);
+        } else if ((c -= 2 * WH) < W4) {
+            s = FS_LP_OFF_W4 / 16 + c;
+        } else if ((c -= W4) < 3 * BC) {
+            s = FS_LP_OFF_B1 / 16 + (c / BC) * (4 * FS_LP_HID / 16) + c % BC;
+        } else {
+            s = FS_LP_OFF_B4 / 16 + (c - 3 * BC);
+        }
+        dst[i] = src[s];
+    }
+}
 
 // ---- the network on a 64-row tile held by one wavefront ----
 // xin[ks][cb]: layer-1 B operand (inputs ks*16 + 8g + e of row cb*32 + lane%32).
@@ -168,7 +207,7 @@ FS_DEV void fs_lp_hidden(const char *img, int woff, int boff, const fs_bf8 (&hin
     for (int ks = 0; ks < FS_LP_KS; ks++)
 #pragma unroll
         for (int ob = 0; ob < FS_LP_OB; ob++) {
-            const fs_bf8 A = W[(ob * 8 + ks) * 64 + lane];
+            const fs_bf8 A = W[(ob * FS_LP_KS + ks) * 64 + lane];
 #pragma unroll
             for (int cb = 0; cb < FS_LP_CB; cb++) acc[ob][cb] = fs_lp_mfma(A, hin[ks][cb], acc[ob][cb]);
         }
@@ -179,8 +218,8 @@ FS_DEV void fs_lp_mlp(const char *img, const fs_bf8 (&xin)[2][FS_LP_CB], fs_f16v
     const int g = lane >> 5;
     fs_f16v acc[4][FS_LP_CB];
     {
-        const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_OFF_W1);
-        const float *b = reinterpret_cast<const float *>(img + FS_LP_OFF_B1);
+        const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_L_W1);
+        const float *b = reinterpret_cast<const float *>(img + FS_LP_L_B1);
 #pragma unroll
         for (int ob = 0; ob < FS_LP_OB; ob++) fs_lp_bias_init(b, ob, g, acc[ob]);
 #pragma unroll
@@ -199,15 +238,15 @@ FS_DEV void fs_lp_mlp(const char *img, const fs_bf8 (&xin)[2][FS_LP_CB], fs_f16v
     fs_bf8 h[8][FS_LP_CB];
     fs_lp_relu_pack(acc, h);
     if constexpr (FS_LP_CB == 2) __builtin_amdgcn_sched_barrier(0);
-    fs_lp_hidden(img, FS_LP_OFF_W2, FS_LP_OFF_B2, h, acc, lane);
+    fs_lp_hidden(img, FS_LP_L_W2, FS_LP_L_B2, h, acc, lane);
     fs_lp_relu_pack(acc, h);
     if constexpr (FS_LP_CB == 2) __builtin_amdgcn_sched_barrier(0);
-    fs_lp_hidden(img, FS_LP_OFF_W3, FS_LP_OFF_B3, h, acc, lane);
+    fs_lp_hidden(img, FS_LP_L_W3, FS_LP_L_B3, h, acc, lane);
     fs_lp_relu_pack(acc, h);
     if constexpr (FS_LP_CB == 2) __builtin_amdgcn_sched_barrier(0);
     {
-        const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_OFF_W4);
-        const float *b = reinterpret_cast<const float *>(img + FS_LP_OFF_B4);
+        const fs_bf8 *W = reinterpret_cast<const fs_bf8 *>(img + FS_LP_L_W4);
+        const float *b = reinterpret_cast<const float *>(img + FS_LP_L_B4);
         fs_lp_bias_init(b, 0, g, out);
 #pragma unroll
         for (int ks = 0; ks < FS_LP_KS; ks++) {
@@ -939,7 +978,7 @@ template <bool AK, bool BK> FS_DEV void fs_lp_gemm_body(const fs_lp_gemm_args &p
         const int m = m0 + 8 * (j >> 2) + 4 * g + (j & 3);
         const long ci = (long)m * p.ldc + n;
         if (p.mode == 0) {
-            reinterpret_cast<float *>(p.C)[(long)blockIdx.z * p.M * p.N + ci] = acc[j] + bias;
+            reinterpret_cast<float *>(p.C)[(long)blockIdx.z * p.zstride + ci] = acc[j] + bias;
         } else if (p.mode == 1) {
             reinterpret_cast<__bf16 *>(p.C)[ci] = (__bf16)fmaxf(acc[j] + bias, 0.0f);
         } else {
@@ -983,16 +1022,22 @@ template <int D> FS_DEV void fs_lp_adam_body(const fs_lp_adam_args &a, int i)
     const float *src = nullptr;
     long stride = 0, idx = 0;
     float grad = 0.0f;
-    if (i < Q::B1) { src = a.dW1; stride = 128 * 32; idx = (i / Q::NIN) * 32 + i % Q::NIN; }
-    else if (i < Q::W2) { grad = a.db[i - Q::B1]; }
-    else if (i < Q::B2) { src = a.dW2; stride = 128 * 128; idx = i - Q::W2; }
-    else if (i < Q::W3) { grad = a.db[128 + i - Q::B2]; }
-    else if (i < Q::B3) { src = a.dW3; stride = 128 * 128; idx = i - Q::W3; }
-    else if (i < Q::W4) { grad = a.db[256 + i - Q::B3]; }
-    else if (i < Q::B4) { src = a.dW4; stride = 32 * 128; idx = i - Q::W4; }
-    else { grad = a.db[384 + i - Q::B4]; }
-    if (src)
+    // a weight into or out of a hidden unit beyond the active width has exactly zero gradient (the
+    // unit's ReLU mask is zero); training does not compute those partials, so they are not read
+    constexpr int A = FS_LP_ACT;
+    if (i < Q::B1) { if ((i / Q::NIN) < A) { src = a.dW1; stride = 128 * 32; idx = (i / Q::NIN) * 32 + i % Q::NIN; } }
+    else if (i < Q::W2) { idx = i - Q::B1; if (idx < A) { src = a.db; stride = FS_LP_NBIAS; } }
+    else if (i < Q::B2) { idx = i - Q::W2; if (idx / 128 < A && idx % 128 < A) { src = a.dW2; stride = 128 * 128; } }
+    else if (i < Q::W3) { idx = i - Q::B2; if (idx < A) { src = a.db; stride = FS_LP_NBIAS; idx += 128; } }
+    else if (i < Q::B3) { idx = i - Q::W3; if (idx / 128 < A && idx % 128 < A) { src = a.dW3; stride = 128 * 128; } }
+    else if (i < Q::W4) { idx = i - Q::B3; if (idx < A) { src = a.db; stride = FS_LP_NBIAS; idx += 256; } }
+    else if (i < Q::B4) { idx = i - Q::W4; if (idx % 128 < A) { src = a.dW4; stride = 32 * 128; } }
+    else { src = a.db; stride = FS_LP_NBIAS; idx = 384 + i - Q::B4; }
+    // the partials are added in chunk order; the unrolled loop keeps sixteen loads in flight
+    if (src) {
+#pragma unroll 16
         for (int z = 0; z < a.splits; z++) grad += src[z * stride + idx];
+    }
     if (a.gradOut) a.gradOut[i] = grad;
     if (!a.apply) return;
     float m = a.m[i] * a.b1 + (1.0f - a.b1) * grad;
@@ -1005,21 +1050,18 @@ template <int D> FS_DEV void fs_lp_adam_body(const fs_lp_adam_args &a, int i)
 // ---- kernels of one module (instantiated for the trait `sys`) ----
 #ifndef FS_LP_NO_KERNELS
 #if FS_LP_COARSE
-// One launch = one whole-path coarse move of every walker (FS_LP_SPW walkers per wavefront).
+// One launch = one whole-path coarse move of every walker (FS_LP_SPW walkers per wavefront).  The
+// module allows up to FS_LP_WAVES wavefronts per block; the launch takes blockDim.x / 64 of them,
+// as many as fill a CU best beside the (width-dependent) LDS image.
 extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp_args a)
 {
     typedef fs_lpc_geom<sys> G;
     extern __shared__ __attribute__((aligned(16))) char fs_lp_smem[];
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(a.img);
-        uint4 *dst = reinterpret_cast<uint4 *>(fs_lp_smem);
-        for (int i = threadIdx.x; i < FS_LP_IMG_BYTES / 16; i += blockDim.x) dst[i] = src[i];
-    }
+    fs_lp_load_image(a.img, fs_lp_smem);
     __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float *stage = reinterpret_cast<float *>(fs_lp_smem + FS_LP_IMG_BYTES + wave * G::WAVE_BYTES);
-    for (long long u = (long long)blockIdx.x * FS_LP_WAVES + wave; u < a.nUnits;
-         u += (long long)gridDim.x * FS_LP_WAVES)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
+    float *stage = reinterpret_cast<float *>(fs_lp_smem + FS_LP_L_BYTES + wave * G::WAVE_BYTES);
+    for (long long u = (long long)blockIdx.x * waves + wave; u < a.nUnits; u += (long long)gridDim.x * waves)
         fs_lpc_unit<sys>(a, fs_lp_smem, stage, (long)u * G::WPW, lane);
 }
 // The fine pass of the whole-path move: one wavefront per walker, FS_LPC_FINE_WAVES per block.
@@ -1037,14 +1079,10 @@ extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp
 {
     typedef fs_lp_geom<sys> G;
     extern __shared__ __attribute__((aligned(16))) char fs_lp_smem[];
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(a.img);
-        uint4 *dst = reinterpret_cast<uint4 *>(fs_lp_smem);
-        for (int i = threadIdx.x; i < FS_LP_IMG_BYTES / 16; i += blockDim.x) dst[i] = src[i];
-    }
+    fs_lp_load_image(a.img, fs_lp_smem);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    float *stage = reinterpret_cast<float *>(fs_lp_smem + FS_LP_IMG_BYTES + wave * G::WAVE_BYTES);
+    float *stage = reinterpret_cast<float *>(fs_lp_smem + FS_LP_L_BYTES + wave * G::WAVE_BYTES);
     for (long long u = (long long)blockIdx.x * FS_LP_WAVES + wave; u < a.nUnits;
          u += (long long)gridDim.x * FS_LP_WAVES)
         fs_lp_unit<sys>(a, fs_lp_smem, stage, (long)(u / G::UPW), (int)(u % G::UPW) * G::SPW, lane);
@@ -1056,11 +1094,7 @@ extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_eval(const 
                                                                           float *O, int rows)
 {
     extern __shared__ __attribute__((aligned(16))) char fs_lp_smem[];
-    {
-        const uint4 *src = reinterpret_cast<const uint4 *>(img_g);
-        uint4 *dst = reinterpret_cast<uint4 *>(fs_lp_smem);
-        for (int i = threadIdx.x; i < FS_LP_IMG_BYTES / 16; i += blockDim.x) dst[i] = src[i];
-    }
+    fs_lp_load_image(img_g, fs_lp_smem);
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, g = lane >> 5, c32 = lane & 31;
 #pragma unroll 1
@@ -1124,15 +1158,17 @@ extern "C" __global__ __launch_bounds__(256) void fs_lp_loss(const float *O, con
     if (threadIdx.x == 0) lossPart[blockIdx.x] = red[0];
 }
 
-// Column sums of the layer gradients (bias gradients): column c of
-// dO (c < 32 -> db4), dA3 (db3), dA2 (db2), dA1 (db1); one block per column,
-// written as db [b1 128][b2 128][b3 128][b4 32].
+// Bias gradients: column sums of dO (-> b4), dA3 (b3), dA2 (b2), dA1 (b1), one partial per chunk of
+// FS_LP_SPLIT_ROWS rows (blockIdx.y), written as db [chunk][b1 128][b2 128][b3 128][b4 32] and summed
+// over the chunks in a fixed order by Adam.  A block sums 32 columns (blockIdx.x) of its chunk: lane
+// t & 31 takes one column and t >> 5 every eighth row, so a wavefront reads two rows' 64 contiguous
+// bytes per load.  Training computes the hidden layers on the FS_LP_ACT active units only; the other
+// columns of dA1..dA3 are not written, and their bias gradients are exactly zero.
 extern "C" __global__ __launch_bounds__(256) void fs_lp_colsum(const __bf16 *dA1, const __bf16 *dA2,
-                                                              const __bf16 *dA3, const __bf16 *dO, float *db,
-                                                              int batch)
+                                                              const __bf16 *dA3, const __bf16 *dO, float *db)
 {
-    __shared__ float red[256];
-    const int c = blockIdx.x;  // 0 .. 415
+    __shared__ float red[8][33];
+    const int c = blockIdx.x * 32 + (threadIdx.x & 31), r0 = threadIdx.x >> 5;  // c: 0 .. 415
     const __bf16 *src;
     int ld, col;
     if (c < 128) { src = dA1; ld = 128; col = c; }
@@ -1140,14 +1176,19 @@ extern "C" __global__ __launch_bounds__(256) void fs_lp_colsum(const __bf16 *dA1
     else if (c < 384) { src = dA3; ld = 128; col = c - 256; }
     else { src = dO; ld = 32; col = c - 384; }
     float s = 0.0f;
-    for (int b = threadIdx.x; b < batch; b += 256) s += (float)src[(long)b * ld + col];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+    if (c >= 384 || col < FS_LP_ACT) {
+        const long b0 = (long)blockIdx.y * FS_LP_SPLIT_ROWS;
+#pragma unroll 4
+        for (int r = r0; r < FS_LP_SPLIT_ROWS; r += 8) s += (float)src[(b0 + r) * ld + col];
     }
-    if (threadIdx.x == 0) db[c] = red[0];
+    red[r0][threadIdx.x & 31] = s;
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) t += red[k][threadIdx.x];
+        db[(long)blockIdx.y * FS_LP_NBIAS + c] = t;
+    }
 }
 
 extern "C" __global__ __launch_bounds__(256) void fs_lp_adam(fs_lp_adam_args a)

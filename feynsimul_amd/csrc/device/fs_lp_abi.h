@@ -13,7 +13,12 @@
 #define FS_LP_OFF_B3 (FS_LP_OFF_B2 + 512)
 #define FS_LP_OFF_B4 (FS_LP_OFF_B3 + 512)
 #define FS_LP_IMG_BYTES (FS_LP_OFF_B4 + 128)
-#define FS_LP_SPLIT_ROWS 2048  // training rows per weight-gradient partial
+// Bytes of a block's LDS copy of the image at active width A (a multiple of 32): the weight blocks
+// and biases of the active units only; FS_LP_IMG_BYTES at A = FS_LP_HID.
+#define FS_LP_LDS_IMG_BYTES(A) (1024 * (((A) / 32) * 2 + 2 * ((A) / 32) * ((A) / 16) + (A) / 16) + 12 * (A) + 128)
+#define FS_LP_BATCH_QUANTUM 2048  // training batches are multiples of this many rows
+#define FS_LP_SPLIT_ROWS 256      // training rows per weight- and bias-gradient partial
+#define FS_LP_NBIAS 416           // bias gradients b1, b2, b3 (128 each), b4 (32)
 
 struct fs_lp_args {
     float *paths;           // [walkers][DOF][N]
@@ -48,13 +53,14 @@ struct fs_lp_gemm_args {
     const float *bias;        // [N] or NULL
     const unsigned short *H;  // mode 2: ReLU mask source, same layout as C
     long long lda_m, lda_k, ldb_k, ldb_n, ldc;
+    long long zstride;        // mode 0: elements between the partials of consecutive blockIdx.z
     int M, N, K, mode;        // 0 FP32 (+bias) partial per blockIdx.z, 1 BF16 relu(acc + bias), 2 BF16 masked
 };
 
 struct fs_lp_adam_args {
     float *P, *m, *v;
     const float *dW1, *dW2, *dW3, *dW4;  // partials [splits][rows][cols] (padded shapes)
-    const float *db;                     // [128 * 3 + 32] bias gradients
+    const float *db;                     // bias-gradient partials [splits][FS_LP_NBIAS]
     float *gradOut;                      // optional: summed gradient, logical order
     int splits;
     float lr, b1, b2, aeps, c1, c2;      // c1 = 1 - b1^t, c2 = 1 - b2^t

@@ -120,6 +120,11 @@ class LearnedProposal:
         self._ws = z(info.workspace_bytes, t.uint8)
         self._loss = z(info.loss_parts, t.float32)
         self._acc = z(kernel._nbrOfWalkers, t.int32)
+        # run(): accept counters, their sums in pinned host memory and the events that mark them, for
+        # two blocks in flight
+        self._accs = [z(kernel._nbrOfWalkers, t.int32) for _ in range(2)]
+        self._accHost = t.zeros((2,), dtype=t.int64, pin_memory=True)
+        self._accEv = [t.cuda.Event() for _ in range(2)]
         self._trace = None
         self._batch = int(batch)
         self._step = 0
@@ -187,20 +192,38 @@ class LearnedProposal:
 
     def run(self, operatorRuns=None, sweepsPerMeasurement=None):
         """operatorRuns blocks of sweepsPerMeasurement learned sweeps, each followed by one
-        measurement of the kernel's operators (series, totals, correlator, histograms).  A block
-        whose learned acceptance falls below fallbackAcceptance is followed by plain bisection
-        sweeps of the same length (with the measurement)."""
+        measurement of the kernel's operators (series, totals, correlator, histograms).  The host
+        does not wait for a block: its acceptance is copied back while the next block runs, and a
+        block that follows one whose learned acceptance fell below fallbackAcceptance (for the first
+        block: the last block of the previous call) adds plain bisection sweeps of the same length
+        before its measurement."""
         k = self._k
         opRuns = int(operatorRuns if operatorRuns is not None else k._operatorRuns)
         M = int(sweepsPerMeasurement if sweepsPerMeasurement is not None else k._metroStepsPerOperatorRun)
-        for _ in range(opRuns):
-            ar = self.sweep(M)
-            if ar < self.fallbackAcceptance:
+        moves = float(self._movesPerSweep * M)
+        low = self.lastAcceptance is not None and self.lastAcceptance < self.fallbackAcceptance
+        for b in range(opRuns):
+            i = b & 1
+            if b > 0:
+                # block b - 1 has been copied back once its event has passed; block b is not waited for
+                self._accEv[i ^ 1].synchronize()
+                self.lastAcceptance = float(self._accHost[i ^ 1]) / moves
+                low = self.lastAcceptance < self.fallbackAcceptance
+            acc = self._accs[i]
+            acc.zero_()
+            self._P.sweep(k._paths.data_ptr(), acc.data_ptr(), self._bufs(), self._sweepCount, M,
+                          k._stream.cuda_stream)
+            self._sweepCount += M
+            self._accHost[i].copy_(acc.sum(), non_blocking=True)
+            self._accEv[i].record(k._stream)
+            if low:
                 self.fallbackBlocks += 1
                 k._launch(1, M)
             else:
                 k._launch(1, 0)
         k._stream.synchronize()
+        if opRuns > 0:
+            self.lastAcceptance = float(self._accHost[(opRuns - 1) & 1]) / moves
 
     # ---- training ----
     def train(self, steps=1):

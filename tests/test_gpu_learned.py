@@ -230,9 +230,10 @@ def test_training_improves_a_stiff_oscillator(gpu):
 
 
 def test_fallback_below_twenty_percent_keeps_ho_energy(gpu):
-    """C1 sizes with a network so poor that learned blocks accept under 20 %: every such block is
-    followed by a plain bisection block (the fallback), and <x^2> stays at the discretised-HO value
-    0.499950082044 within 4 standard errors."""
+    """C1 sizes with a network so poor that learned blocks accept under 20 %: the block after each
+    such block adds a plain bisection block (the fallback; the first block looks at the acceptance of
+    the sweeps before it), and <x^2> stays at the discretised-HO value 0.499950082044 within 4
+    standard errors."""
     _case("learned fallback, C1")
     system = HarmOsc()
     W, N, S, beta = 4096, 256, 4, 10.0
