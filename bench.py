@@ -303,6 +303,54 @@ def hbm_ceiling(gib=2, repeats=7):
     return {"metric": "HBM ceilings (fs_hbm_probe)", "bytes": nbytes, "ceilings": out, "peak_GBps": HBM_PEAK_GBS}
 
 
+INFINITY_CACHE_BYTES = 256 << 20  # MI355X last-level (Infinity) cache in front of HBM
+
+
+def walker_scaling(launches=10, warmup=3):
+    """Time per slice-update of the plain sampler against the number of chains, for the He3 workload
+    (N = 1024, S = 6, 32 sweeps + 1 measurement per launch) and the anharmonic oscillator (N = 4096,
+    S = 4, 32 sweeps + 1 energy measurement, no correlator).  With few chains the paths fit in the
+    256 MB Infinity Cache and each chain's wavefront is resident for the whole launch; if HBM
+    bandwidth bounded the kernel, those launches would update slices faster than the full workloads."""
+    import numpy as np
+    from feynsimul_amd.kernel_args import KernelArgs
+    from feynsimul_amd.kernel import PIMCKernel
+    from feynsimul_amd.physical_systems import Lm2m2Cluster, AnharmOsc
+    out = {"metric": "slice-updates/s of the plain sampler against chain count", "cache_bytes": INFINITY_CACHE_BYTES,
+           "he3": [], "aho": []}
+    he3 = Lm2m2Cluster(3)
+    aho = AnharmOsc(AHO_LAM)
+    for name, system, NN, beta, S_, counts in (("he3", he3, N, BETA, S, (1024, 2048, 4096, 5120, 8192, 16384)),
+                                               ("aho", aho, AHO_N, AHO_BETA, 4, (4096, 8192, 16384, 32768, 65536))):
+        for walkers in counts:
+            progress("walker scaling: %s, %d chains" % (name, walkers))
+            extra = {}
+            if name == "he3":
+                extra = dict(histogramObservables=tuple(system.pairDistance(i, j) for i, j in system.pairs),
+                             histogramRange=(0.0, HIST_MAX_A), histogramBins=256)
+            ops = (system.energyOp, system.meanPairDistanceOp) if name == "he3" else (system.energyOp,)
+            k = PIMCKernel(KernelArgs(system=system, nbrOfWalkers=walkers, N=NN, beta=beta, S=S_, enableOperator=True,
+                                      enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                                      enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                                      enableGlobalOldPath=False, enableBins=False, metroStepsPerOperatorRun=32,
+                                      operatorRuns=1, nbrOfWalkersPerWorkGroup=1, operators=ops, seed=3,
+                                      sampler="wave", pathStorage="memory", **extra))
+            rng = np.random.default_rng(1)
+            if name == "he3":
+                pos = system.initialPositions(walkers, rng, side=3.5, sigma=3.0)
+                k.setPaths(np.repeat(pos[:, :, None], NN, axis=2).reshape(walkers, system.DOF * NN))
+            else:
+                k.setPaths(rng.uniform(-1, 1, (walkers, NN)).astype(np.float32))
+            ms, _ = timed_launches(k, launches, warmup)
+            seg = 1 << S_
+            upd = walkers * (NN // seg) * (seg - 1) * 32
+            out[name].append({"chains": walkers, "path_bytes": walkers * NN * system.DOF * 4, "ms_per_launch": ms,
+                              "slice_updates_per_s": upd / (ms * 1e-3),
+                              "paths_fit_in_cache": walkers * NN * system.DOF * 4 <= INFINITY_CACHE_BYTES})
+            del k
+    return out
+
+
 def lp_bench(steps, warmup, spw=0):
     """The learned sweep alone on the He3 workload (fs_lp_sweep): `steps` sweeps timed with HIP events on
     the kernel's stream after `warmup`, from paths 512 plain sweeps into the run; a network with a
@@ -1056,6 +1104,9 @@ def main():
     ap.add_argument("--valu-calibration", action="store_true",
                     help="run only the xorshift128 stream kernel on 1,048,576 streams (3 launches), the VALU "
                          "issue-rate reference for rocprofv3 SQ counters")
+    ap.add_argument("--walker-scaling", action="store_true",
+                    help="time the plain sampler of the He3 and anharmonic workloads at several chain counts, "
+                         "including counts whose paths fit in the Infinity Cache, and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of 1 GiB (5 times), to scale HBM counters")
     ap.add_argument("--waves-per-simd", type=int, default=None,
@@ -1087,6 +1138,9 @@ def main():
         return
     if args.hbm_ceiling:
         print(json.dumps(hbm_ceiling()), flush=True)
+        return
+    if args.walker_scaling:
+        print(json.dumps(walker_scaling()), flush=True)
         return
     if args.valu_calibration:
         from feynsimul_amd import _native
@@ -1183,7 +1237,12 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if args.storage in (None, "memory") else None,
                      "kernel": "fs_wave_sweep", "avg_launch_ms": kern_s * 1e3,
-                     "bytes_per_update": BYTES_PER_UPDATE, "updates_per_launch": updates},
+                     "bytes_per_update": BYTES_PER_UPDATE, "updates_per_launch": updates,
+                     "limiter": "not HBM: with the paths in the Infinity Cache (4096-5120 chains) the rate is "
+                                "0.92-0.95 of this one and saturates with wavefronts per SIMD; the SIMD's issue of "
+                                "the block loop's instruction mix bounds it (VALU active 0.59 of the calibration "
+                                "kernel's), and its HBM traffic runs level with the measured copy ceiling",
+                     "limiter_source": "profiles/fs_wave_sweep_walker_scaling.md (bench.py --walker-scaling)"},
         "valu": dict(valu_fields(VALU_PER_UPDATE * updates / kern_s, TRAFFIC_PROFILE), ops_per_update=VALU_PER_UPDATE),
         "acceptance": k.getAcceptanceRate(),
         "energy_K": E,
