@@ -630,7 +630,7 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
 RAGGED_PROFILE = os.path.join(ROOT, "profiles", "fs_batch_sweep_ragged.md")
 
 
-def ragged_bench(steps, warmup, cpu=True, levels=None, shape=None):
+def ragged_bench(steps, warmup, cpu=True, levels=None, shape=None, separate=True):
     """Batched ragged LM2M2 systems in one persistent launch (32 sweeps per launch), autotuned."""
     import numpy as np
     import torch
@@ -699,10 +699,47 @@ def ragged_bench(steps, warmup, cpu=True, levels=None, shape=None):
             "register_classes": {"kernel_of_type": batch.kernelOfType, "waves_of_kernel": batch.kernelWaves},
             "accepted_slice_updates_per_s": accepted / wall,
             "acceptance": [round(a, 4) for a in acc]}
+    if separate:
+        del batch, graph
+        torch.cuda.empty_cache()
+        out["separate"] = ragged_separate(kas, starts, launch_s)
     if cpu and int(os.environ.get("RANK", "0")) == 0:
         progress("CPU baseline, ragged types")
         out["cpu_baseline"] = cpu_baseline_ragged(kas)
     return out
+
+
+def ragged_separate(kas, starts, batch_launch_s, fill=8):
+    """The same ragged types launched one at a time, as the reference's run loop does (one program per
+    system): each type's stand-alone kernel at its 2048 chains, and at `fill` x 2048 chains (enough
+    wavefronts to fill the GPU, the type's full-occupancy rate), both at the batch's bisection levels.
+    separate_ms is what the sixteen launches take back to back; full_occupancy_ms the sixteen types at
+    their full-occupancy rates (the time no scheduler can beat without a faster sampler); the batch's
+    efficiency is full_occupancy_ms over its own launch time."""
+    import copy
+    import numpy as np
+    import torch
+    from feynsimul_amd.kernel import PIMCKernel
+    rows = []
+    for ka, start in zip(kas, starts):
+        progress("ragged: type n=%d N=%d alone" % (int(ka.system.DOF) // 3, ka.N))
+        row = {"atoms": int(ka.system.DOF) // 3, "N": ka.N, "S": ka.S}
+        for chains in (ka.nbrOfWalkers, fill * ka.nbrOfWalkers):
+            kb = copy.copy(ka)
+            kb.nbrOfWalkers = chains
+            k = PIMCKernel(kb)
+            k.setPaths(np.tile(start, (chains // start.shape[0], 1)))
+            ms, _ = timed_launches(k, 3, 1)
+            row["ms_%d_chains" % chains] = ms
+            del k
+            torch.cuda.empty_cache()
+        row["ms_at_full_occupancy_rate"] = row["ms_%d_chains" % (fill * ka.nbrOfWalkers)] / fill
+        rows.append(row)
+    sep = sum(r["ms_%d_chains" % kas[0].nbrOfWalkers] for r in rows)
+    full = sum(r["ms_at_full_occupancy_rate"] for r in rows)
+    return {"types": rows, "separate_ms": sep, "full_occupancy_ms": full, "batch_ms": batch_launch_s * 1e3,
+            "batch_speedup_over_separate": sep / (batch_launch_s * 1e3),
+            "scheduler_efficiency": full / (batch_launch_s * 1e3)}
 
 
 # ---- anharmonic oscillator: the correlator and RNG stress workload ----
@@ -1098,6 +1135,8 @@ def main():
                     help="comma-separated S of the 16 ragged types (skips the level autotuner)")
     ap.add_argument("--ragged-shape", default=None,
                     help="waves per workgroup, order, kernels (e.g. 4,cost,one): skips the launch-shape autotuner")
+    ap.add_argument("--no-separate", action="store_true",
+                    help="ragged: skip timing each type alone (at 2048 chains and at full occupancy)")
     ap.add_argument("--lp-spw", type=int, default=0, help="segments per wavefront of the learned sweep (0: automatic)")
     ap.add_argument("--hbm-ceiling", action="store_true",
                     help="measure the HBM ceilings (copy, read-only, write-only streams of 2 GiB) and exit")
@@ -1160,8 +1199,8 @@ def main():
         if args.ragged_shape:
             wpb, order, kern = args.ragged_shape.split(",")
             shape = {"wavesPerBlock": int(wpb), "order": order, "kernels": kern, "kernelWaves": None}
-        print(json.dumps(ragged_bench(args.steps, args.warmup, cpu=not args.no_cpu, levels=lv, shape=shape)),
-              flush=True)
+        print(json.dumps(ragged_bench(args.steps, args.warmup, cpu=not args.no_cpu, levels=lv, shape=shape,
+                                      separate=not args.no_separate)), flush=True)
         return
     if args.workload == "ho":
         print(json.dumps(ho_bench(args.steps, args.warmup, cpu=not args.no_cpu, ess=not args.no_ess)), flush=True)
