@@ -207,6 +207,56 @@ def test_training_step_matches_oracle(gpu, D):
     np.testing.assert_allclose(theta1, th, rtol=0, atol=2e-6)
 
 
+@pytest.mark.parametrize("width", [32, 64, 96])
+def test_network_and_training_at_active_width(gpu, width):
+    """Below the full 128 units the kernels read only the active units' weights and biases into LDS,
+    and training computes the hidden layers on the active units only (GEMM sizes of the width, no
+    gradient partials of inactive units read): the outputs match the oracle's forward pass of the
+    masked parameters, one Adam step's loss and gradient match the oracle's, the gradient of every
+    inactive parameter is exactly zero, and those parameters stay exactly zero."""
+    _case("learned network and training at width %d" % width)
+    D = 9
+    rng = np.random.default_rng(6)
+    system, W, N, S, beta = Lm2m2Cluster(3), 64, 128, 6, 10.0
+    base = system.initialPositions(W, rng)
+    paths = (base[:, :, None] + 0.1 * rng.standard_normal((W, D, N))).astype(np.float32)
+    k = _kernel(system, W, N, S, beta)
+    k.setPaths(paths.reshape(W, D * N))
+    B = 4096
+    lp = LearnedProposal(k, batch=B, seed=78, width=width)
+    theta = L.init_params(D, 2, scale_last=0.3, active=width)
+    lp.setParameters(theta)
+    X = rng.standard_normal((1000, 2 * D + 1)).astype(np.float32)
+    got = lp.evalNetwork(X)
+    want, _ = L.mlp_forward(L.unflatten(theta, D), X)
+    err = np.abs(got - want)
+    loss = lp.train(1)
+    g = lp.lastGradient()
+    theta1 = lp.getParameters()
+    SEG = 1 << S
+    w, s0, p = L.train_rows(78, 1, B, W, N, SEG)
+    h = p & -p
+    n = S - np.log2(h).astype(np.int64)
+    a = paths[w, :, (s0 + p - h) % N]
+    b = paths[w, :, (s0 + p + h) % N]
+    x = paths[w, :, (s0 + p) % N]
+    f = L.features(a, b, n, S, True, 1.0)
+    sig = np.sqrt(SEG * (beta / N) / (2.0 * 2.0 ** n)).astype(np.float32)
+    loss_o, go = L.nll_and_grad(L.unflatten(theta, D), f, a, b, x, sig, D)
+    go = np.concatenate([go[k_].reshape(-1) for k_, _ in L.param_shapes(D)])
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    print("outputs max %.2e mean %.2e; loss %.6f oracle %.6f, gradient rel err %.2e"
+          % (err.max(), err.mean(), loss, loss_o, rel))
+    assert err.max() < 1e-2 and err.mean() < 2e-4
+    assert abs(loss - loss_o) < 1e-3 * abs(loss_o) + 1e-4
+    assert rel < 3e-2
+    assert np.array_equal(L.mask_width(g, D, width), g)
+    assert np.array_equal(L.mask_width(theta1, D, width), theta1)
+    th = theta.copy()
+    L.adam_update(th, g, np.zeros_like(th), np.zeros_like(th), 1)
+    np.testing.assert_allclose(theta1, th, rtol=0, atol=2e-6)
+
+
 def test_training_improves_a_stiff_oscillator(gpu):
     """k = 25 (omega = 5): over a 16-slice segment the Levy bridge misses the oscillator's
     pull (omega T / 2 = 1.56), so online training must lower the NLL and raise the acceptance

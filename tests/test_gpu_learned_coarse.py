@@ -81,14 +81,14 @@ def _setup(case, rng):
     return system, W, N, beta, D, paths0, _lm2m2_V(system), 0.05
 
 
-def _run_case(case, theta_scale=None, seed=777, sweep=3):
+def _run_case(case, theta_scale=None, seed=777, sweep=3, width=None):
     rng = np.random.default_rng(11)
     system, W, N, beta, D, paths0, V, scale = _setup(case, rng)
     if theta_scale is not None:
         scale = theta_scale
     k = _kernel(system, W, N, beta)
     k.setPaths(paths0.reshape(W, D * N))
-    lp = LearnedProposal(k, batch=2048, seed=seed, coarse=True)
+    lp = LearnedProposal(k, batch=2048, seed=seed, coarse=True, width=width)
     theta = L.init_params(D, 5, scale_last=scale, active=lp.width)
     lp.setParameters(theta)
     lp._sweepCount = sweep
@@ -104,10 +104,14 @@ def _run_case(case, theta_scale=None, seed=777, sweep=3):
 
 # ho8192 (128 intervals) and he3_512 (8 intervals x 9 DOF) exceed 64 values per lane: their accepted bridges are
 # drawn a second time from the same counters instead of being held in registers
-@pytest.mark.parametrize("case", ["ho128", "ho256", "aho512", "he3", "ho8192", "he3_512"])
-def test_coarse_move_matches_oracle(gpu, case):
-    _case("whole-path move %s" % case)
-    paths0, got, ref, tr, d = _run_case(case)
+# The network kernel copies only the active units' weights and biases into LDS and picks its wavefronts per
+# block by occupancy, which both depend on the width: the default 64 units, and 32, 96 and 128 below.
+@pytest.mark.parametrize("case,width", [("ho128", None), ("ho256", None), ("aho512", None), ("he3", None),
+                                        ("ho8192", None), ("he3_512", None), ("aho512", 32), ("he3", 32),
+                                        ("ho256", 96), ("aho512", 128)])
+def test_coarse_move_matches_oracle(gpu, case, width):
+    _case("whole-path move %s width %s" % (case, width))
+    paths0, got, ref, tr, d = _run_case(case, width=width)
     W = paths0.shape[0]
     for name, col, tol in (("lq", 0, 1e-3), ("dK", 1, 1e-3), ("dV", 2, 1e-3)):
         err = np.abs(tr[:, col] - d[name])
