@@ -1062,7 +1062,7 @@ extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, waves = blockDim.x >> 6;
     float *stage = reinterpret_cast<float *>(fs_lp_smem + FS_LP_L_BYTES + wave * G::WAVE_BYTES);
     for (long long u = (long long)blockIdx.x * waves + wave; u < a.nUnits; u += (long long)gridDim.x * waves)
-        fs_lpc_unit<sys>(a, fs_lp_smem, stage, (long)u * G::WPW, lane);
+        fs_lpc_unit<sys>(a, fs_lp_smem, stage, (long)(a.w0 + u * G::WPW), lane);
 }
 // The fine pass of the whole-path move: one wavefront per walker, FS_LPC_FINE_WAVES per block.
 #define FS_LPC_FINE_WAVES 4
@@ -1072,7 +1072,7 @@ extern "C" __global__ __launch_bounds__(64 * FS_LPC_FINE_WAVES) void fs_lpc_fine
     __shared__ float nodes[FS_LPC_FINE_WAVES][G::M * G::D];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long w = (long)blockIdx.x * FS_LPC_FINE_WAVES + wave;
-    if (w < a.nWalkers) fs_lpc_fine_walker<sys>(a, nodes[wave], w, lane);
+    if (w < a.nMove) fs_lpc_fine_walker<sys>(a, nodes[wave], (long)(a.w0 + w), lane);
 }
 #else
 extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp_args a)

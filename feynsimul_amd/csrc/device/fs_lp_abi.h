@@ -31,6 +31,8 @@ struct fs_lp_args {
     float inScale;
     float *trace;           // optional [walkers][N / 2^S][4]: log q ratio, dK, dV, log alpha (test support)
     float *nodes;           // whole-path move: [walkers][M DOF + 2] new coarse nodes, log q ratio, dKeff
+    long long w0;           // whole-path move: first walker of this launch (walkers w0 .. w0 + nMove - 1)
+    long long nMove;        // whole-path move: walkers this launch moves
 };
 
 struct fs_lp_train_args {

@@ -197,3 +197,30 @@ def test_coarse_training_rows_match_oracle(gpu):
     loss_o, _ = L.nll_and_grad(L.unflatten(theta, D), f, a, b, x, sig, D)
     print("loss %.6f oracle %.6f" % (loss, loss_o))
     assert abs(loss - loss_o) < 1e-3 * abs(loss_o) + 1e-4
+
+
+def test_two_stream_halves_match_one_stream(gpu, monkeypatch):
+    """With at least 2 x 8192 walkers the move runs the two halves of the walkers on forked streams; every
+    walker's moves depend only on its own path and counters, so paths, accept counts and the trace equal
+    those of the single-stream launches bit for bit."""
+    _case("whole-path move, two streams against one")
+    system, W, N, beta = AnharmOsc(0.1), 16384, 256, 8.0
+    rng = np.random.default_rng(3)
+    paths0 = _ho_paths(W, N, beta, rng)
+    theta = L.init_params(1, 5, scale_last=0.03, active=32)
+    out = []
+    for streams in ("1", "2"):
+        monkeypatch.setenv("FS_LPC_STREAMS", streams)
+        k = _kernel(system, W, N, beta)
+        k.setPaths(paths0)
+        lp = LearnedProposal(k, batch=2048, seed=99, coarse=True, width=32)
+        lp.setParameters(theta)
+        lp.traceSweeps()
+        ar = lp.sweep(3)
+        out.append((k.getPaths(), lp._acc.cpu().numpy(), lp.getTrace(), ar))
+        del lp, k
+    print("acceptance %.3f / %.3f" % (out[0][3], out[1][3]))
+    assert 0.0 < out[0][3] < 1.0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
