@@ -528,8 +528,8 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
 #ifndef FS_LPC_L
 #define FS_LPC_L 64
 #endif
-template <class Sys> struct fs_lpc_geom {
-    static constexpr int N = Sys::N, S = Sys::S, D = Sys::DOF, L = FS_LPC_L, M = N / L, WPW = FS_LP_SPW;
+template <class Sys, int WPW_ = FS_LP_SPW> struct fs_lpc_geom {
+    static constexpr int N = Sys::N, S = Sys::S, D = Sys::DOF, L = FS_LPC_L, M = N / L, WPW = WPW_;
     static constexpr int NODES = WPW * M * D;                                // floats per node set
     static constexpr int SCR = M * D + 2;                                    // workspace floats per walker
     static constexpr int WAVE_BYTES = ((2 * NODES + 2 * WPW) * 4 + 15) / 16 * 16;
@@ -561,33 +561,23 @@ FS_DEV void fs_lpc_bridge(const float *cn, int j, int M, int lane, uint4 &px, fs
     }
 }
 
-template <class Sys>
-FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long w0, int lane)
+// The move's anchor r of walker w: node j of the coarse level is slice r + 64 j.
+template <int N> FS_DEV int fs_lpc_anchor(const fs_lp_args &a, long w)
 {
-    typedef fs_lpc_geom<Sys> G;
-    constexpr int N = G::N, S = G::S, D = G::D, L = G::L, M = G::M, WPW = G::WPW, LC = S - 6;
-    const int g = lane >> 5, c32 = lane & 31;
-    float *cnew = stage, *cold = stage + G::NODES, *lq = stage + 2 * G::NODES;
-    auto anchor = [&](long w) {
-        return (int)(fs_philox(make_uint4(0u, (fs_u32)w, a.sweep, FS_LP_TAG_OFFSET), a.key0, a.key1).x & (N - 1));
-    };
-    // old coarse nodes; the anchor (node 0) is also the first new node
-#pragma unroll 1
-    for (int k = 0; k < WPW; k++) {
-        const float *path = a.paths + (w0 + k) * (long)(N * D);
-        const int rk = anchor(w0 + k);
-        for (int i = lane; i < M * D; i += 64) {
-            const int j = i / D, d = i % D;
-            const float v = path[d * N + ((rk + j * L) & (N - 1))];
-            cold[k * M * D + i] = v;
-            if (j == 0) cnew[k * M * D + i] = v;
-        }
-    }
-    if (lane < 2 * WPW) lq[lane] = 0.0f;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    return (int)(fs_philox(make_uint4(0u, (fs_u32)w, a.sweep, FS_LP_TAG_OFFSET), a.key0, a.key1).x & (N - 1));
+}
 
-    // ---- coarse levels: network proposals and reverse rows, as in the segment move ----
+// Levels 1 .. LC of the coarse bisection of WPW walkers w0 .. w0 + WPW - 1: network proposals and reverse
+// rows, as in the segment move.  The caller has put each walker's old nodes in cold [WPW][M][D], its anchor
+// node in cnew[k][0] and zeros in lq [2 WPW] (wavefront LDS); the new nodes are left in cnew and the parts of
+// the log q ratio in lq (proposal rows, then reverse rows).
+template <class Sys, int WPW>
+FS_DEV void fs_lpc_levels(const fs_lp_args &a, const char *img, float *cnew, const float *cold, float *lq, long w0,
+                          int lane)
+{
+    typedef fs_lpc_geom<Sys, WPW> G;
+    constexpr int N = G::N, S = G::S, D = G::D, M = G::M, LC = S - 6;
+    const int g = lane >> 5, c32 = lane & 31;
 #pragma unroll 1
     for (int n = 1; n <= LC; n++) {
         const int h = M >> n, nn = 1 << (n - 1);
@@ -700,44 +690,73 @@ FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long
             __builtin_amdgcn_wave_barrier();
         }
     }
+}
 
+// Change of the coarse kinetic action of walker k of a node set, Keff_new - Keff_old with
+// Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps L), on every lane (lane i takes the links i, i + 64, ...).
+template <class Sys, int WPW>
+FS_DEV float fs_lpc_dkin(const float *cnew, const float *cold, int k, int lane)
+{
+    typedef fs_lpc_geom<Sys, WPW> G;
+    constexpr int D = G::D, M = G::M;
+    float dk = 0.0f;
+    for (int i = lane; i < M * D; i += 64) {
+        const int j = i / D, d = i % D, i1 = ((j + 1) & (M - 1)) * D + d;
+        const float en = cnew[k * M * D + i1] - cnew[k * M * D + i], eo = cold[k * M * D + i1] - cold[k * M * D + i];
+        dk += en * en - eo * eo;
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) dk += __shfl_xor(dk, o, 64);
+    return dk * (0.5f / ((float)Sys::EPS * (float)G::L));
+}
+
+template <class Sys>
+FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long w0, int lane)
+{
+    typedef fs_lpc_geom<Sys> G;
+    constexpr int N = G::N, D = G::D, L = G::L, M = G::M, WPW = G::WPW;
+    float *cnew = stage, *cold = stage + G::NODES, *lq = stage + 2 * G::NODES;
+    // old coarse nodes; the anchor (node 0) is also the first new node
+#pragma unroll 1
+    for (int k = 0; k < WPW; k++) {
+        const float *path = a.paths + (w0 + k) * (long)(N * D);
+        const int rk = fs_lpc_anchor<N>(a, w0 + k);
+        for (int i = lane; i < M * D; i += 64) {
+            const int j = i / D, d = i % D;
+            const float v = path[d * N + ((rk + j * L) & (N - 1))];
+            cold[k * M * D + i] = v;
+            if (j == 0) cnew[k * M * D + i] = v;
+        }
+    }
+    if (lane < 2 * WPW) lq[lane] = 0.0f;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    fs_lpc_levels<Sys, WPW>(a, img, cnew, cold, lq, w0, lane);
     // the new nodes, the log q ratio and the change of the coarse kinetic action of every walker,
     // for the fine pass
-    const float inv2eL = 0.5f / ((float)Sys::EPS * (float)L);
 #pragma unroll 1
     for (int k = 0; k < WPW; k++) {
         float *dst = a.nodes + (w0 + k) * (long)G::SCR;
-        float dk = 0.0f;
-        for (int i = lane; i < M * D; i += 64) {
-            const int j = i / D, d = i % D, i1 = ((j + 1) & (M - 1)) * D + d;
-            const float en = cnew[k * M * D + i1] - cnew[k * M * D + i], eo = cold[k * M * D + i1] - cold[k * M * D + i];
-            dk += en * en - eo * eo;
-            dst[i] = cnew[k * M * D + i];
-        }
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) dk += __shfl_xor(dk, o, 64);
+        for (int i = lane; i < M * D; i += 64) dst[i] = cnew[k * M * D + i];
+        const float dk = fs_lpc_dkin<Sys, WPW>(cnew, cold, k, lane);
         if (lane == 0) {
             dst[M * D] = lq[k] + lq[WPW + k];
-            dst[M * D + 1] = dk * inv2eL;
+            dst[M * D + 1] = dk;
         }
     }
 }
 
-// The fine pass of one walker (one wavefront): bridges between the new nodes, log alpha, the
-// decision, and the write of an accepted path.  cn: LDS for the walker's new nodes.
+// The fine pass of one walker (one wavefront) given its new coarse nodes cn [M][D] (LDS), its log q ratio lqk
+// and coarse kinetic change dk: bridges between the nodes, log alpha, the decision, and the write of an
+// accepted path into path [D][N] (global memory, or the walker's LDS copy).  Returns the decision, the same on
+// every lane.
 template <class Sys>
-FS_DEV void fs_lpc_fine_walker(const fs_lp_args &a, float *cn, long w, int lane)
+FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, float dk, float *path, long w, int lane)
 {
     typedef fs_lpc_geom<Sys> G;
     constexpr int N = G::N, D = G::D, L = G::L, M = G::M;
     const fs_u32 wl = (fs_u32)w;
-    float *path = a.paths + w * (long)(N * D);
-    const int rk = (int)(fs_philox(make_uint4(0u, wl, a.sweep, FS_LP_TAG_OFFSET), a.key0, a.key1).x & (N - 1));
-    const float *src = a.nodes + w * (long)G::SCR;
-    for (int i = lane; i < M * D; i += 64) cn[i] = src[i];
-    const float lqk = src[M * D], dk = src[M * D + 1];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+    const int rk = fs_lpc_anchor<N>(a, w);
     const float eps = (float)Sys::EPS;
     const float u = fs_lp_u24(fs_philox(make_uint4(0u, wl, a.sweep, FS_LP_TAG_ACCEPT), a.key0, a.key1).x);
     float dv = 0.0f;
@@ -795,8 +814,9 @@ FS_DEV void fs_lpc_fine_walker(const fs_lp_args &a, float *cn, long w, int lane)
 #pragma unroll
                 for (int d = 0; d < D; d++) path[d * N + s] = xs[j][d];
             }
-            if (lane == 0) a.accepts[w] += 1u;
+            return true;
         }
+        return false;
     } else {
         // otherwise the bridges are drawn again (the same normals) to write an accepted path
 #pragma unroll 2
@@ -820,9 +840,76 @@ FS_DEV void fs_lpc_fine_walker(const fs_lp_args &a, float *cn, long w, int lane)
 #pragma unroll
                 for (int d = 0; d < D; d++) path[d * N + s] = xn[d];
             }
-            if (lane == 0) a.accepts[w] += 1u;
+            return true;
         }
+        return false;
     }
+}
+
+// The fine pass of one walker from the network kernel's workspace (new nodes, log q ratio, coarse kinetic
+// change); cn: LDS for the walker's new nodes.
+template <class Sys>
+FS_DEV void fs_lpc_fine_walker(const fs_lp_args &a, float *cn, long w, int lane)
+{
+    typedef fs_lpc_geom<Sys> G;
+    constexpr int N = G::N, D = G::D, M = G::M;
+    const float *src = a.nodes + w * (long)G::SCR;
+    for (int i = lane; i < M * D; i += 64) cn[i] = src[i];
+    const float lqk = src[M * D], dk = src[M * D + 1];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (fs_lpc_fine_path<Sys>(a, cn, lqk, dk, a.paths + w * (long)(N * D), w, lane) && lane == 0) a.accepts[w] += 1u;
+}
+
+// ---- the whole-path move fused for small paths (N DOF <= FS_LPC_FUSED_MAX) ----
+// One wavefront per walker holds the walker's path in LDS for the whole launch and makes nMoves moves in
+// a row: the coarse nodes from the LDS path, the network's levels (rows of one walker per tile), the fine
+// pass on the LDS path, then the next move; the path returns to HBM at the end.  The random numbers are
+// counted exactly as in the two-kernel move, and every row's network output and every sum is formed in the
+// same order, so the result equals that of nMoves two-kernel moves.  For small paths the two-kernel move is
+// bound by launch gaps and by one wavefront per SIMD (four walkers per wavefront).
+#define FS_LPC_FUSED_MAX 1024
+#define FS_LPC_FUSED_WAVES 4
+template <class Sys> struct fs_lpc_fused_geom {
+    typedef fs_lpc_geom<Sys, 1> G;
+    static constexpr int PATH = Sys::N * Sys::DOF;
+    static constexpr int WAVE_BYTES = ((PATH + 2 * G::NODES + 2) * 4 + 15) / 16 * 16;
+};
+
+template <class Sys>
+FS_DEV void fs_lpc_fused_walker(const fs_lp_args &a, const char *img, float *wb, long w, int nMoves, int lane)
+{
+    typedef fs_lpc_geom<Sys, 1> G;
+    constexpr int N = G::N, D = G::D, L = G::L, M = G::M, PATH = N * D;
+    float *path = wb, *cnew = wb + PATH, *cold = cnew + G::NODES, *lq = cold + G::NODES;
+    float *gp = a.paths + w * (long)PATH;
+    for (int i = lane; i < PATH; i += 64) path[i] = gp[i];
+    fs_u32 nacc = 0;
+    fs_lp_args am = a;
+#pragma unroll 1
+    for (int m = 0; m < nMoves; m++) {
+        am.sweep = a.sweep + (unsigned)m;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        const int rk = fs_lpc_anchor<N>(am, w);
+        for (int i = lane; i < M * D; i += 64) {
+            const int j = i / D, d = i % D;
+            const float v = path[d * N + ((rk + j * L) & (N - 1))];
+            cold[i] = v;
+            if (j == 0) cnew[i] = v;
+        }
+        if (lane < 2) lq[lane] = 0.0f;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        fs_lpc_levels<Sys, 1>(am, img, cnew, cold, lq, w, lane);
+        const float dk = fs_lpc_dkin<Sys, 1>(cnew, cold, 0, lane);
+        const float lqk = lq[0] + lq[1];
+        if (fs_lpc_fine_path<Sys>(am, cnew, lqk, dk, path, w, lane)) nacc++;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int i = lane; i < PATH; i += 64) gp[i] = path[i];
+    if (lane == 0) a.accepts[w] += nacc;
 }
 
 // ---- pack: FP32 master parameters (logical) -> inference image and training matrices ----
@@ -1073,6 +1160,21 @@ extern "C" __global__ __launch_bounds__(64 * FS_LPC_FINE_WAVES) void fs_lpc_fine
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const long w = (long)blockIdx.x * FS_LPC_FINE_WAVES + wave;
     if (w < a.nMove) fs_lpc_fine_walker<sys>(a, nodes[wave], (long)(a.w0 + w), lane);
+}
+// nMoves whole-path moves per launch for small paths: one wavefront per walker, FS_LPC_FUSED_WAVES per
+// block sharing the LDS image of the network's active units.
+extern "C" __global__ __launch_bounds__(64 * FS_LPC_FUSED_WAVES) void fs_lpc_fused(fs_lp_args a, int nMoves)
+{
+    if constexpr (sys::N * sys::DOF <= FS_LPC_FUSED_MAX) {
+        extern __shared__ __attribute__((aligned(16))) char fs_lp_smem[];
+        fs_lp_load_image(a.img, fs_lp_smem);
+        __syncthreads();
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const long w = (long)blockIdx.x * FS_LPC_FUSED_WAVES + wave;
+        if (w >= a.nMove) return;  // whole wavefronts; no block barrier follows
+        float *wb = reinterpret_cast<float *>(fs_lp_smem + FS_LP_L_BYTES + wave * fs_lpc_fused_geom<sys>::WAVE_BYTES);
+        fs_lpc_fused_walker<sys>(a, fs_lp_smem, wb, (long)(a.w0 + w), nMoves, lane);
+    }
 }
 #else
 extern "C" __global__ __launch_bounds__(64 * FS_LP_WAVES) void fs_lp_sweep(fs_lp_args a)

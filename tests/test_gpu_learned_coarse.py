@@ -224,3 +224,31 @@ def test_two_stream_halves_match_one_stream(gpu, monkeypatch):
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("case,width", [("ho256", 32), ("aho512", 64), ("ho128", 128)])
+def test_fused_small_path_move_matches_two_kernel_move(gpu, monkeypatch, case, width):
+    """Paths of at most 1024 floats take the fused move (one wavefront per walker, the path in LDS, every move
+    of a call in one launch); its random numbers, network rows and sums are the two-kernel move's, so five
+    moves in a row give the same paths, accept counts and last-move trace bit for bit."""
+    _case("fused whole-path move %s width %d against the two-kernel move" % (case, width))
+    rng = np.random.default_rng(17)
+    system, W, N, beta, D, paths0, V, scale = _setup(case, rng)
+    theta = L.init_params(D, 5, scale_last=scale, active=width)
+    out = []
+    for fused in ("0", "1"):
+        monkeypatch.setenv("FS_LPC_FUSED", fused)
+        k = _kernel(system, W, N, beta)
+        k.setPaths(paths0.reshape(W, D * N))
+        lp = LearnedProposal(k, batch=2048, seed=321, coarse=True, width=width)
+        lp.setParameters(theta)
+        lp._sweepCount = 11
+        lp.traceSweeps()
+        ar = lp.sweep(5)
+        out.append((k.getPaths(), lp._acc.cpu().numpy(), lp.getTrace(), ar))
+        del lp, k
+    print("acceptance %.3f / %.3f" % (out[0][3], out[1][3]))
+    assert 0.0 < out[0][3] < 1.0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
