@@ -120,9 +120,9 @@ class LearnedProposal:
         self._ws = z(info.workspace_bytes, t.uint8)
         self._loss = z(info.loss_parts, t.float32)
         self._acc = z(kernel._nbrOfWalkers, t.int32)
-        # run(): accept counters, their sums in pinned host memory and the events that mark them, for
-        # two blocks in flight
-        self._accs = [z(kernel._nbrOfWalkers, t.int32) for _ in range(2)]
+        # run(): accept counters, their running sums in pinned host memory and the events that mark them,
+        # for two blocks in flight
+        self._accs = [z(kernel._nbrOfWalkers, t.int32)]
         self._accHost = t.zeros((2,), dtype=t.int64, pin_memory=True)
         self._accEv = [t.cuda.Event() for _ in range(2)]
         self._trace = None
@@ -136,11 +136,16 @@ class LearnedProposal:
 
     # ---- buffers and parameters ----
     def _bufs(self):
+        # the buffers are allocated once, in the constructor, except the trace (traceSweeps resets the cache)
+        b = self.__dict__.get("_bufsCache")
+        if b is not None:
+            return b
         b = _native.FsLpBuffers()
         b.params, b.adam_m, b.adam_v = self._params.data_ptr(), self._m.data_ptr(), self._v.data_ptr()
         b.image, b.train_w, b.train_b = self._image.data_ptr(), self._tw.data_ptr(), self._tb.data_ptr()
         b.workspace, b.loss_parts = self._ws.data_ptr(), self._loss.data_ptr()
         b.trace = self._trace.data_ptr() if self._trace is not None else None
+        self._bufsCache = b
         return b
 
     @property
@@ -202,17 +207,23 @@ class LearnedProposal:
         M = int(sweepsPerMeasurement if sweepsPerMeasurement is not None else k._metroStepsPerOperatorRun)
         moves = float(self._movesPerSweep * M)
         low = self.lastAcceptance is not None and self.lastAcceptance < self.fallbackAcceptance
+        # accept counts accumulate over the call's blocks; a block's acceptance is the difference of the
+        # running totals copied back after it and after the block before it
+        acc = self._accs[0]
+        acc.zero_()
+        done = 0  # running total after the last block whose copy has been read
+        bufs = self._bufs()
+        paths, stream = k._paths.data_ptr(), k._stream.cuda_stream
         for b in range(opRuns):
             i = b & 1
             if b > 0:
                 # block b - 1 has been copied back once its event has passed; block b is not waited for
                 self._accEv[i ^ 1].synchronize()
-                self.lastAcceptance = float(self._accHost[i ^ 1]) / moves
+                total = int(self._accHost[i ^ 1])
+                self.lastAcceptance = (total - done) / moves
+                done = total
                 low = self.lastAcceptance < self.fallbackAcceptance
-            acc = self._accs[i]
-            acc.zero_()
-            self._P.sweep(k._paths.data_ptr(), acc.data_ptr(), self._bufs(), self._sweepCount, M,
-                          k._stream.cuda_stream)
+            self._P.sweep(paths, acc.data_ptr(), bufs, self._sweepCount, M, stream)
             self._sweepCount += M
             self._accHost[i].copy_(acc.sum(), non_blocking=True)
             self._accEv[i].record(k._stream)
@@ -223,7 +234,7 @@ class LearnedProposal:
                 k._launch(1, 0)
         k._stream.synchronize()
         if opRuns > 0:
-            self.lastAcceptance = float(self._accHost[(opRuns - 1) & 1]) / moves
+            self.lastAcceptance = (int(self._accHost[(opRuns - 1) & 1]) - done) / moves
 
     # ---- training ----
     def train(self, steps=1):
@@ -250,6 +261,7 @@ class LearnedProposal:
         """Keep (log q ratio, dK, dV, log alpha) of every segment of the last sweep (test support)."""
         k = self._k
         per = 1 if self.coarse else k._N >> k._S
+        self._bufsCache = None
         self._trace = (k._torch.zeros((k._nbrOfWalkers, per, 4), dtype=k._torch.float32,
                                       device=k._device) if on else None)
 
