@@ -75,6 +75,12 @@ def _setup(case, rng):
         V = _poly_V(0.0 if case.startswith("ho") else 0.1)
         return system, W, N, beta, D, paths0, V, 0.03
     N = 512 if case == "he3_512" else 128
+    if case == "he2":
+        # two atoms, DOF 6: a path of 768 floats, which takes the fused move
+        system, W, beta, D = Lm2m2Cluster(2), 32, 1.0, 6
+        base = system.initialPositions(W, rng)
+        paths0 = (base[:, :, None] + 0.02 * rng.standard_normal((W, D, N))).astype(np.float32)
+        return system, W, N, beta, D, paths0, _lm2m2_V(system), 0.05
     system, W, beta, D = Lm2m2Cluster(3), 16, 1.0 * N / 128, 9
     base = system.initialPositions(W, rng)
     paths0 = (base[:, :, None] + 0.02 * rng.standard_normal((W, D, N))).astype(np.float32)
@@ -97,7 +103,7 @@ def _run_case(case, theta_scale=None, seed=777, sweep=3, width=None):
     got = k.getPaths().reshape(W, D, N)
     tr = lp.getTrace()[:, 0, :]
     ref = paths0.copy()
-    _, d = L.coarse_move(ref, L.unflatten(theta, D), V, eps=beta / N, seed=seed, sweep=sweep, atoms3d=(D == 9),
+    _, d = L.coarse_move(ref, L.unflatten(theta, D), V, eps=beta / N, seed=seed, sweep=sweep, atoms3d=(D % 3 == 0),
                          return_detail=True)
     return paths0, got, ref, tr, d
 
@@ -108,7 +114,7 @@ def _run_case(case, theta_scale=None, seed=777, sweep=3, width=None):
 # block by occupancy, which both depend on the width: the default 64 units, and 32, 96 and 128 below.
 @pytest.mark.parametrize("case,width", [("ho128", None), ("ho256", None), ("aho512", None), ("he3", None),
                                         ("ho8192", None), ("he3_512", None), ("aho512", 32), ("he3", 32),
-                                        ("ho256", 96), ("aho512", 128)])
+                                        ("ho256", 96), ("aho512", 128), ("he2", None)])
 def test_coarse_move_matches_oracle(gpu, case, width):
     _case("whole-path move %s width %s" % (case, width))
     paths0, got, ref, tr, d = _run_case(case, width=width)
@@ -226,7 +232,7 @@ def test_two_stream_halves_match_one_stream(gpu, monkeypatch):
     assert np.array_equal(out[0][2], out[1][2])
 
 
-@pytest.mark.parametrize("case,width", [("ho256", 32), ("aho512", 64), ("ho128", 128)])
+@pytest.mark.parametrize("case,width", [("ho256", 32), ("aho512", 64), ("ho128", 128), ("he2", 32)])
 def test_fused_small_path_move_matches_two_kernel_move(gpu, monkeypatch, case, width):
     """Paths of at most 1024 floats take the fused move (one wavefront per walker, the path in LDS, every move
     of a call in one launch); its random numbers, network rows and sums are the two-kernel move's, so five
