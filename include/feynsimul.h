@@ -271,9 +271,10 @@ typedef struct fs_lp_config {
                                  1: whole-path move, N = 64 x 2^k: the network proposes the nodes every
                                  64 slices (levels 1 .. k of the whole cyclic path), a Levy bridge the
                                  slices between them, one Metropolis-Hastings decision per path */
-    int32_t width;            /* active hidden units of inference, a multiple of 32 up to 128 (0 = 128):
-                                 the parameters keep the 128-wide layout, and units beyond the active
-                                 width must be zero (training keeps them zero) */
+    int32_t width;            /* active hidden units, a multiple of 32 up to 128 (0 = 128): inference and
+                                 training compute on these only; the parameters keep the 128-wide layout,
+                                 and units beyond the active width must be zero (training keeps them zero,
+                                 their gradients being exactly zero) */
 } fs_lp_config;
 typedef struct fs_lp_info {
     int64_t n_params;         /* FP32 master parameters (and each Adam moment) */
@@ -307,7 +308,8 @@ int fs_lp_build_log(const fs_lp *p, char *buf, size_t len, size_t *needed);
 /* Rebuild the BF16 inference image and training matrices from params. */
 int fs_lp_pack(fs_lp *p, const fs_lp_buffers *b, void *stream);
 /* n_sweeps learned sweeps (sweep counters sweep0 ...) over paths [walkers][DOF][N];
- * accepted segments (whole-path move: accepted paths) are added to accepts[walkers]. */
+ * accepted segments (whole-path move: accepted paths) are added to accepts[walkers].
+ * The whole-path move of a path of at most 1024 floats makes all n_sweeps moves in one launch. */
 int fs_lp_sweep(fs_lp *p, void *paths, void *accepts, const fs_lp_buffers *b, uint64_t sweep0, int32_t n_sweeps,
                 void *stream);
 /* One Adam step (step counts from 1) on `batch` rows drawn from paths, then a repack. */
