@@ -539,7 +539,7 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
                                      enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
                                      enableGlobalOldPath=False, enableBins=False, metroStepsPerOperatorRun=M,
                                      operatorRuns=opRuns, nbrOfWalkersPerWorkGroup=1, operators=(system.energyOp,),
-                                     seed=seed, sampler="wave", pathStorage="memory"))
+                                     seed=seed, sampler="wave"))
     kb = kernel(burn, 1, 41)
     kb.setPaths(np.random.default_rng(40).uniform(-1, 1, (W, NN)).astype(np.float32))
     kb.run()
