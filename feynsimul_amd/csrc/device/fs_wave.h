@@ -248,11 +248,12 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
             }
             // two blocks per iteration in memory storage: the loaded block and the
             // stream state then change registers in place, with no copies at the loop edge.
-            // Register storage unrolls up to four blocks: the blocks of a step are independent (a
-            // segment's ends are positions no move of the step changes), so one block's acceptance
-            // chain overlaps the next block's draws, and a fully unrolled loop indexes the path's
-            // register slots statically (C1, four blocks: 0.120 -> 0.088 ms per 32-sweep launch)
-            constexpr int BLOCK_UNROLL = Sys::RESIDENT ? (NB < 4 ? NB : 4) : (NB >= 2 ? 2 : 1);
+            // Register storage unrolls every block: the blocks of a step are independent (a segment's
+            // ends are positions no move of the step changes), so one block's acceptance chain overlaps
+            // the next block's draws, and the path's register slots are indexed statically instead of
+            // through lane-uniform dynamic register indexing (harmonic oscillator, 32-sweep launches: four
+            // blocks 0.120 -> 0.087 ms, eight blocks 0.781 -> 0.485 ms, sixteen 0.298 -> 0.278 ms)
+            constexpr int BLOCK_UNROLL = Sys::RESIDENT ? NB : (NB >= 2 ? 2 : 1);
 #pragma unroll BLOCK_UNROLL
             for (int g = 0; g < NB; g++) {
                 // xb: this lane's slot in block g + 1; on lane r (position 0) that is the
