@@ -122,10 +122,11 @@ class LearnedProposal:
         self._acc = z(kernel._nbrOfWalkers, t.int32)
         # run(): accept counters, their running sums in pinned host memory and the events that mark them,
         # for two blocks in flight
-        self._accs = [z(kernel._nbrOfWalkers, t.int32)]
+        self._runAcc = z(kernel._nbrOfWalkers, t.int32)
         self._accHost = t.zeros((2,), dtype=t.int64, pin_memory=True)
         self._accEv = [t.cuda.Event() for _ in range(2)]
         self._trace = None
+        self._bufsCache = None
         self._batch = int(batch)
         self._step = 0
         self._sweepCount = 0
@@ -137,9 +138,8 @@ class LearnedProposal:
     # ---- buffers and parameters ----
     def _bufs(self):
         # the buffers are allocated once, in the constructor, except the trace (traceSweeps resets the cache)
-        b = self.__dict__.get("_bufsCache")
-        if b is not None:
-            return b
+        if self._bufsCache is not None:
+            return self._bufsCache
         b = _native.FsLpBuffers()
         b.params, b.adam_m, b.adam_v = self._params.data_ptr(), self._m.data_ptr(), self._v.data_ptr()
         b.image, b.train_w, b.train_b = self._image.data_ptr(), self._tw.data_ptr(), self._tb.data_ptr()
@@ -209,7 +209,7 @@ class LearnedProposal:
         low = self.lastAcceptance is not None and self.lastAcceptance < self.fallbackAcceptance
         # accept counts accumulate over the call's blocks; a block's acceptance is the difference of the
         # running totals copied back after it and after the block before it
-        acc = self._accs[0]
+        acc = self._runAcc
         acc.zero_()
         done = 0  # running total after the last block whose copy has been read
         bufs = self._bufs()
