@@ -284,9 +284,10 @@ typedef struct fs_lp_info {
     int64_t workspace_bytes;  /* training workspace */
     int64_t loss_parts;       /* FP32 partial losses of a training step (sum / batch = mean NLL) */
     int32_t segs_per_wave;
-    int32_t waves_per_block;
+    int32_t waves_per_block;  /* of a sweep launch (the whole-path move: its network kernel, or the fused
+                                 move for paths of at most 1024 floats) */
     int32_t lds_bytes;        /* dynamic LDS of a sweep block */
-    int32_t grid_blocks;      /* sweep blocks (persistent) */
+    int32_t grid_blocks;      /* sweep blocks (persistent, except the fused move's one block per 4 walkers) */
 } fs_lp_info;
 typedef struct fs_lp_buffers {
     void *params, *adam_m, *adam_v;  /* FP32 [n_params] */
