@@ -19,10 +19,11 @@
 // Inference (fs_lp_sweep): one launch is one sweep.  A wavefront owns SPW
 // consecutive segments of one walker and walks the S levels; the rows of a
 // level (its proposals, then the reverse rows that score the old path) go
-// through the MLP in tiles of 64 rows.  The network is computed transposed,
+// through the MLP in tiles of 32 FS_LP_CB rows.  The network is computed transposed,
 // C^T[features x rows] = W[features x in] . H[in x rows], with
 // v_mfma_f32_32x32x16_bf16: weights are the A operand (read from LDS, where a
-// block keeps the whole 82 KB image), activations the B operand.  The C
+// block keeps the active units' part of the image, 82 KB at the full width),
+// activations the B operand.  The C
 // layout of one layer is the B layout of the next once the next layer's input
 // features are permuted to match (the pack kernel does this), so hidden
 // activations never leave registers.  Proposals are staged in LDS; the old
@@ -146,7 +147,7 @@ FS_DEV void fs_lp_load_image(const char *img_g, char *lds)
     }
 }
 
-// ---- the network on a 64-row tile held by one wavefront ----
+// ---- the network on a tile of 32 FS_LP_CB rows held by one wavefront ----
 // xin[ks][cb]: layer-1 B operand (inputs ks*16 + 8g + e of row cb*32 + lane%32).
 // out[cb][j]: output slot (g, j) of row cb*32 + lane%32; slot j of lane group g
 // holds mu-offset (j even) or s (j odd) of coordinate d = 2 (j/2) + g.
@@ -716,16 +717,33 @@ FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long
     typedef fs_lpc_geom<Sys> G;
     constexpr int N = G::N, D = G::D, L = G::L, M = G::M, WPW = G::WPW;
     float *cnew = stage, *cold = stage + G::NODES, *lq = stage + 2 * G::NODES;
-    // old coarse nodes; the anchor (node 0) is also the first new node
+    // old coarse nodes; the anchor (node 0) is also the first new node.  Each node is a 128-byte line of its
+    // path: when a walker's nodes fit a lane each (M DOF <= 64), the WPW walkers' loads are all issued
+    // before the first is waited for
+    if constexpr (M * D <= 64) {
+        const int j = lane / D, d = lane % D;
+        float v[WPW];
+#pragma unroll
+        for (int k = 0; k < WPW; k++) {
+            const int rk = fs_lpc_anchor<N>(a, w0 + k);
+            v[k] = lane < M * D ? a.paths[(w0 + k) * (long)(N * D) + d * N + ((rk + j * L) & (N - 1))] : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < WPW; k++) {
+            if (lane < M * D) cold[k * M * D + lane] = v[k];
+            if (lane < D) cnew[k * M * D + lane] = v[k];
+        }
+    } else {
 #pragma unroll 1
-    for (int k = 0; k < WPW; k++) {
-        const float *path = a.paths + (w0 + k) * (long)(N * D);
-        const int rk = fs_lpc_anchor<N>(a, w0 + k);
-        for (int i = lane; i < M * D; i += 64) {
-            const int j = i / D, d = i % D;
-            const float v = path[d * N + ((rk + j * L) & (N - 1))];
-            cold[k * M * D + i] = v;
-            if (j == 0) cnew[k * M * D + i] = v;
+        for (int k = 0; k < WPW; k++) {
+            const float *path = a.paths + (w0 + k) * (long)(N * D);
+            const int rk = fs_lpc_anchor<N>(a, w0 + k);
+            for (int i = lane; i < M * D; i += 64) {
+                const int j = i / D, d = i % D;
+                const float v = path[d * N + ((rk + j * L) & (N - 1))];
+                cold[k * M * D + i] = v;
+                if (j == 0) cnew[k * M * D + i] = v;
+            }
         }
     }
     if (lane < 2 * WPW) lq[lane] = 0.0f;

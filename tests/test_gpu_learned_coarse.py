@@ -67,16 +67,17 @@ def _lm2m2_V(system):
 
 
 def _setup(case, rng):
-    if case in ("ho128", "ho256", "aho512", "ho8192"):
-        N, beta = {"ho128": (128, 2.0), "ho256": (256, 10.0), "aho512": (512, 8.0), "ho8192": (8192, 4.0)}[case]
+    if case in ("ho128", "ho256", "aho512", "ho8192", "aho2048", "aho4096"):
+        N, beta = {"ho128": (128, 2.0), "ho256": (256, 10.0), "aho512": (512, 8.0), "ho8192": (8192, 4.0),
+                   "aho2048": (2048, 8.0), "aho4096": (4096, 16.0)}[case]
         system = HarmOsc() if case.startswith("ho") else AnharmOsc(0.1)
         W, D = 64, 1
         paths0 = _ho_paths(W, N, beta, rng)[:, None, :]
         V = _poly_V(0.0 if case.startswith("ho") else 0.1)
         return system, W, N, beta, D, paths0, V, 0.03
-    N = 512 if case == "he3_512" else 128
-    if case == "he2":
-        # two atoms, DOF 6: a path of 768 floats, which takes the fused move
+    N = {"he3_512": 512, "he2_256": 256}.get(case, 128)
+    if case.startswith("he2"):
+        # two atoms, DOF 6: at N = 128 a path of 768 floats, which takes the fused move
         system, W, beta, D = Lm2m2Cluster(2), 32, 1.0, 6
         base = system.initialPositions(W, rng)
         paths0 = (base[:, :, None] + 0.02 * rng.standard_normal((W, D, N))).astype(np.float32)
@@ -210,7 +211,8 @@ def test_two_stream_halves_match_one_stream(gpu, monkeypatch):
     walker's moves depend only on its own path and counters, so paths, accept counts and the trace equal
     those of the single-stream launches bit for bit."""
     _case("whole-path move, two streams against one")
-    system, W, N, beta = AnharmOsc(0.1), 16384, 256, 8.0
+    # N DOF > 1024: the two-kernel move (smaller paths take the fused move, which has no second stream)
+    system, W, N, beta = AnharmOsc(0.1), 16384, 2048, 8.0
     rng = np.random.default_rng(3)
     paths0 = _ho_paths(W, N, beta, rng)
     theta = L.init_params(1, 5, scale_last=0.03, active=32)
@@ -255,6 +257,39 @@ def test_fused_small_path_move_matches_two_kernel_move(gpu, monkeypatch, case, w
         del lp, k
     print("acceptance %.3f / %.3f" % (out[0][3], out[1][3]))
     assert 0.0 < out[0][3] < 1.0
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+# two-kernel moves (N DOF > 1024) with 32, 4 x 6, 64 and 128 coarse values per walker
+@pytest.mark.parametrize("case", ["aho2048", "he2_256", "aho4096", "ho8192"])
+def test_moves_of_one_call_match_single_move_calls(gpu, case):
+    """A call of four two-kernel moves reuses the workspace for each move's nodes and counts the moves'
+    random numbers by sweep0 + i; it must equal four calls of one move, bit for bit: paths, accept
+    counts and the last move's trace."""
+    _case("whole-path move %s: one call of four moves against four calls" % case)
+    rng = np.random.default_rng(23)
+    system, W, N, beta, D, paths0, V, scale = _setup(case, rng)
+    assert N * D > 1024  # not the fused move
+    theta = L.init_params(D, 5, scale_last=scale, active=32)
+    out = []
+    for calls in (1, 4):
+        k = _kernel(system, W, N, beta)
+        k.setPaths(paths0.reshape(W, D * N))
+        lp = LearnedProposal(k, batch=2048, seed=57, coarse=True, width=32)
+        lp.setParameters(theta)
+        lp._sweepCount = 5
+        lp.traceSweeps()
+        acc = np.zeros(W, np.int64)
+        for _ in range(calls):
+            lp.sweep(4 // calls)  # each call zeroes and refills the accept counts
+            acc += lp._acc.cpu().numpy()
+        out.append((k.getPaths(), acc, lp.getTrace()))
+        del lp, k
+    acc = out[0][1].sum()
+    print("accepted %d of %d moves" % (acc, 4 * W))
+    assert 0 < acc < 4 * W
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
