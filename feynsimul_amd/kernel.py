@@ -19,6 +19,7 @@ numbers match the defect list kept with the design notes):
     (in the reference they alias rows other work-items rewrite during it);
   * seeds are reproducible when ``KernelArgs.seed`` is set.
 """
+import functools
 import math
 import os
 import time
@@ -50,6 +51,18 @@ def humanReadableSize(size):
     p = int(np.log(size) / np.log(1024))
     names = ["", "ki", "Mi", "Gi", "Ti", "Pi", "Ei"]
     return "%.3g" % (size / 1024.0 ** p) + " " + names[p] + "B"
+
+
+def _on_engine_stream(method):
+    """Runs `method` with the engine's stream as torch's current stream: the torch work it does on the
+    engine's device buffers (copies, fills, sums, reads back) is then ordered with the engine's launches
+    on that stream, whichever stream the caller has made current."""
+    @functools.wraps(method)
+    def call(self, *args, **kwargs):
+        k = getattr(self, "_k", self)
+        with k._torch.cuda.stream(k._stream):
+            return method(self, *args, **kwargs)
+    return call
 
 
 class _Queue:
@@ -449,6 +462,7 @@ class PIMCKernel:
         self._ev1.record(self._stream)
         self._account_run(opRuns, M)
 
+    @_on_engine_stream
     def initRanlux(self):
         """Seeds every work-item's RANLUX state (kernel.py:508-530), one distinct seed each."""
         if not self._enableRanlux:
@@ -457,6 +471,7 @@ class PIMCKernel:
         seeds = _splitmix64_first(np.uint64(self._seed) + np.arange(n, dtype=np.uint64))
         self._ranlux_fill(self._rl, seeds)
 
+    @_on_engine_stream
     def groupInitRanlux(self):
         """Seeds the group (segment-offset) RANLUX states (kernel.py:532-554): one stream per walker group."""
         if not self._enableRanlux:
@@ -478,10 +493,12 @@ class PIMCKernel:
                                                    self._stream.cuda_stream))
         self._stream.synchronize()
 
+    @_on_engine_stream
     def getPaths(self):
         """Paths as (walkers, N * DOF), DOF-major per walker (kernel.py:557-565)."""
         return self._paths.cpu().numpy()
 
+    @_on_engine_stream
     def setPaths(self, paths):
         """Replaces the paths (kernel.py:567-574); cast to the kernel's precision."""
         a = np.asarray(paths).astype(np.float64 if self._enableDouble else np.float32)
@@ -489,6 +506,7 @@ class PIMCKernel:
         self._paths.copy_(self._torch.from_numpy(np.ascontiguousarray(a)))
         self._stream.synchronize()
 
+    @_on_engine_stream
     def getOperators(self):
         """Per-walker operator means, shape (nOps, walkers) (kernel.py:576-606)."""
         if not self._enableOperator:
@@ -501,6 +519,7 @@ class PIMCKernel:
         per = raw.reshape(self._nbrOfThreads, nOps)
         return per[tid].mean(axis=1).T.copy()
 
+    @_on_engine_stream
     def getOperatorTotals(self):
         """Running FP64 (Kahan) sums of the per-measurement slice sums, (nOps, walkers); wave sampler."""
         if self._opTotals is None:
@@ -514,6 +533,7 @@ class PIMCKernel:
             raise Exception("this kernel is a member of a BatchedPIMC; the batch sets its launch shape")
         self._engine.set_waves_per_simd(int(cap or 0))
 
+    @_on_engine_stream
     def setSeriesCapacity(self, measurements):
         """Keeps every measurement's per-walker slice mean (for autocorrelation / ESS); wave sampler."""
         if self._sampler != "wave" or not self._cfg.n_ops:
@@ -521,6 +541,7 @@ class PIMCKernel:
         self._series = self._zeros((int(measurements), self._nbrOfWalkers, self._cfg.n_ops), self._torch.float32)
         self._seriesCount = 0
 
+    @_on_engine_stream
     def getSeries(self):
         """(measurements, walkers, nOps) float32 of the stored measurements."""
         if self._series is None:
@@ -533,6 +554,7 @@ class PIMCKernel:
             raise Exception("call setSeriesCapacity first")
         return self._series[: self._seriesCount]
 
+    @_on_engine_stream
     def getCorrelator(self):
         """(mean, standard error) of the correlators, each (nCorr, lags) (kernel.py:608-617)."""
         if not self._enableCorrelator:
@@ -546,11 +568,13 @@ class PIMCKernel:
         se = per.std(axis=0) / np.sqrt(self._nbrOfWalkers)
         return (mean, se)
 
+    @_on_engine_stream
     def resetCorrelator(self):
         if self._correlatorValues is not None:
             self._correlatorValues.zero_()
         self._corRuns = 0
 
+    @_on_engine_stream
     def getBinCounts(self):
         """Probability density on the DOF-dimensional grid (kernel.py:619-637)."""
         if not self._enableBins:
@@ -561,6 +585,7 @@ class PIMCKernel:
         counts = self._binCounts.cpu().numpy().view(np.uint32).reshape((self._binResolutionPerDOF,) * DOF)
         return counts.astype(np.float64 if self._enableDouble else np.float32) / (binVol * tot)
 
+    @_on_engine_stream
     def resetBins(self):
         if self._binCounts is not None:
             self._binCounts.zero_()
@@ -569,6 +594,7 @@ class PIMCKernel:
             self._obsHist.zero_()
         self._obsRuns = 0
 
+    @_on_engine_stream
     def getObservableHistogram(self):
         """(bin centres, density, counts) of the histogram of ``histogramObservables``.
 
@@ -683,6 +709,7 @@ class PIMCKernel:
         return (self._nbrOfWalkers * self._metroStepsPerOperatorRun * self._operatorRuns
                 * (self._enableBisection + self._enablePathShift + self._enableSingleNodeMove))
 
+    @_on_engine_stream
     def evalSystem(self, x):
         """Potential and operators of the compiled module at points x (n, DOF): (pot[n], ops[n, nOps])."""
         t = self._torch

@@ -22,6 +22,7 @@ holds the device buffers.
 import numpy as np
 
 from . import _native
+from .kernel import _on_engine_stream
 
 
 def _mask_width(theta, dof, active, hidden=128):
@@ -155,9 +156,11 @@ class LearnedProposal:
     def parameterCount(self):
         return int(self._P.info.n_params)
 
+    @_on_engine_stream
     def getParameters(self):
         return self._params.cpu().numpy()
 
+    @_on_engine_stream
     def setParameters(self, theta, resetAdam=True):
         """FP32 parameters in logical order W1 [128][2D+1], b1, W2, b2, W3, b3, W4 [2D][128], b4.
         The parameters of hidden units beyond the active width are set to zero."""
@@ -175,6 +178,7 @@ class LearnedProposal:
         self._k._stream.synchronize()
 
     # ---- sampling ----
+    @_on_engine_stream
     def sweep(self, nSweeps=1, wait=True):
         """nSweeps learned sweeps; returns their acceptance rate (segments, or paths for the
         whole-path move, accepted / proposed)."""
@@ -188,6 +192,7 @@ class LearnedProposal:
         self.lastAcceptance = float(self._acc.sum().item()) / (self._movesPerSweep * int(nSweeps))
         return self.lastAcceptance
 
+    @_on_engine_stream
     def lastAcceptanceOfQueued(self, nSweeps):
         """Acceptance rate of the last sweep(nSweeps, wait=False) call, after it has finished."""
         k = self._k
@@ -195,6 +200,7 @@ class LearnedProposal:
         self.lastAcceptance = float(self._acc.sum().item()) / (self._movesPerSweep * int(nSweeps))
         return self.lastAcceptance
 
+    @_on_engine_stream
     def run(self, operatorRuns=None, sweepsPerMeasurement=None):
         """operatorRuns blocks of sweepsPerMeasurement learned sweeps, each followed by one
         measurement of the kernel's operators (series, totals, correlator, histograms).  The host
@@ -237,6 +243,7 @@ class LearnedProposal:
             self.lastAcceptance = (int(self._accHost[(opRuns - 1) & 1]) - done) / moves
 
     # ---- training ----
+    @_on_engine_stream
     def train(self, steps=1):
         """Adam steps on rows drawn from the current paths; returns the mean NLL of the last step."""
         k = self._k
@@ -246,6 +253,7 @@ class LearnedProposal:
         k._stream.synchronize()
         return float(self._loss.double().sum().item()) / self._batch
 
+    @_on_engine_stream
     def trainOnline(self, steps, plainSweepsPerStep=2):
         """Online training: each Adam step follows plainSweepsPerStep plain bisection sweeps of
         the kernel, whose paths supply its rows.  Returns the mean NLL of the last step."""
@@ -265,9 +273,11 @@ class LearnedProposal:
         self._trace = (k._torch.zeros((k._nbrOfWalkers, per, 4), dtype=k._torch.float32,
                                       device=k._device) if on else None)
 
+    @_on_engine_stream
     def getTrace(self):
         return self._trace.cpu().numpy()
 
+    @_on_engine_stream
     def lastGradient(self):
         """Gradient of the mean NLL used by the last Adam step, logical order (test support)."""
         t = self._k._torch
@@ -276,6 +286,7 @@ class LearnedProposal:
         self._k._stream.synchronize()
         return g.cpu().numpy()
 
+    @_on_engine_stream
     def evalNetwork(self, X):
         """Network outputs (rows, 2 DOF) for features X (rows, 2 DOF + 1) (test support)."""
         t = self._k._torch

@@ -128,3 +128,30 @@ def test_limits_and_verbose_dump(gpu, sampler, tmp_path, monkeypatch):
     assert (tmp_path / "cleaned.hip").read_text() == k._engine.source
     pre = (tmp_path / "preprocessed.hip").read_text()
     assert "potential" in pre and "#define" not in pre
+
+
+def test_calls_under_another_current_stream(gpu):
+    """The kernel's and the learned proposal's torch work on their device buffers runs on the engine's
+    stream whatever stream the caller has made current: the same calls made under a second current stream
+    give the same paths, operator means and proposal acceptance bit for bit."""
+    import torch
+    from feynsimul_amd.learned import LearnedProposal
+    _case("calls under another current stream")
+    system, W, N = AnharmOsc(0.1), 256, 128
+    p0 = _paths(system, W, N, np.random.default_rng(5))
+    out = []
+    for other in (False, True):
+        k = PIMCKernel(_ka(system, W, N, 4, 4.0, "wave", opRuns=3, steps=8))
+        lp = LearnedProposal(k, batch=2048, seed=3)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s) if other else torch.cuda.stream(torch.cuda.current_stream()):
+            k.setPaths(p0)
+            k.run()
+            lp.train(2)
+            ar = lp.sweep(2)
+            lp.run(2, 1)
+            out.append((k.getPaths(), k.getOperators(), ar, lp.lastAcceptance))
+        del lp, k
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2:] == out[1][2:]
