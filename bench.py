@@ -497,19 +497,32 @@ def run_to_sigma(block, get_series, sigma, exact, max_seconds, measurements_per_
 
 
 # the learned leg of the fixed-sigma comparison: the whole-path move, its network on 32 hidden units,
-# trained online (Adam steps, each after plain sweeps), then MOVES_PER_MEASUREMENT moves per E0
+# trained online (Adam steps, each after plain sweeps)
 # (one plain sweep per Adam step: on the anharmonic workload 500 steps reach the acceptance of 500 steps
 # after two sweeps each, 0.41 against 0.42, in 0.47 s of training instead of 0.76 s)
 FIXED_SIGMA_LEARNED = {"ho": {"train_steps": 300, "batch": 16384, "plain_sweeps_per_step": 1},
                        "aho": {"train_steps": 500, "batch": 65536, "plain_sweeps_per_step": 1}}
-LEARNED_WIDTH, MOVES_PER_MEASUREMENT = 32, 4
+LEARNED_WIDTH = 32
+# Measurement cadences each leg is run at (sweeps or moves per E0 measurement): a measurement costs about
+# as much as reading every path once, so measuring after every sweep wastes time, while measuring far
+# less often than tau_int discards information the sweeps paid for.  Each cadence runs for SCAN_SECONDS
+# of sampling; the one with the most ESS/s is then run to sigma.
+PLAIN_CADENCES = {"ho": (1, 2, 4, 8, 16, 32), "aho": (2, 4, 8, 16, 32)}
+LEARNED_CADENCES = (1, 2, 4, 8)
+SCAN_SECONDS = 3.0
+
+
+def _ess_per_s(st, sigma, t_extra=0.0):
+    """ESS/s at the target sigma: the ESS grows linearly with sampling time, so a run that stopped at
+    another sigma is scaled to the time it would need for sigma; t_extra (training) is added to it."""
+    return st["ess"] / (st["seconds_to_sigma"] + t_extra) * (st["sigma"] / sigma) ** 2
 
 
 def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
-    """Plain bisection and the learned whole-path move from the same equilibrated paths, each run to
-    the same standard error sigma of E0 (virial energy, slice mean; plain: one measurement every 32
-    sweeps, learned: one every MOVES_PER_MEASUREMENT moves).  ESS/s counts the learned proposal's
-    online training.  ratio = ESS/s learned / ESS/s plain at that sigma.
+    """Plain bisection and the learned whole-path move from the same equilibrated paths, each at the
+    measurement cadence that gives it the most ESS/s (a short run per cadence), then run to the same
+    standard error sigma of E0 (virial energy, slice mean).  The learned leg's ESS/s counts its online
+    training.  ratio = ESS/s learned / ESS/s plain at that sigma.
     `ceiling`: the plain sampler's tau_int of E0 in sweeps (a measurement after every sweep) and, given
     the measured HBM copy ceiling in GB/s, the largest ESS/s ratio over plain that any move replacing a
     sweep could reach."""
@@ -547,16 +560,30 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
     del kb
     # series buffers of at most 1 GiB
     cap = min(65536, (1 << 30) // (W * 4)) // runs * runs
-    # plain
-    kp = kernel(SWEEPS_PER_STEP, runs, 42)
-    kp.setPaths(start)
-    kp.setSeriesCapacity(cap)
-    plain = run_to_sigma(lambda: kp.run(wait=False), lambda: kp.getSeriesTensor()[:, :, 0], sigma, exact,
-                         max_seconds, runs, cap)
-    plain["acceptance"] = kp.getAcceptanceRate()
-    plain["ms_per_sweep"] = 1e3 * plain["seconds"] / (plain["measurements"] * SWEEPS_PER_STEP)
-    del kp
-    torch.cuda.empty_cache()
+
+    # plain: a measurement every k sweeps
+    def plain_leg(k, seconds):
+        kp = kernel(k, runs, 42)
+        kp.setPaths(start)
+        kp.setSeriesCapacity(cap)
+        st = run_to_sigma(lambda: kp.run(wait=False), lambda: kp.getSeriesTensor()[:, :, 0], sigma, exact,
+                          seconds, runs, cap)
+        st["sweeps_per_measurement"] = k
+        st["acceptance"] = kp.getAcceptanceRate()
+        # sweeps and their share of the measurements
+        st["ms_per_sweep"] = 1e3 * st["seconds"] / (st["measurements"] * k)
+        st["ess_per_s"] = _ess_per_s(st, sigma)
+        del kp
+        torch.cuda.empty_cache()
+        return st
+    scan_plain = {k: plain_leg(k, SCAN_SECONDS) for k in PLAIN_CADENCES[workload]}
+    k_plain = max(scan_plain, key=lambda k: scan_plain[k]["ess_per_s"])
+    plain = plain_leg(k_plain, max_seconds)
+    plain["cadence_scan_ess_per_s"] = {str(k): v["ess_per_s"] for k, v in scan_plain.items()}
+    plain["cadence_scan_tau_int"] = {str(k): v["tau_int_measurements"] for k, v in scan_plain.items()}
+    # the sweep's own cost, from the sparsest cadence (measurements a small share)
+    k_sparse = max(scan_plain)
+    ms_sweep = scan_plain[k_sparse]["ms_per_sweep"]
     # E0 after every plain sweep: tau_int in sweeps is what a move that replaced the sweep and gave an
     # independent E0 value every sweep (tau_int = 1) could gain per sweep
     kt = kernel(1, runs, 45)
@@ -568,18 +595,18 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
     del kt
     torch.cuda.empty_cache()
     ceiling = {"plain_tau_int_sweeps": tau_sweeps, "window_sweeps": window, "measurements": TAU_MEASUREMENTS,
-               "plain_ms_per_sweep": plain["ms_per_sweep"]}
+               "plain_ms_per_sweep": ms_sweep, "plain_ms_per_sweep_from_cadence": k_sparse}
     if copy_GBps:
         # such a move still reads and writes every slice once: at the measured HBM copy ceiling that takes
-        # path_ms, so its ESS/s can exceed the plain sampler's by at most tau_sweeps * plain_ms / path_ms
+        # path_ms, so its ESS/s can exceed the plain sampler's (measuring after every sweep at no cost) by
+        # at most tau_sweeps * plain_ms / path_ms
         path_ms = 1e3 * 2 * W * NN * 4 / (copy_GBps * 1e9)
         ceiling.update({"path_read_write_ms_at_copy_ceiling": path_ms, "copy_ceiling_GBps": copy_GBps,
-                        "max_ess_per_s_ratio_any_sweep_move": tau_sweeps * plain["ms_per_sweep"] / path_ms})
-    # learned: the whole-path move, trained online from the same paths
+                        "max_ess_per_s_ratio_any_sweep_move": tau_sweeps * ms_sweep / path_ms})
+    # learned: the whole-path move, trained online from the same paths, then a measurement every k moves
     cfg = FIXED_SIGMA_LEARNED[workload]
-    MPM = MOVES_PER_MEASUREMENT
 
-    def learned_leg(width):
+    def learned_leg(width, cadences):
         kl = kernel(SWEEPS_PER_STEP, runs, 43)
         kl.setPaths(start)
         lp = LearnedProposal(kl, batch=cfg["batch"], lr=1e-3, seed=44, coarse=True, width=width)
@@ -589,31 +616,43 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
         last = lp.trainOnline(cfg["train_steps"] - 1, plainSweepsPerStep=cfg["plain_sweeps_per_step"])
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t0
-        kl.setSeriesCapacity(cap)
-        r = run_to_sigma(lambda: lp.run(runs, MPM), lambda: kl.getSeriesTensor()[:, :, 0], sigma, exact,
-                         max(max_seconds - t_train, 1.0), runs, cap)
+
+        def leg(k, seconds):
+            kl.setSeriesCapacity(cap)
+            fb = lp.fallbackBlocks
+            r = run_to_sigma(lambda: lp.run(runs, k), lambda: kl.getSeriesTensor()[:, :, 0], sigma, exact,
+                             seconds, runs, cap)
+            r.update({"moves_per_measurement": k, "acceptance": lp.lastAcceptance,
+                      "fallback_blocks": lp.fallbackBlocks - fb,
+                      "ms_per_move": 1e3 * r["seconds"] / (r["measurements"] * k)})
+            r["ess_per_s"] = _ess_per_s(r, sigma, t_train)
+            r["ess_per_s_without_training"] = _ess_per_s(r, sigma)
+            return r
+        scan = {k: leg(k, SCAN_SECONDS) for k in cadences}
+        k_best = max(scan, key=lambda k: scan[k]["ess_per_s"])
+        r = leg(k_best, max(max_seconds - t_train, 1.0))
         r.update({"seconds_train": t_train, "adam_steps": cfg["train_steps"], "training_batch": cfg["batch"],
                   "plain_sweeps_per_adam_step": cfg["plain_sweeps_per_step"],
-                  "nll_first": first, "nll_last": last, "acceptance": lp.lastAcceptance,
-                  "fallback_blocks": lp.fallbackBlocks, "hidden_units": width, "moves_per_measurement": MPM,
-                  "ms_per_move": 1e3 * r["seconds"] / (r["measurements"] * MPM)})
-        r["ess_per_s"] = r["ess"] / (r["seconds_to_sigma"] + t_train) * (r["sigma"] / sigma) ** 2
-        r["ess_per_s_without_training"] = r["ess"] / r["seconds_to_sigma"] * (r["sigma"] / sigma) ** 2
+                  "nll_first": first, "nll_last": last, "hidden_units": width,
+                  "cadence_scan_ess_per_s": {str(k): v["ess_per_s"] for k, v in scan.items()},
+                  "cadence_scan_tau_int": {str(k): v["tau_int_measurements"] for k, v in scan.items()}})
         del kl, lp
         torch.cuda.empty_cache()
         return r
 
-    learned = learned_leg(LEARNED_WIDTH)
-    plain["ess_per_s"] = plain["ess"] / plain["seconds_to_sigma"] * (plain["sigma"] / sigma) ** 2
-    # the full 128-unit network of the segment proposal, for comparison (the anharmonic workload)
-    full = learned_leg(128) if workload == "aho" else None
+    learned = learned_leg(LEARNED_WIDTH, LEARNED_CADENCES)
+    # the full 128-unit network of the segment proposal, for comparison (the anharmonic workload), at the
+    # 32-unit network's cadence
+    full = learned_leg(128, (learned["moves_per_measurement"],)) if workload == "aho" else None
     # per unit of work: effective samples per learned move over effective samples per plain sweep
-    per = [plain["ess"] / (plain["measurements"] * SWEEPS_PER_STEP), learned["ess"] / (learned["measurements"] * MPM)]
+    per = [plain["ess"] / (plain["measurements"] * k_plain),
+           learned["ess"] / (learned["measurements"] * learned["moves_per_measurement"])]
     out = {"workload": {"ho": "harmonic oscillator N=256, beta=10, 4096 chains",
                          "aho": "anharmonic oscillator x^2/2 + 0.1 x^4, N=4096, beta=40, 65536 chains"}[workload]
-                        + ", plain S=%d, E0 every 32 sweeps" % S,
+                        + ", plain S=%d, E0 every %d sweeps (the best of %s)" % (S, k_plain, list(PLAIN_CADENCES[workload])),
             "learned_form": "whole-path move: network on the nodes every 64 slices (%d hidden units), Levy "
-                            "bridges between them, one MH decision per path, E0 every %d moves" % (LEARNED_WIDTH, MPM),
+                            "bridges between them, one MH decision per path, E0 every %d moves (the best of %s)"
+                            % (LEARNED_WIDTH, learned["moves_per_measurement"], list(LEARNED_CADENCES)),
             "sigma": sigma, "E0_exact": exact, "plain": plain, "learned": learned,
             "ratio_learned_over_plain": learned["ess_per_s"] / plain["ess_per_s"],
             "ratio_without_training": learned["ess_per_s_without_training"] / plain["ess_per_s"],

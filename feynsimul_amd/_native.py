@@ -9,7 +9,7 @@ import os
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfeynsimul.so")
 
-FS_ABI_VERSION = 5
+FS_ABI_VERSION = 6
 FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
 FS_RNG_XORSHIFT, FS_RNG_RANLUX = 0, 1
@@ -53,7 +53,7 @@ class FsLpConfig(ctypes.Structure):
                 ("segs_per_wave", ctypes.c_int32), ("in_scale", ctypes.c_float), ("lr", ctypes.c_float),
                 ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("adam_eps", ctypes.c_float),
                 ("seed", ctypes.c_uint64), ("coarse", ctypes.c_int32),
-                ("width", ctypes.c_int32)]
+                ("width", ctypes.c_int32), ("bridge_omega2", ctypes.c_float)]
 
 
 class FsLpInfo(ctypes.Structure):

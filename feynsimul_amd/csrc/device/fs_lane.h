@@ -40,6 +40,21 @@ FS_DEV float fs_lane_scan(float v)
     v += FS_DPP_F(v, 0x143, 0xC);  // row_bcast:31 into rows 2 and 3
     return v;
 }
+
+// Inclusive decayed prefix sum w_i = sum_{m <= i} q^(i - m) v_m: the additions of fs_lane_scan with
+// each shifted term weighted by q to the power of its distance, uniform inside the rows (q1 = q,
+// q2 = q^2, q4, q8) and per lane for the row broadcasts (wb1 = q^(i - 15) in row 1, q^(i - 47) in
+// row 3; wb2 = q^(i - 31) in rows 2 and 3).  With every weight 1 it equals fs_lane_scan bit for bit.
+FS_DEV float fs_lane_scan_decay(float v, float q1, float q2, float q4, float q8, float wb1, float wb2)
+{
+    v = fmaf(q1, FS_DPP_F(v, 0x111, 0xF), v);   // row_shr:1
+    v = fmaf(q2, FS_DPP_F(v, 0x112, 0xF), v);   // row_shr:2
+    v = fmaf(q4, FS_DPP_F(v, 0x114, 0xF), v);   // row_shr:4
+    v = fmaf(q8, FS_DPP_F(v, 0x118, 0xF), v);   // row_shr:8
+    v = fmaf(wb1, FS_DPP_F(v, 0x142, 0xA), v);  // row_bcast:15 into rows 1 and 3
+    v = fmaf(wb2, FS_DPP_F(v, 0x143, 0xC), v);  // row_bcast:31 into rows 2 and 3
+    return v;
+}
 #undef FS_DPP_F
 
 // The value of lane l (wave-uniform l) in scalar registers, one v_readlane_b32

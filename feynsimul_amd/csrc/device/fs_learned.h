@@ -518,6 +518,12 @@ FS_DEV void fs_lp_unit(const fs_lp_args &a, const char *img, float *stage, long 
 // only the coarse part of the kinetic action:
 //   log alpha = -eps (V_new - V_old) - (Keff_new - Keff_old) + log q(old nodes) - log q(new nodes),
 //   Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps L).
+// With a harmonic reference omega^2 (hw = omega^2 / 2 > 0) the intervals take the exact Gaussian bridge
+// of the kinetic links plus eps omega^2 |x|^2 / 2 on their interior (an AR(1) chain conditioned on its
+// end, built from a decayed lane scan): V is then scored less omega^2 |x|^2 / 2 on every slice, and Keff
+// becomes kc sum_j |c_{j+1} - rho^L c_j|^2 with kc = 1 / (2 V_L), V_L the variance of a node given the one
+// before under the reference (fs_lpc_dkin).  A reference close to the potential's curvature leaves little
+// of the fine slices' action to the accept test.
 // Random numbers are counted by (node or lane and block, walker, sweep, purpose) as in the segment
 // move.  A lane keeps its new slices in registers until the decision when they fit (M DOF <= 64);
 // otherwise the bridge's normals are drawn twice (to score the path, and to write it once accepted).
@@ -547,19 +553,65 @@ FS_DEV float fs_lpc_fine_normal(uint4 &px, int k, int lane, fs_u32 wl, const fs_
     return fs_lp_normal(px, (k >> 1) & 1, k & 1);
 }
 
-// The new fine slices of interval j on every lane (lane 63: node j + 1).
-template <int D>
-FS_DEV void fs_lpc_bridge(const float *cn, int j, int M, int lane, uint4 &px, fs_u32 wl, const fs_lp_args &a,
-                          float eps, float (&x)[D])
+// The bridge between neighbouring nodes, per lane (slice k = lane + 1 of the interval).  hw = 0: the Levy
+// bridge x_k = a + (b - a) k / 64 + W_k - (k / 64) W_64, W the prefix sums of increments sqrt(eps) z.
+// hw = omega^2 / 2 > 0: the harmonic reference bridge, the AR(1) chain W_k = rho W_{k-1} + sqrt(rho eps) z_k
+// (rho + 1/rho = 2 + eps^2 omega^2) conditioned on its end, x_k = a rho^k + W_k + g_k (b - a rho^64 - W_64),
+// g_k = rho^(64 - k) (1 - rho^(2k)) / (1 - rho^128): the exact Gaussian bridge of the links' kinetic action
+// plus eps omega^2 |x|^2 / 2 on the interior.
+struct fs_lpc_brg {
+    float s, fi, wb1, wb2, pk, g;  // increment scale; per lane: k / 64, broadcast weights, rho^k, g_k
+};
+template <bool HARM> FS_DEV fs_lpc_brg fs_lpc_bridge_coefs(const fs_lp_args &a, float eps, int lane)
 {
-    const float fi = (float)(lane + 1) * (1.0f / (float)FS_LPC_L);
-#pragma unroll
-    for (int d = 0; d < D; d++) {
-        const float v = fs_lane_scan(sqrtf(eps) * fs_lpc_fine_normal(px, j * D + d, lane, wl, a));
-        const float tot = fs_readlane(v, 63);
-        const float ca = cn[j * D + d], cb = cn[((j + 1) & (M - 1)) * D + d];
-        x[d] = lane == 63 ? cb : ca + (cb - ca) * fi + v - fi * tot;
+    fs_lpc_brg c;
+    c.fi = (float)(lane + 1) * (1.0f / (float)FS_LPC_L);
+    c.s = HARM ? a.bs : sqrtf(eps);
+    c.wb1 = c.wb2 = c.pk = c.g = 1.0f;
+    if constexpr (HARM) {
+        const float l = a.l2q, k = (float)(lane + 1);
+        c.wb1 = exp2f((float)((lane & 15) + 1) * l);
+        c.wb2 = exp2f((float)(lane > 31 ? lane - 31 : 0) * l);
+        c.pk = exp2f(k * l);
+        c.g = exp2f(((float)FS_LPC_L - k) * l) * (1.0f - exp2f(2.0f * k * l)) / (1.0f - exp2f(2.0f * (float)FS_LPC_L * l));
     }
+    return c;
+}
+// Fine slice of this lane from its normal z and the interval's nodes ca, cb (lane 63: cb).
+template <bool HARM>
+FS_DEV float fs_lpc_bridge_x(const fs_lpc_brg &c, const fs_lp_args &a, float z, float ca, float cb, int lane)
+{
+    if constexpr (!HARM) {
+        const float v = fs_lane_scan(c.s * z);
+        const float tot = fs_readlane(v, 63);
+        return lane == 63 ? cb : ca + (cb - ca) * c.fi + v - c.fi * tot;
+    }
+    const float v = fs_lane_scan_decay(c.s * z, a.q1, a.q2, a.q4, a.q8, c.wb1, c.wb2);
+    const float tot = fs_readlane(v, 63);
+    return lane == 63 ? cb : ca * c.pk + v + c.g * (cb - ca * a.rhoL - tot);
+}
+// The potential the fine pass scores: V, less hw |x|^2 under the harmonic reference.
+template <class Sys, bool HARM> FS_DEV float fs_lpc_veff(const float *x, float hw)
+{
+    float v = (float)Sys::V(x);
+    if constexpr (HARM) {
+        float q = 0.0f;
+#pragma unroll
+        for (int d = 0; d < Sys::DOF; d++) q += x[d] * x[d];
+        v -= hw * q;
+    }
+    return v;
+}
+
+// The new fine slices of interval j on every lane (lane 63: node j + 1).
+template <int D, bool HARM>
+FS_DEV void fs_lpc_bridge(const float *cn, int j, int M, int lane, uint4 &px, fs_u32 wl, const fs_lp_args &a,
+                          const fs_lpc_brg &c, float (&x)[D])
+{
+#pragma unroll
+    for (int d = 0; d < D; d++)
+        x[d] = fs_lpc_bridge_x<HARM>(c, a, fs_lpc_fine_normal(px, j * D + d, lane, wl, a), cn[j * D + d],
+                                     cn[((j + 1) & (M - 1)) * D + d], lane);
 }
 
 // The move's anchor r of walker w: node j of the coarse level is slice r + 64 j.
@@ -693,22 +745,28 @@ FS_DEV void fs_lpc_levels(const fs_lp_args &a, const char *img, float *cnew, con
     }
 }
 
-// Change of the coarse kinetic action of walker k of a node set, Keff_new - Keff_old with
-// Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps L), on every lane (lane i takes the links i, i + 64, ...).
+// Change of the coarse action of walker k of a node set, Keff_new - Keff_old, on every lane (lane i takes
+// the links i, i + 64, ...): Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps L) for the Levy bridge, and
+// kc sum_j |c_{j+1} - rho^L c_j|^2 under the harmonic reference: minus log of the product of the
+// intervals' bridge normalisations, less the reference potential eps hw |c_j|^2 of the nodes, which the
+// fine pass scores with every other slice's V - hw |x|^2.
 template <class Sys, int WPW>
-FS_DEV float fs_lpc_dkin(const float *cnew, const float *cold, int k, int lane)
+FS_DEV float fs_lpc_dkin(const fs_lp_args &a, const float *cnew, const float *cold, int k, int lane)
 {
     typedef fs_lpc_geom<Sys, WPW> G;
     constexpr int D = G::D, M = G::M;
+    const bool harm = a.hw > 0.0f;
+    const float pL = harm ? a.rhoL : 1.0f;
     float dk = 0.0f;
     for (int i = lane; i < M * D; i += 64) {
         const int j = i / D, d = i % D, i1 = ((j + 1) & (M - 1)) * D + d;
-        const float en = cnew[k * M * D + i1] - cnew[k * M * D + i], eo = cold[k * M * D + i1] - cold[k * M * D + i];
+        const float en = cnew[k * M * D + i1] - pL * cnew[k * M * D + i];
+        const float eo = cold[k * M * D + i1] - pL * cold[k * M * D + i];
         dk += en * en - eo * eo;
     }
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) dk += __shfl_xor(dk, o, 64);
-    return dk * (0.5f / ((float)Sys::EPS * (float)G::L));
+    return dk * (harm ? a.kc : 0.5f / ((float)Sys::EPS * (float)G::L));
 }
 
 template <class Sys>
@@ -756,7 +814,7 @@ FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long
     for (int k = 0; k < WPW; k++) {
         float *dst = a.nodes + (w0 + k) * (long)G::SCR;
         for (int i = lane; i < M * D; i += 64) dst[i] = cnew[k * M * D + i];
-        const float dk = fs_lpc_dkin<Sys, WPW>(cnew, cold, k, lane);
+        const float dk = fs_lpc_dkin<Sys, WPW>(a, cnew, cold, k, lane);
         if (lane == 0) {
             dst[M * D] = lq[k] + lq[WPW + k];
             dst[M * D + 1] = dk;
@@ -767,9 +825,10 @@ FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long
 // The fine pass of one walker (one wavefront) given its new coarse nodes cn [M][D] (LDS), its log q ratio lqk
 // and coarse kinetic change dk: bridges between the nodes, log alpha, the decision, and the write of an
 // accepted path into path [D][N] (global memory, or the walker's LDS copy).  Returns the decision, the same on
-// every lane.
-template <class Sys>
-FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, float dk, float *path, long w, int lane)
+// every lane.  HARM: the harmonic reference bridge (hw > 0), else the Levy bridge.
+template <class Sys, bool HARM>
+FS_DEV bool fs_lpc_fine_path_t(const fs_lp_args &a, const float *cn, float lqk, float dk, float *path, long w,
+                               int lane)
 {
     typedef fs_lpc_geom<Sys> G;
     constexpr int N = G::N, D = G::D, L = G::L, M = G::M;
@@ -777,6 +836,8 @@ FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, fl
     const int rk = fs_lpc_anchor<N>(a, w);
     const float eps = (float)Sys::EPS;
     const float u = fs_lp_u24(fs_philox(make_uint4(0u, wl, a.sweep, FS_LP_TAG_ACCEPT), a.key0, a.key1).x);
+    const fs_lpc_brg bc = fs_lpc_bridge_coefs<HARM>(a, eps, lane);
+    const float hw = a.hw;
     float dv = 0.0f;
     uint4 px = make_uint4(0u, 0u, 0u, 0u);
     // the old slices of the next interval are loaded while this one is computed
@@ -804,10 +865,9 @@ FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, fl
             float xold[D];
 #pragma unroll
             for (int d = 0; d < D; d++) xold[d] = path[d * N + ((rk + j * L + lane + 1) & (N - 1))];
-            dv -= (float)Sys::V(xold);
+            dv -= fs_lpc_veff<Sys, HARM>(xold, hw);
         }
         float xs[M][D];
-        const float fi = (float)(lane + 1) * (1.0f / (float)L), se = sqrtf(eps);
 #pragma unroll
         for (int b = 0; b < (M * D + 3) / 4; b++) {
             const uint4 x =
@@ -815,16 +875,13 @@ FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, fl
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const int k = 4 * b + t, j = k / D, d = k % D;
-                if (k < M * D) {
-                    const float v = fs_lane_scan(se * fs_lp_normal(x, t >> 1, t & 1));
-                    const float tot = fs_readlane(v, 63);
-                    const float ca = cn[j * D + d], cb = cn[((j + 1) & (M - 1)) * D + d];
-                    xs[j][d] = lane == 63 ? cb : ca + (cb - ca) * fi + v - fi * tot;
-                }
+                if (k < M * D)
+                    xs[j][d] = fs_lpc_bridge_x<HARM>(bc, a, fs_lp_normal(x, t >> 1, t & 1), cn[j * D + d],
+                                                     cn[((j + 1) & (M - 1)) * D + d], lane);
             }
         }
 #pragma unroll
-        for (int j = 0; j < M; j++) dv += (float)Sys::V(xs[j]);
+        for (int j = 0; j < M; j++) dv += fs_lpc_veff<Sys, HARM>(xs[j], hw);
         if (decide(dv)) {
 #pragma unroll
             for (int j = 0; j < M; j++) {
@@ -845,15 +902,15 @@ FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, fl
                 xold[d] = xnext[d];
                 if (j + 1 < M) xnext[d] = path[d * N + ((rk + (j + 1) * L + lane + 1) & (N - 1))];
             }
-            fs_lpc_bridge<D>(cn, j, M, lane, px, wl, a, eps, xn);
-            dv += (float)Sys::V(xn) - (float)Sys::V(xold);
+            fs_lpc_bridge<D, HARM>(cn, j, M, lane, px, wl, a, bc, xn);
+            dv += fs_lpc_veff<Sys, HARM>(xn, hw) - fs_lpc_veff<Sys, HARM>(xold, hw);
         }
         if (decide(dv)) {
             px = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll 2
             for (int j = 0; j < M; j++) {
                 float xn[D];
-                fs_lpc_bridge<D>(cn, j, M, lane, px, wl, a, eps, xn);
+                fs_lpc_bridge<D, HARM>(cn, j, M, lane, px, wl, a, bc, xn);
                 const int s = (rk + j * L + lane + 1) & (N - 1);
 #pragma unroll
                 for (int d = 0; d < D; d++) path[d * N + s] = xn[d];
@@ -862,6 +919,13 @@ FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, fl
         }
         return false;
     }
+}
+
+template <class Sys>
+FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, float dk, float *path, long w, int lane)
+{
+    return a.hw > 0.0f ? fs_lpc_fine_path_t<Sys, true>(a, cn, lqk, dk, path, w, lane)
+                       : fs_lpc_fine_path_t<Sys, false>(a, cn, lqk, dk, path, w, lane);
 }
 
 // The fine pass of one walker from the network kernel's workspace (new nodes, log q ratio, coarse kinetic
@@ -920,7 +984,7 @@ FS_DEV void fs_lpc_fused_walker(const fs_lp_args &a, const char *img, float *wb,
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         fs_lpc_levels<Sys, 1>(am, img, cnew, cold, lq, w, lane);
-        const float dk = fs_lpc_dkin<Sys, 1>(cnew, cold, 0, lane);
+        const float dk = fs_lpc_dkin<Sys, 1>(am, cnew, cold, 0, lane);
         const float lqk = lq[0] + lq[1];
         if (fs_lpc_fine_path<Sys>(am, cnew, lqk, dk, path, w, lane)) nacc++;
     }

@@ -33,6 +33,10 @@ struct fs_lp_args {
     float *nodes;           // whole-path move: [walkers][M DOF + 2] new coarse nodes, log q ratio, dKeff
     long long w0;           // whole-path move: first walker of this launch (walkers w0 .. w0 + nMove - 1)
     long long nMove;        // whole-path move: walkers this launch moves
+    // whole-path move, harmonic reference bridge (hw = omega^2 / 2 > 0; 0: the Levy bridge): the AR(1)
+    // factor rho, its powers q1 = rho, q2, q4, q8 and rhoL = rho^64, log2 rho, bs = sqrt(rho eps) and
+    // kc = 1 / (2 V_64), V_64 the variance of a node given the one before under the reference
+    float hw, bs, q1, q2, q4, q8, rhoL, kc, l2q;
 };
 
 struct fs_lp_train_args {

@@ -63,6 +63,25 @@ def _he_init(dof, seed, hidden=128, active=128):
     return _mask_width(np.concatenate([p.reshape(-1) for p in parts]).astype(np.float32), dof, active, hidden)
 
 
+def harmonic_omega2(kernel, points=65536, h=1e-3, seed=0):
+    """omega^2 = <x . grad V(x)> / <|x|^2> over up to `points` slices of the kernel's paths, at least 0:
+    the frequency of the harmonic potential that has the same virial as V on these paths (exactly the
+    oscillator's own for a harmonic V).  x . grad V is the derivative of V(t x) at t = 1, by central
+    differences of the compiled potential (PIMCKernel.evalSystem)."""
+    t = kernel._torch
+    D, N, W = int(kernel._system.DOF), int(kernel._N), int(kernel._nbrOfWalkers)
+    rng = np.random.default_rng(seed)
+    n = min(points, W * N)
+    iw, isl = t.from_numpy(rng.integers(0, W, n)), t.from_numpy(rng.integers(0, N, n))
+    with t.cuda.stream(kernel._stream):
+        iw, isl = iw.to(kernel._device), isl.to(kernel._device)
+        x = kernel._paths.view(W, D, N)[iw, :, isl].double().cpu().numpy()      # (n, D)
+    vp, _ = kernel.evalSystem(x * (1.0 + h))
+    vm, _ = kernel.evalSystem(x * (1.0 - h))
+    xg = (vp - vm) / (2.0 * h)
+    return max(0.0, float(xg.mean() / (x * x).sum(axis=1).mean()))
+
+
 class LearnedProposal:
     """Learned bisection sweeps on the paths of one PIMCKernel.
 
@@ -79,10 +98,15 @@ class LearnedProposal:
     seed:     Philox key of the proposal's random numbers (default: the kernel's seed + 1)
     atoms3d:  centre the inputs on the atoms' mean (default: system.translationInvariant)
     inScale:  scale of the input coordinates
+    bridgeOmega2: the whole-path move's bridges: 0 (default) the free-particle (Levy) bridge; omega^2 > 0
+              the exact bridge of the kinetic links plus a harmonic reference omega^2 |x|^2 / 2 per
+              slice (the accept test then scores V - omega^2 |x|^2 / 2); "auto" takes omega^2 =
+              <x . grad V> / <|x|^2> over the kernel's current paths (at least 0), the harmonic
+              potential's own frequency for an oscillator
     """
 
     def __init__(self, kernel, batch=65536, lr=1e-3, seed=None, atoms3d=None, inScale=1.0, segsPerWave=0,
-                 initSeed=0, fallbackAcceptance=0.2, coarse=False, width=None):
+                 initSeed=0, fallbackAcceptance=0.2, coarse=False, width=None, bridgeOmega2=0.0):
         if kernel._enableDouble:
             raise Exception("the learned proposal runs on FP32 paths")
         if kernel._engine is None:
@@ -107,6 +131,12 @@ class LearnedProposal:
         self.coarse = bool(coarse)
         self.width = int(width if width is not None else (64 if coarse else 128))
         lp.width = self.width
+        if isinstance(bridgeOmega2, str):
+            if bridgeOmega2 != "auto":
+                raise ValueError("bridgeOmega2 must be a number or 'auto'")
+            bridgeOmega2 = harmonic_omega2(kernel)
+        self.bridgeOmega2 = float(bridgeOmega2)
+        lp.bridge_omega2 = self.bridgeOmega2
         # moves proposed per sweep: every segment, or (whole-path move) every path
         self._movesPerSweep = kernel._nbrOfWalkers * (1 if coarse else kernel._N >> kernel._S)
         self._P = _native.Proposal(kernel._cfg, kernel._systemSource, lp)

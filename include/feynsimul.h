@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define FS_ABI_VERSION 5
+#define FS_ABI_VERSION 6
 
 enum {
     FS_OK = 0,
@@ -275,6 +275,9 @@ typedef struct fs_lp_config {
                                  training compute on these only; the parameters keep the 128-wide layout,
                                  and units beyond the active width must be zero (training keeps them zero,
                                  their gradients being exactly zero) */
+    float bridge_omega2;      /* whole-path move: omega^2 of a harmonic reference, omega^2 |x|^2 / 2 per slice,
+                                 whose exact bridge fills the intervals between the nodes (the accept test then
+                                 scores V - omega^2 |x|^2 / 2); 0 = the free-particle (Levy) bridge (ABI 6) */
 } fs_lp_config;
 typedef struct fs_lp_info {
     int64_t n_params;         /* FP32 master parameters (and each Adam moment) */

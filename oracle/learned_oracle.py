@@ -346,31 +346,99 @@ def lane_scan(v):
     return (v + t).astype(np.float32)
 
 
-def bridge(cn, z, eps):
+def harmonic_coefs(eps, omega2):
+    """The harmonic reference bridge's constants (float64): rho with rho + 1/rho = 2 + eps^2 omega^2 (the
+    AR(1) chain x_{i+1} = rho x_i + sqrt(rho eps) z_i, conditioned on its end, is the Gaussian bridge of
+    the links' kinetic action plus eps omega^2 x^2 / 2 on the interior slices), rho^64 and
+    V_64 = rho eps (1 - rho^128) / (1 - rho^2), the variance of x_64 given x_0.  omega2 = 0: the free
+    bridge, rho = 1, V_64 = 64 eps."""
+    if omega2 == 0.0:
+        return 1.0, 1.0, eps * FINE_L
+    e2w2 = eps * eps * omega2
+    rho = 1.0 + 0.5 * e2w2 - np.sqrt(e2w2 * (1.0 + 0.25 * e2w2))
+    return rho, rho ** FINE_L, rho * eps * (1.0 - rho ** (2 * FINE_L)) / (1.0 - rho * rho)
+
+
+def decay_scan(v, q):
+    """Inclusive decayed prefix sum over axis 2 (64 lanes), w_i = sum_{m <= i} q^(i - m) v_m, in the
+    order of `lane_scan` with each shifted term weighted by q to the power of its distance (the GPU's
+    FMA form; q = 1 is `lane_scan`)."""
+    v = v.astype(np.float32).copy()
+    lane = np.arange(FINE_L)
+    for o in (1, 2, 4, 8):
+        t = np.zeros_like(v)
+        t[:, :, o:, :] = v[:, :, :-o, :]
+        t[:, :, (lane % 16) < o, :] = 0.0
+        v = (v + np.float32(q ** o) * t).astype(np.float32)
+    w1 = np.float32(q) ** ((lane & 15) + 1).astype(np.float32)
+    t = np.zeros_like(v)
+    t[:, :, 16:32, :] = v[:, :, 15:16, :]
+    t[:, :, 48:64, :] = v[:, :, 47:48, :]
+    v = (v + w1[None, None, :, None] * t).astype(np.float32)
+    w2 = np.float32(q) ** np.maximum(lane - 31, 0).astype(np.float32)
+    t = np.zeros_like(v)
+    t[:, :, 32:64, :] = v[:, :, 31:32, :]
+    return (v + w2[None, None, :, None] * t).astype(np.float32)
+
+
+def bridge(cn, z, eps, omega2=0.0):
     """Fine slices of every interval: cn (W, M + 1, D) nodes, z (W, M, 64, D) -> (W, M, 64, D), lane 63
-    the node j + 1: x_i = c_j + (c_{j+1} - c_j) i / 64 + W_i - (i / 64) W_64, W the prefix sums
-    (`lane_scan`) of the increments sqrt(eps) z."""
-    v = lane_scan((np.float32(np.sqrt(np.float32(eps))) * z).astype(np.float32))
-    tot = v[:, :, FINE_L - 1:FINE_L, :]
-    fi = (np.arange(1, FINE_L + 1, dtype=np.float32) / np.float32(FINE_L))[None, None, :, None]
+    the node j + 1.  omega2 = 0: the Levy bridge x_i = c_j + (c_{j+1} - c_j) i / 64 + W_i - (i / 64) W_64,
+    W the prefix sums (`lane_scan`) of the increments sqrt(eps) z.  omega2 > 0: the harmonic reference
+    bridge (`harmonic_coefs`) x_i = c_j rho^i + W_i + g_i (c_{j+1} - c_j rho^64 - W_64), W the AR(1)
+    chain of `decay_scan` on sqrt(rho eps) z, g_i = rho^(64 - i) (1 - rho^(2i)) / (1 - rho^128)."""
     ca, cb = cn[:, :-1, None, :], cn[:, 1:, None, :]
-    x = (ca + (cb - ca) * fi + v - fi * tot).astype(np.float32)
+    if omega2 == 0.0:
+        v = lane_scan((np.float32(np.sqrt(np.float32(eps))) * z).astype(np.float32))
+        tot = v[:, :, FINE_L - 1:FINE_L, :]
+        fi = (np.arange(1, FINE_L + 1, dtype=np.float32) / np.float32(FINE_L))[None, None, :, None]
+        x = (ca + (cb - ca) * fi + v - fi * tot).astype(np.float32)
+    else:
+        rho, rhoL, _ = harmonic_coefs(eps, omega2)
+        v = decay_scan((np.float32(np.sqrt(rho * eps)) * z).astype(np.float32), rho)
+        tot = v[:, :, FINE_L - 1:FINE_L, :]
+        k = np.arange(1, FINE_L + 1, dtype=np.float64)
+        pk = (rho ** k).astype(np.float32)[None, None, :, None]
+        g = (rho ** (FINE_L - k) * (1.0 - rho ** (2 * k)) / (1.0 - rho ** (2 * FINE_L))).astype(np.float32)
+        g = g[None, None, :, None]
+        x = (ca * pk + v + g * (cb - ca * np.float32(rhoL) - tot)).astype(np.float32)
     x[:, :, FINE_L - 1, :] = cb[:, :, 0, :]
     return x
 
 
+def link_action(c, eps, omega2=0.0):
+    """sum_j |c_{j+1} - rho^64 c_j|^2 / (2 V_64) over a cyclic node chain c (W, M + 1, D) (node M = node 0),
+    float64 (omega2 = 0: sum_j |c_{j+1} - c_j|^2 / (2 eps 64)): the coarse action the whole-path move
+    scores beside V - omega2 |x|^2 / 2 on every slice, nodes included."""
+    rho, rhoL, VL = harmonic_coefs(eps, omega2)
+    c = c.astype(np.float64)
+    return np.sum((c[:, 1:, :] - rhoL * c[:, :-1, :]) ** 2, axis=(1, 2)) / (2.0 * VL)
+
+
+def coarse_action(c, eps, omega2=0.0):
+    """The minimum over the fine slices of each interval's links' kinetic action plus eps omega^2 |x|^2 / 2
+    on its interior slices, summed over the intervals of a cyclic node chain c, float64:
+    link_action - eps omega^2 / 2 sum_j |c_j|^2 (the reference potential of the nodes themselves is not in
+    it; the move scores it with the rest of V - omega2 |x|^2 / 2)."""
+    k = link_action(c, eps, omega2)
+    if omega2 != 0.0:
+        k -= 0.5 * eps * omega2 * np.sum(c[:, :-1, :].astype(np.float64) ** 2, axis=(1, 2))
+    return k
+
+
 def coarse_move(paths, p, V, *, eps, seed, sweep, atoms3d=False, in_scale=1.0, emulate_bf16=True,
-                return_detail=False, walker0=0):
+                return_detail=False, walker0=0, omega2=0.0):
     """The whole-path move: paths float32 (W, D, N), N = 64 M, M = 2^LC, updated in place.
 
     Anchor r (Philox (0, walker, sweep, TAG_OFFSET) mod N); coarse nodes c_j = x(r + 64 j), node M =
     node 0.  Levels n = 1..LC of the bisection of the whole path (S = log2 N: sigma_n^2 = N eps /
     (2 * 2^n), level feature n / S) draw the new nodes from the network with normals counted by
-    (node j, walker, sweep, TAG_NORMAL | block), and score the old ones (the reverse rows); a Levy
-    bridge (`bridge`, normals `fine_normals`) fills each interval.  The bridge samples the fine
-    links' kinetic action exactly given the nodes, so
-        log alpha = -eps (V_new - V_old) - (Keff_new - Keff_old) + log q_old - log q_new,
-        Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps 64).
+    (node j, walker, sweep, TAG_NORMAL | block), and score the old ones (the reverse rows); a bridge
+    (`bridge`, normals `fine_normals`) fills each interval.  The bridge samples the fine links' kinetic
+    action plus the harmonic reference eps omega2 |x|^2 / 2 on its interior exactly given the nodes, so
+        log alpha = -eps [(V - omega2 |x|^2 / 2)_new - (...)_old] - (Keff_new - Keff_old) + log q_old - log q_new
+    with the potential sums over every slice and Keff = `link_action` (omega2 = 0: the Levy bridge,
+    Keff = sum_j |c_{j+1} - c_j|^2 / (2 eps 64)).
     Returns accepted moves per walker (and, with return_detail, the parts of log alpha)."""
     W, D, N = paths.shape
     S = int(np.log2(N))
@@ -406,12 +474,12 @@ def coarse_move(paths, p, V, *, eps, seed, sweep, atoms3d=False, in_scale=1.0, e
                 zo = ((cold[:, m, :] - mu) / sd).astype(np.float32)
                 lq += np.sum(-0.5 * zo * zo - s, axis=(1, 2)).astype(np.float32)
         cnew[:, M, :] = cnew[:, 0, :]
-    xn = bridge(cnew, fine_normals(seed, wg, sweep, M, D), eps)                   # (W, M, 64, D)
+    xn = bridge(cnew, fine_normals(seed, wg, sweep, M, D), eps, omega2)           # (W, M, 64, D)
     fine = (r[:, None, None] + FINE_L * np.arange(M)[None, :, None] + 1 + np.arange(FINE_L)[None, None, :]) % N
     xo = paths[wi[:, None, None, None], np.arange(D)[None, None, None, :], fine[..., None]].astype(np.float32)
-    dV = (V(xn.astype(np.float64)).sum(axis=(1, 2)) - V(xo.astype(np.float64)).sum(axis=(1, 2))).astype(np.float32)
-    keff = lambda c: np.sum((c[:, 1:, :] - c[:, :-1, :]).astype(np.float64) ** 2, axis=(1, 2)) / (2.0 * eps * FINE_L)
-    dK = (keff(cnew) - keff(cold)).astype(np.float32)
+    Veff = lambda x: V(x) - 0.5 * omega2 * np.sum(x * x, axis=-1)
+    dV = (Veff(xn.astype(np.float64)).sum(axis=(1, 2)) - Veff(xo.astype(np.float64)).sum(axis=(1, 2))).astype(np.float32)
+    dK = (link_action(cnew, eps, omega2) - link_action(cold, eps, omega2)).astype(np.float32)
     logA = (-dK - np.float32(eps) * dV + lq).astype(np.float32)
     u = accept_uniforms(seed, wg, sweep, 0)
     with np.errstate(over="ignore"):

@@ -88,14 +88,14 @@ def _setup(case, rng):
     return system, W, N, beta, D, paths0, _lm2m2_V(system), 0.05
 
 
-def _run_case(case, theta_scale=None, seed=777, sweep=3, width=None):
+def _run_case(case, theta_scale=None, seed=777, sweep=3, width=None, omega2=0.0):
     rng = np.random.default_rng(11)
     system, W, N, beta, D, paths0, V, scale = _setup(case, rng)
     if theta_scale is not None:
         scale = theta_scale
     k = _kernel(system, W, N, beta)
     k.setPaths(paths0.reshape(W, D * N))
-    lp = LearnedProposal(k, batch=2048, seed=seed, coarse=True, width=width)
+    lp = LearnedProposal(k, batch=2048, seed=seed, coarse=True, width=width, bridgeOmega2=omega2)
     theta = L.init_params(D, 5, scale_last=scale, active=lp.width)
     lp.setParameters(theta)
     lp._sweepCount = sweep
@@ -105,7 +105,7 @@ def _run_case(case, theta_scale=None, seed=777, sweep=3, width=None):
     tr = lp.getTrace()[:, 0, :]
     ref = paths0.copy()
     _, d = L.coarse_move(ref, L.unflatten(theta, D), V, eps=beta / N, seed=seed, sweep=sweep, atoms3d=(D % 3 == 0),
-                         return_detail=True)
+                         return_detail=True, omega2=omega2)
     return paths0, got, ref, tr, d
 
 
@@ -113,12 +113,17 @@ def _run_case(case, theta_scale=None, seed=777, sweep=3, width=None):
 # drawn a second time from the same counters instead of being held in registers
 # The network kernel copies only the active units' weights and biases into LDS and picks its wavefronts per
 # block by occupancy, which both depend on the width: the default 64 units, and 32, 96 and 128 below.
-@pytest.mark.parametrize("case,width", [("ho128", None), ("ho256", None), ("aho512", None), ("he3", None),
-                                        ("ho8192", None), ("he3_512", None), ("aho512", 32), ("he3", 32),
-                                        ("ho256", 96), ("aho512", 128), ("he2", None)])
-def test_coarse_move_matches_oracle(gpu, case, width):
-    _case("whole-path move %s width %s" % (case, width))
-    paths0, got, ref, tr, d = _run_case(case, width=width)
+# the last five: the harmonic reference bridge (omega2 > 0), in the fused move (ho256, he2), with registers
+# holding the new slices (aho2048) and with the bridges drawn twice (ho8192, he3_512)
+@pytest.mark.parametrize("case,width,omega2", [("ho128", None, 0.0), ("ho256", None, 0.0), ("aho512", None, 0.0),
+                                               ("he3", None, 0.0), ("ho8192", None, 0.0), ("he3_512", None, 0.0),
+                                               ("aho512", 32, 0.0), ("he3", 32, 0.0), ("ho256", 96, 0.0),
+                                               ("aho512", 128, 0.0), ("he2", None, 0.0),
+                                               ("ho256", 32, 1.0), ("he2", 32, 0.3), ("aho2048", 32, 1.5),
+                                               ("ho8192", 32, 0.6), ("he3_512", 32, 0.2)])
+def test_coarse_move_matches_oracle(gpu, case, width, omega2):
+    _case("whole-path move %s width %s omega2 %s" % (case, width, omega2))
+    paths0, got, ref, tr, d = _run_case(case, width=width, omega2=omega2)
     W = paths0.shape[0]
     for name, col, tol in (("lq", 0, 1e-3), ("dK", 1, 1e-3), ("dV", 2, 1e-3)):
         err = np.abs(tr[:, col] - d[name])
@@ -132,7 +137,7 @@ def test_coarse_move_matches_oracle(gpu, case, width):
     assert np.array_equal(changed[clear], d["accept"][clear])
     for w in np.nonzero(changed & d["accept"])[0]:
         assert np.abs(got[w] - ref[w]).max() < 5e-3
-    if case in ("ho128", "aho512", "ho8192"):
+    if case in ("ho128", "aho512", "ho8192", "aho2048"):
         assert 0 < d["accept"].sum() < W
 
 
@@ -148,16 +153,18 @@ def test_zero_output_is_the_levy_bridge(gpu):
     np.testing.assert_allclose(d["lq"], d["dK"], rtol=1e-4, atol=1e-4)
 
 
-def test_coarse_move_keeps_ho_energy(gpu):
+@pytest.mark.parametrize("omega2", [0.0, 0.6])
+def test_coarse_move_keeps_ho_energy(gpu, omega2):
     """C1 sizes (N = 256, beta = 10, 4096 chains): whole-path moves alone, with the network trained
-    online on the coarse nodes, sample the exact discretised <x^2> within 4 standard errors."""
-    _case("whole-path move, C1 energy")
+    online on the coarse nodes, sample the exact discretised <x^2> within 4 standard errors, with the Levy
+    bridge and with a harmonic reference bridge of another frequency than the oscillator's."""
+    _case("whole-path move, C1 energy, omega2 %s" % omega2)
     system = HarmOsc()
     W, N, beta = 4096, 256, 10.0
     k = _kernel(system, W, N, beta, opRuns=1, M=1500, S=4)
     k.setPaths(np.random.default_rng(0).uniform(-1, 1, (W, N)).astype(np.float32))
     k.run()
-    lp = LearnedProposal(k, batch=16384, seed=31, coarse=True, fallbackAcceptance=0.0)
+    lp = LearnedProposal(k, batch=16384, seed=31, coarse=True, fallbackAcceptance=0.0, bridgeOmega2=omega2)
     first = lp.trainOnline(1)
     last = lp.trainOnline(99)
     ar = lp.sweep(64)
@@ -234,12 +241,13 @@ def test_two_stream_halves_match_one_stream(gpu, monkeypatch):
     assert np.array_equal(out[0][2], out[1][2])
 
 
-@pytest.mark.parametrize("case,width", [("ho256", 32), ("aho512", 64), ("ho128", 128), ("he2", 32)])
-def test_fused_small_path_move_matches_two_kernel_move(gpu, monkeypatch, case, width):
+@pytest.mark.parametrize("case,width,omega2", [("ho256", 32, 0.0), ("aho512", 64, 0.0), ("ho128", 128, 0.0),
+                                               ("he2", 32, 0.0), ("aho512", 32, 1.4), ("he2", 32, 0.3)])
+def test_fused_small_path_move_matches_two_kernel_move(gpu, monkeypatch, case, width, omega2):
     """Paths of at most 1024 floats take the fused move (one wavefront per walker, the path in LDS, every move
     of a call in one launch); its random numbers, network rows and sums are the two-kernel move's, so five
     moves in a row give the same paths, accept counts and last-move trace bit for bit."""
-    _case("fused whole-path move %s width %d against the two-kernel move" % (case, width))
+    _case("fused whole-path move %s width %d omega2 %s against the two-kernel move" % (case, width, omega2))
     rng = np.random.default_rng(17)
     system, W, N, beta, D, paths0, V, scale = _setup(case, rng)
     theta = L.init_params(D, 5, scale_last=scale, active=width)
@@ -248,7 +256,7 @@ def test_fused_small_path_move_matches_two_kernel_move(gpu, monkeypatch, case, w
         monkeypatch.setenv("FS_LPC_FUSED", fused)
         k = _kernel(system, W, N, beta)
         k.setPaths(paths0.reshape(W, D * N))
-        lp = LearnedProposal(k, batch=2048, seed=321, coarse=True, width=width)
+        lp = LearnedProposal(k, batch=2048, seed=321, coarse=True, width=width, bridgeOmega2=omega2)
         lp.setParameters(theta)
         lp._sweepCount = 11
         lp.traceSweeps()
@@ -293,3 +301,22 @@ def test_moves_of_one_call_match_single_move_calls(gpu, case):
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_harmonic_reference_of_the_oscillator_leaves_no_fine_action(gpu):
+    """With the reference at the oscillator's own frequency (V = x^2 / 2, omega2 = 1, as "auto" finds it) the
+    potential part of log alpha, sum (V - x^2 / 2) over the slices, vanishes up to rounding, so the
+    decision rests on the coarse action and the network's density alone."""
+    _case("whole-path move, harmonic reference of the oscillator")
+    rng = np.random.default_rng(5)
+    system, W, N, beta, D, paths0, V, scale = _setup("ho256", rng)
+    k = _kernel(system, W, N, beta)
+    k.setPaths(paths0.reshape(W, D * N))
+    lp = LearnedProposal(k, batch=2048, seed=9, coarse=True, width=32, bridgeOmega2="auto")
+    assert abs(lp.bridgeOmega2 - 1.0) < 1e-3
+    lp.setParameters(L.init_params(D, 5, scale_last=scale, active=32))
+    lp.traceSweeps()
+    lp.sweep(1)
+    tr = lp.getTrace()[:, 0, :]
+    print("max |dV| %.3e, max |dK| %.3e" % (np.abs(tr[:, 2]).max(), np.abs(tr[:, 1]).max()))
+    assert np.abs(tr[:, 2]).max() < 1e-3 * max(1.0, np.abs(tr[:, 1]).max())

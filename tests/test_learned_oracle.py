@@ -182,6 +182,118 @@ def test_bridge_law():
     assert abs(np.mean(dx[:, 1:] * dx[:, :-1]) / eps) < 0.05
 
 
+def _harmonic_precision(NL, eps, omega2):
+    """Precision matrix of the interior slices 1 .. NL - 1 of one interval under the links' kinetic action
+    plus eps omega2 x^2 / 2 on the interior (the ends fixed)."""
+    T = np.zeros((NL - 1, NL - 1))
+    for i in range(NL - 1):
+        T[i, i] = 2.0 / eps + eps * omega2
+        if i > 0:
+            T[i, i - 1] = T[i - 1, i] = -1.0 / eps
+    return T
+
+
+@pytest.mark.parametrize("omega2", [0.0, 0.7, 2.5])
+def test_harmonic_bridge_is_the_gaussian_conditional(omega2):
+    """The bridge is linear in its normals: fed unit normals it gives the columns of a map A, and A A^T must
+    be the inverse of the interior precision matrix (numpy's inverse of the tridiagonal matrix, not the
+    closed forms the bridge uses); fed zero normals it must give the conditional mean T^-1 r with
+    r = (a e_1 + b e_63) / eps.  A wrong rho, a missing end correction or a wrong increment variance fails."""
+    NL, eps = 64, 0.04
+    T = _harmonic_precision(NL, eps, omega2)
+    C = np.linalg.inv(T)
+    z = np.zeros((NL, 1, NL, 1), np.float32)
+    z[np.arange(NL), 0, np.arange(NL), 0] = 1.0
+    cn = np.zeros((NL, 2, 1), np.float32)
+    A = L.bridge(cn, z, eps, omega2)[:, 0, :NL - 1, 0].astype(np.float64).T      # (63 slices, 64 normals)
+    np.testing.assert_allclose(A @ A.T, C, rtol=2e-4, atol=2e-5 * C.max())
+    a, b = 0.7, -1.3
+    cn = np.array([[[a], [b]]], np.float32)
+    x = L.bridge(cn, np.zeros((1, 1, NL, 1), np.float32), eps, omega2)[0, 0, :, 0].astype(np.float64)
+    r = np.zeros(NL - 1)
+    r[0], r[-1] = a / eps, b / eps
+    np.testing.assert_allclose(x[:NL - 1], np.linalg.solve(T, r), rtol=1e-5, atol=1e-5)
+    assert x[NL - 1] == np.float32(b)
+
+
+@pytest.mark.parametrize("omega2", [0.0, 0.7, 2.5])
+def test_coarse_action_is_the_minimum_of_the_interval_actions(omega2):
+    """coarse_action of a cyclic node chain equals the sum over the intervals of the minimum, over the
+    interior slices, of the links' kinetic action plus eps omega2 x^2 / 2 on the interior (computed by
+    solving the interior's linear system), exactly: the Metropolis-Hastings ratio of the whole-path move
+    rests on it."""
+    NL, eps, M = 64, 0.04, 4
+    T = _harmonic_precision(NL, eps, omega2)
+    c = np.random.default_rng(2).standard_normal((3, M + 1, 1))
+    c[:, M] = c[:, 0]
+    tot = np.zeros(3)
+    for w in range(3):
+        for j in range(M):
+            a, b = c[w, j, 0], c[w, j + 1, 0]
+            r = np.zeros(NL - 1)
+            r[0], r[-1] = a / eps, b / eps
+            y = np.linalg.solve(T, r)
+            xs = np.concatenate([[a], y, [b]])
+            tot[w] += np.sum(np.diff(xs) ** 2) / (2 * eps) + 0.5 * eps * omega2 * np.sum(y ** 2)
+    np.testing.assert_allclose(L.coarse_action(c, eps, omega2), tot, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("omega2", [0.0, 0.6, 1.5])
+def test_coarse_log_alpha_is_the_hastings_ratio(omega2):
+    """log alpha of the whole-path move, less the network's log q_old - log q_new, equals what the target and
+    the bridges' densities give independently: -S(x') + S(x) + sum_j [log p(f_j | c) - log p(f'_j | c')],
+    with S the discretised action of the whole path and p(f | a, b) the Gaussian of an interval's interior
+    slices under its links' kinetic action and eps omega2 x^2 / 2 inside (numpy's linear algebra, not the
+    closed forms of the move).  A node's reference potential counted twice, or a missing coarse term, fails
+    it (C1's eps = 0.039, where such a slip biased <x^2> by 8 standard errors)."""
+    W, N, beta = 12, 256, 10.0
+    eps = beta / N
+    V = lambda x: 0.5 * x[..., 0] ** 2 + 0.1 * x[..., 0] ** 4
+    rng = np.random.default_rng(6)
+    paths = (0.5 * rng.standard_normal((W, 1, N))).astype(np.float32)
+    old = paths.copy()
+    p = L.unflatten(L.init_params(1, 3, scale_last=0.2), 1)
+    _, d = L.coarse_move(paths, p, V, eps=eps, seed=8, sweep=1, return_detail=True, omega2=omega2)
+    new = old.copy()
+    for w in range(W):
+        new[w, 0, d["fine"][w].reshape(-1)] = d["new_fine"][w, :, :, 0].reshape(-1)
+    T = _harmonic_precision(64, eps, omega2)
+    _, logdet = np.linalg.slogdet(T / (2 * np.pi))
+
+    def action(x):
+        x = x[:, 0, :].astype(np.float64)
+        return np.sum((np.roll(x, -1, axis=1) - x) ** 2, axis=1) / (2 * eps) + eps * V(x[..., None]).sum(axis=1)
+
+    def log_bridges(x, r):
+        x = x[:, 0, :].astype(np.float64)
+        out = np.zeros(W)
+        for w in range(W):
+            for j in range(N // 64):
+                s0 = r[w] + 64 * j
+                a, b = x[w, s0 % N], x[w, (s0 + 64) % N]
+                f = x[w, (s0 + 1 + np.arange(63)) % N]
+                rr = np.zeros(63)
+                rr[0], rr[-1] = a / eps, b / eps
+                e = f - np.linalg.solve(T, rr)
+                out[w] += -0.5 * e @ T @ e + 0.5 * logdet
+        return out
+
+    want = -action(new) + action(old) + log_bridges(old, d["r"]) - log_bridges(new, d["r"]) + d["lq"]
+    print("max |logA - want| %.2e, max |logA| %.2e" % (np.abs(d["logA"] - want).max(), np.abs(want).max()))
+    np.testing.assert_allclose(d["logA"], want, rtol=2e-3, atol=2e-3)
+
+
+def test_decay_scan_is_a_decayed_prefix_sum():
+    v = np.random.default_rng(4).standard_normal((2, 2, 64, 1)).astype(np.float32)
+    q = 0.97
+    ref = np.zeros_like(v, dtype=np.float64)
+    acc = np.zeros((2, 2, 1))
+    for i in range(64):
+        acc = q * acc + v[:, :, i, :]
+        ref[:, :, i, :] = acc
+    np.testing.assert_allclose(L.decay_scan(v, q), ref, rtol=1e-5, atol=1e-5)
+
+
 def test_coarse_training_rows_are_coarse_nodes():
     w, s0, p = L.train_rows(5, 3, 8192, 100, 1024, 1024, fine=L.FINE_L)
     assert (p % 64 == 0).all() and p.min() == 64 and p.max() == 1024 - 64
@@ -189,12 +301,15 @@ def test_coarse_training_rows_are_coarse_nodes():
     assert w.min() >= 0 and w.max() < 100
 
 
-def test_coarse_poor_network_keeps_the_exact_law():
+@pytest.mark.parametrize("omega2", [0.0, 1.0, 0.4])
+def test_coarse_poor_network_keeps_the_exact_law(omega2):
     """256-slice HO ring (four coarse nodes per path, two coarse levels), a random network whose coarse proposals are off
     by about 0.2 sigma: started in the exact Gaussian of the discretised action, whole-path moves keep
-    it (1.8 % of the variance at most).  Planting a dropped coarse kinetic term, a dropped reverse
+    it (1.8 % of the variance at most), with the Levy bridge (omega2 = 0), the harmonic reference bridge
+    of the oscillator's own frequency (omega2 = 1: the fine slices drop out of log alpha) and a
+    mismatched one (omega2 = 0.4).  Planting a dropped coarse kinetic term, a dropped reverse
     density or a bridge of variance 2 eps in the oracle moves the covariance by 23 %, 22 % and 16 % of
-    the variance in the same run."""
+    the variance in the same run (omega2 = 0)."""
     N, beta, W, T = 256, 2.0, 8192, 24
     eps = beta / N
     Ce = _cov_exact(N, beta)
@@ -204,7 +319,7 @@ def test_coarse_poor_network_keeps_the_exact_law():
     acc = 0
     samples = []
     for t in range(T):
-        acc += L.coarse_move(paths, p, _ho_V, eps=eps, seed=17, sweep=t).sum()
+        acc += L.coarse_move(paths, p, _ho_V, eps=eps, seed=17, sweep=t, omega2=omega2).sum()
         if t >= T - 8:
             samples.append(paths[:, 0, ::16].astype(np.float64).copy())
     X = np.concatenate(samples)
