@@ -407,7 +407,8 @@ def lpc_bench(steps, warmup, width=None, train_steps=500):
                               sampler="wave", pathStorage="memory"))
     k.setPaths(np.random.default_rng(40).uniform(-1, 1, (AHO_W, AHO_N)).astype(np.float32))
     k.run()
-    lp = LearnedProposal(k, batch=65536, lr=1e-3, seed=44, coarse=True, width=width)
+    lp = LearnedProposal(k, batch=65536, lr=1e-3, seed=44, coarse=True, width=width,
+                         bridgeOmega2=FIXED_SIGMA_LEARNED["aho"]["bridge"])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     lp.trainOnline(train_steps, plainSweepsPerStep=1)
@@ -433,7 +434,8 @@ def lpc_bench(steps, warmup, width=None, train_steps=500):
             "warmup": warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "bf16/f32", "data": "synthetic: anharmonic-oscillator paths after 2048 plain sweeps",
             "config": {"workload": "anharmonic oscillator x^2/2 + 0.1 x^4, N=4096, beta=40, 65536 chains, "
-                                   "one whole-path move per step, %d hidden units, %d Adam steps" % (width, train_steps)},
+                                   "one whole-path move per step, %d hidden units, %d Adam steps, harmonic "
+                                   "reference bridges (omega^2 %.4g)" % (width, train_steps, lp.bridgeOmega2)},
             "acceptance": ar,
             "training": {"adam_steps": train_steps, "plain_sweeps_per_step": 1, "seconds": t_train,
                          "ms_per_adam_step": ms_adam, "batch": 65536},
@@ -500,8 +502,9 @@ def run_to_sigma(block, get_series, sigma, exact, max_seconds, measurements_per_
 # trained online (Adam steps, each after plain sweeps)
 # (one plain sweep per Adam step: on the anharmonic workload 500 steps reach the acceptance of 500 steps
 # after two sweeps each, 0.41 against 0.42, in 0.47 s of training instead of 0.76 s)
-FIXED_SIGMA_LEARNED = {"ho": {"train_steps": 300, "batch": 16384, "plain_sweeps_per_step": 1},
-                       "aho": {"train_steps": 500, "batch": 65536, "plain_sweeps_per_step": 1}}
+# bridge: the whole-path move's harmonic reference, omega^2 = <x . grad V> / <|x|^2> of the start paths
+FIXED_SIGMA_LEARNED = {"ho": {"train_steps": 300, "batch": 16384, "plain_sweeps_per_step": 1, "bridge": "auto"},
+                       "aho": {"train_steps": 500, "batch": 65536, "plain_sweeps_per_step": 1, "bridge": "auto"}}
 LEARNED_WIDTH = 32
 # Measurement cadences each leg is run at (sweeps or moves per E0 measurement): a measurement costs about
 # as much as reading every path once, so measuring after every sweep wastes time, while measuring far
@@ -530,7 +533,7 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
     import torch
     from feynsimul_amd.kernel_args import KernelArgs
     from feynsimul_amd.kernel import PIMCKernel
-    from feynsimul_amd.learned import LearnedProposal
+    from feynsimul_amd.learned import LearnedProposal, harmonic_omega2
     from feynsimul_amd.physical_systems import HarmOsc, AnharmOsc
     from feynsimul_amd.stats import integrated_autocorr_time
     if workload == "ho":
@@ -609,9 +612,15 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
     def learned_leg(width, cadences):
         kl = kernel(SWEEPS_PER_STEP, runs, 43)
         kl.setPaths(start)
-        lp = LearnedProposal(kl, batch=cfg["batch"], lr=1e-3, seed=44, coarse=True, width=width)
+        # the reference's omega^2 from the start paths (counted with the training time)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        omega2 = harmonic_omega2(kl) if cfg["bridge"] == "auto" else float(cfg["bridge"])
+        t_omega = time.perf_counter() - t0
+        lp = LearnedProposal(kl, batch=cfg["batch"], lr=1e-3, seed=44, coarse=True, width=width,
+                             bridgeOmega2=omega2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter() - t_omega
         first = lp.train(1)
         last = lp.trainOnline(cfg["train_steps"] - 1, plainSweepsPerStep=cfg["plain_sweeps_per_step"])
         torch.cuda.synchronize()
@@ -633,7 +642,7 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
         r = leg(k_best, max(max_seconds - t_train, 1.0))
         r.update({"seconds_train": t_train, "adam_steps": cfg["train_steps"], "training_batch": cfg["batch"],
                   "plain_sweeps_per_adam_step": cfg["plain_sweeps_per_step"],
-                  "nll_first": first, "nll_last": last, "hidden_units": width,
+                  "nll_first": first, "nll_last": last, "hidden_units": width, "bridge_omega2": lp.bridgeOmega2,
                   "cadence_scan_ess_per_s": {str(k): v["ess_per_s"] for k, v in scan.items()},
                   "cadence_scan_tau_int": {str(k): v["tau_int_measurements"] for k, v in scan.items()}})
         del kl, lp
@@ -650,9 +659,11 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
     out = {"workload": {"ho": "harmonic oscillator N=256, beta=10, 4096 chains",
                          "aho": "anharmonic oscillator x^2/2 + 0.1 x^4, N=4096, beta=40, 65536 chains"}[workload]
                         + ", plain S=%d, E0 every %d sweeps (the best of %s)" % (S, k_plain, list(PLAIN_CADENCES[workload])),
-            "learned_form": "whole-path move: network on the nodes every 64 slices (%d hidden units), Levy "
-                            "bridges between them, one MH decision per path, E0 every %d moves (the best of %s)"
-                            % (LEARNED_WIDTH, learned["moves_per_measurement"], list(LEARNED_CADENCES)),
+            "learned_form": "whole-path move: network on the nodes every 64 slices (%d hidden units), harmonic "
+                            "reference bridges between them (omega^2 %.4g from the start paths), one MH decision "
+                            "per path, E0 every %d moves (the best of %s)"
+                            % (LEARNED_WIDTH, learned["bridge_omega2"], learned["moves_per_measurement"],
+                               list(LEARNED_CADENCES)),
             "sigma": sigma, "E0_exact": exact, "plain": plain, "learned": learned,
             "ratio_learned_over_plain": learned["ess_per_s"] / plain["ess_per_s"],
             "ratio_without_training": learned["ess_per_s_without_training"] / plain["ess_per_s"],
@@ -1168,6 +1179,8 @@ def main():
     ap.add_argument("--aho-sigma", type=float, default=AHO_SIGMA, help="standard error target of E0 on the anharmonic "
                                                                    "oscillator in the fixed-sigma comparison")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal phase")
+    ap.add_argument("--fixed-sigma", action="store_true",
+                    help="only the ESS/s comparison at fixed sigma (harmonic and anharmonic oscillators)")
     ap.add_argument("--learned-train", type=int, default=200, help="online Adam steps before learned sampling")
     ap.add_argument("--learned-measurements", type=int, default=32, help="learned measurements (32 sweeps each)")
     ap.add_argument("--ragged-levels", default=None,
@@ -1219,6 +1232,14 @@ def main():
         return
     if args.walker_scaling:
         print(json.dumps(walker_scaling()), flush=True)
+        return
+    if args.fixed_sigma:
+        ceil = hbm_ceiling()
+        progress("ESS/s at fixed sigma: harmonic oscillator")
+        fs = {"ho": fixed_sigma("ho", HO_SIGMA)}
+        progress("ESS/s at fixed sigma: anharmonic oscillator, S=6")
+        fs["aho_S6"] = fixed_sigma("aho", args.aho_sigma, S=6, copy_GBps=ceil["ceilings"]["copy"]["GBps"])
+        print(json.dumps({"ess_at_fixed_sigma": fs}), flush=True)
         return
     if args.valu_calibration:
         from feynsimul_amd import _native
