@@ -23,7 +23,7 @@ EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "f
            "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
            "fs_source", "fs_run", "fs_set_waves_per_simd", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
            "fs_ranlux_draw", "fs_rng_bench", "fs_lp_create", "fs_lp_destroy", "fs_lp_get_info", "fs_lp_build_log", "fs_lp_pack",
-           "fs_lp_sweep", "fs_lp_train", "fs_lp_eval", "fs_lp_grad")
+           "fs_lp_sweep", "fs_lp_sweep_measure", "fs_lp_train", "fs_lp_eval", "fs_lp_grad")
 
 
 class FsConfig(ctypes.Structure):
@@ -125,6 +125,8 @@ def lib():
     L.fs_lp_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_lp_pack.argtypes = [vp, ctypes.POINTER(FsLpBuffers), vp]
     L.fs_lp_sweep.argtypes = [vp, vp, vp, ctypes.POINTER(FsLpBuffers), u64, i32, vp]
+    L.fs_lp_sweep_measure.argtypes = [vp, vp, vp, ctypes.POINTER(FsLpBuffers), ctypes.POINTER(FsBuffers), u64, i32,
+                                      i64, vp]
     L.fs_lp_train.argtypes = [vp, vp, ctypes.POINTER(FsLpBuffers), u64, vp]
     L.fs_lp_eval.argtypes = [vp, ctypes.POINTER(FsLpBuffers), vp, vp, i64, vp]
     L.fs_lp_grad.argtypes = [vp, ctypes.POINTER(FsLpBuffers), vp, vp]
@@ -331,6 +333,10 @@ class Proposal:
 
     def sweep(self, paths, accepts, bufs, sweep0, n_sweeps, stream):
         check(lib().fs_lp_sweep(self._h, paths, accepts, ctypes.byref(bufs), sweep0, n_sweeps, stream))
+
+    def sweep_measure(self, paths, accepts, bufs, kbufs, sweep0, n_sweeps, series_offset, stream):
+        check(lib().fs_lp_sweep_measure(self._h, paths, accepts, ctypes.byref(bufs), ctypes.byref(kbufs), sweep0,
+                                        n_sweeps, series_offset, stream))
 
     def train(self, paths, bufs, step, stream):
         check(lib().fs_lp_train(self._h, paths, ctypes.byref(bufs), step, stream))

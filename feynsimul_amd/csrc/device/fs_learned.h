@@ -822,16 +822,51 @@ FS_DEV void fs_lpc_unit(const fs_lp_args &a, const char *img, float *stage, long
     }
 }
 
+// One measurement of walker w's operators from this lane's slice sums part[NOPS], recorded as a
+// measure-only launch of the sampler records it (fs_wave.h): the FP32 lane sums summed over the wavefront,
+// the slice mean into opMeans (and series), the slice sum added to the FP64 Kahan pair of opTotals.
+template <class Sys> FS_DEV void fs_lpc_record(const fs_lp_args &a, long w, const float *part, int lane)
+{
+#pragma unroll
+    for (int k = 0; k < Sys::NOPS; k++) {
+        float m = part[k];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) m += __shfl_xor(m, o, 64);
+        if (lane == 0) {
+            a.opMeans[w * Sys::NOPS + k] = (double)m / (double)Sys::N;
+            if (a.opTotals != nullptr) {
+                double *tk = a.opTotals + (w * Sys::NOPS + k) * 2;
+                const double y = (double)m - tk[1];
+                const double t = tk[0] + y;
+                tk[1] = (t - tk[0]) - y;
+                tk[0] = t;
+            }
+            if (a.series != nullptr)
+                a.series[((long)a.seriesOffset * a.nWalkers + w) * Sys::NOPS + k] = m / (float)Sys::N;
+        }
+    }
+}
+// The kernel's operators can be measured by the move when it has operators and no correlator, grid bins or
+// observable histogram.
+template <class Sys> constexpr bool fs_lpc_measurable()
+{
+    return Sys::NOPS > 0 && Sys::NCORR == 0 && !Sys::BINS && Sys::NHOBS == 0;
+}
+
 // The fine pass of one walker (one wavefront) given its new coarse nodes cn [M][D] (LDS), its log q ratio lqk
 // and coarse kinetic change dk: bridges between the nodes, log alpha, the decision, and the write of an
 // accepted path into path [D][N] (global memory, or the walker's LDS copy).  Returns the decision, the same on
-// every lane.  HARM: the harmonic reference bridge (hw > 0), else the Levy bridge.
-template <class Sys, bool HARM>
+// every lane.  HARM: the harmonic reference bridge (hw > 0), else the Levy bridge.  MEAS: the operators of
+// the resulting path are recorded (fs_lpc_record), from sums over the old and the new slices.
+template <class Sys, bool HARM, bool MEAS>
 FS_DEV bool fs_lpc_fine_path_t(const fs_lp_args &a, const float *cn, float lqk, float dk, float *path, long w,
                                int lane)
 {
     typedef fs_lpc_geom<Sys> G;
-    constexpr int N = G::N, D = G::D, L = G::L, M = G::M;
+    constexpr int N = G::N, D = G::D, L = G::L, M = G::M, NO = Sys::NOPS > 0 ? Sys::NOPS : 1;
+    float po[NO], pn[NO];
+#pragma unroll
+    for (int k = 0; k < NO; k++) po[k] = pn[k] = 0.0f;
     const fs_u32 wl = (fs_u32)w;
     const int rk = fs_lpc_anchor<N>(a, w);
     const float eps = (float)Sys::EPS;
@@ -866,6 +901,7 @@ FS_DEV bool fs_lpc_fine_path_t(const fs_lp_args &a, const float *cn, float lqk, 
 #pragma unroll
             for (int d = 0; d < D; d++) xold[d] = path[d * N + ((rk + j * L + lane + 1) & (N - 1))];
             dv -= fs_lpc_veff<Sys, HARM>(xold, hw);
+            if constexpr (MEAS) Sys::ops(xold, po);
         }
         float xs[M][D];
 #pragma unroll
@@ -875,23 +911,42 @@ FS_DEV bool fs_lpc_fine_path_t(const fs_lp_args &a, const float *cn, float lqk, 
 #pragma unroll
             for (int t = 0; t < 4; t++) {
                 const int k = 4 * b + t, j = k / D, d = k % D;
-                if (k < M * D)
+                if (k < M * D) {
                     xs[j][d] = fs_lpc_bridge_x<HARM>(bc, a, fs_lp_normal(x, t >> 1, t & 1), cn[j * D + d],
                                                      cn[((j + 1) & (M - 1)) * D + d], lane);
+                    // a slice is scored as soon as its last coordinate is drawn (slices in order, as before)
+                    if (d == D - 1) dv += fs_lpc_veff<Sys, HARM>(xs[j], hw);
+                }
             }
         }
-#pragma unroll
-        for (int j = 0; j < M; j++) dv += fs_lpc_veff<Sys, HARM>(xs[j], hw);
-        if (decide(dv)) {
+        const bool acc = decide(dv);
+        if (acc) {
+            // an accepted path's operators are summed while it is written
 #pragma unroll
             for (int j = 0; j < M; j++) {
                 const int s = (rk + j * L + lane + 1) & (N - 1);
 #pragma unroll
                 for (int d = 0; d < D; d++) path[d * N + s] = xs[j][d];
+                if constexpr (MEAS) {
+                    // the slice through an empty asm, so that the operators are computed here rather than
+                    // from products kept live since the potential was scored
+                    float xt[D];
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        xt[d] = xs[j][d];
+                        asm volatile("" : "+v"(xt[d]));
+                    }
+                    Sys::ops(xt, pn);
+                }
             }
-            return true;
         }
-        return false;
+        if constexpr (MEAS) {
+            float pm[NO];
+#pragma unroll
+            for (int k = 0; k < NO; k++) pm[k] = acc ? pn[k] : po[k];
+            fs_lpc_record<Sys>(a, w, pm, lane);
+        }
+        return acc;
     } else {
         // otherwise the bridges are drawn again (the same normals) to write an accepted path
 #pragma unroll 2
@@ -904,8 +959,19 @@ FS_DEV bool fs_lpc_fine_path_t(const fs_lp_args &a, const float *cn, float lqk, 
             }
             fs_lpc_bridge<D, HARM>(cn, j, M, lane, px, wl, a, bc, xn);
             dv += fs_lpc_veff<Sys, HARM>(xn, hw) - fs_lpc_veff<Sys, HARM>(xold, hw);
+            if constexpr (MEAS) {
+                Sys::ops(xold, po);
+                Sys::ops(xn, pn);
+            }
         }
-        if (decide(dv)) {
+        const bool acc = decide(dv);
+        if constexpr (MEAS) {
+            float pm[NO];
+#pragma unroll
+            for (int k = 0; k < NO; k++) pm[k] = acc ? pn[k] : po[k];
+            fs_lpc_record<Sys>(a, w, pm, lane);
+        }
+        if (acc) {
             px = make_uint4(0u, 0u, 0u, 0u);
 #pragma unroll 2
             for (int j = 0; j < M; j++) {
@@ -924,8 +990,13 @@ FS_DEV bool fs_lpc_fine_path_t(const fs_lp_args &a, const float *cn, float lqk, 
 template <class Sys>
 FS_DEV bool fs_lpc_fine_path(const fs_lp_args &a, const float *cn, float lqk, float dk, float *path, long w, int lane)
 {
-    return a.hw > 0.0f ? fs_lpc_fine_path_t<Sys, true>(a, cn, lqk, dk, path, w, lane)
-                       : fs_lpc_fine_path_t<Sys, false>(a, cn, lqk, dk, path, w, lane);
+    if constexpr (fs_lpc_measurable<Sys>()) {
+        if (a.measure)
+            return a.hw > 0.0f ? fs_lpc_fine_path_t<Sys, true, true>(a, cn, lqk, dk, path, w, lane)
+                               : fs_lpc_fine_path_t<Sys, false, true>(a, cn, lqk, dk, path, w, lane);
+    }
+    return a.hw > 0.0f ? fs_lpc_fine_path_t<Sys, true, false>(a, cn, lqk, dk, path, w, lane)
+                       : fs_lpc_fine_path_t<Sys, false, false>(a, cn, lqk, dk, path, w, lane);
 }
 
 // The fine pass of one walker from the network kernel's workspace (new nodes, log q ratio, coarse kinetic
@@ -968,6 +1039,7 @@ FS_DEV void fs_lpc_fused_walker(const fs_lp_args &a, const char *img, float *wb,
     for (int i = lane; i < PATH; i += 64) path[i] = gp[i];
     fs_u32 nacc = 0;
     fs_lp_args am = a;
+    am.measure = 0;
 #pragma unroll 1
     for (int m = 0; m < nMoves; m++) {
         am.sweep = a.sweep + (unsigned)m;
@@ -990,6 +1062,21 @@ FS_DEV void fs_lpc_fused_walker(const fs_lp_args &a, const char *img, float *wb,
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    if constexpr (fs_lpc_measurable<Sys>()) {
+        if (a.measure) {
+            // the operators of the final path, slice by slice from LDS
+            float part[Sys::NOPS];
+#pragma unroll
+            for (int k = 0; k < Sys::NOPS; k++) part[k] = 0.0f;
+            for (int t = lane; t < N; t += 64) {
+                float x[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) x[d] = path[d * N + t];
+                Sys::ops(x, part);
+            }
+            fs_lpc_record<Sys>(a, w, part, lane);
+        }
+    }
     for (int i = lane; i < PATH; i += 64) gp[i] = path[i];
     if (lane == 0) a.accepts[w] += nacc;
 }

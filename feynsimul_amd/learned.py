@@ -162,6 +162,9 @@ class LearnedProposal:
         self._step = 0
         self._sweepCount = 0
         self.fallbackAcceptance = float(fallbackAcceptance)
+        # run(): the whole-path move records each block's measurement itself when it can (False: a
+        # measure-only launch of the kernel after every block)
+        self.measureInMove = True
         self.lastAcceptance = None
         self.fallbackBlocks = 0
         self.setParameters(_he_init(self._dof, initSeed, active=self.width))
@@ -237,11 +240,19 @@ class LearnedProposal:
         does not wait for a block: its acceptance is copied back while the next block runs, and a
         block that follows one whose learned acceptance fell below fallbackAcceptance (for the first
         block: the last block of the previous call) adds plain bisection sweeps of the same length
-        before its measurement."""
+        before its measurement.  The whole-path move of a wave-sampler kernel that measures operators
+        only makes the measurement itself, in its last move of the block, instead of a measure-only
+        launch of the kernel (the same quantities; slice sums in another order)."""
         k = self._k
         opRuns = int(operatorRuns if operatorRuns is not None else k._operatorRuns)
         M = int(sweepsPerMeasurement if sweepsPerMeasurement is not None else k._metroStepsPerOperatorRun)
         moves = float(self._movesPerSweep * M)
+        # the whole-path move records the measurement itself (fs_lp_sweep_measure) when the kernel's
+        # measurement is operators only; otherwise a measure-only launch of the kernel follows each block
+        c = k._cfg
+        selfMeasure = (self.measureInMove and self.coarse and k._sampler == "wave" and M > 0 and c.n_ops > 0
+                       and c.n_corr == 0 and not c.bins and c.n_hist_obs == 0)
+        kb = k._buffers() if selfMeasure else None
         low = self.lastAcceptance is not None and self.lastAcceptance < self.fallbackAcceptance
         # accept counts accumulate over the call's blocks; a block's acceptance is the difference of the
         # running totals copied back after it and after the block before it
@@ -259,14 +270,22 @@ class LearnedProposal:
                 self.lastAcceptance = (total - done) / moves
                 done = total
                 low = self.lastAcceptance < self.fallbackAcceptance
-            self._P.sweep(paths, acc.data_ptr(), bufs, self._sweepCount, M, stream)
+            if selfMeasure and not low:
+                if k._series is not None and k._seriesCount + 1 > k._series.shape[0]:
+                    raise Exception("series buffer full: %d measurements stored, capacity %d"
+                                    % (k._seriesCount, k._series.shape[0]))
+                self._P.sweep_measure(paths, acc.data_ptr(), bufs, kb, self._sweepCount, M,
+                                      k._seriesCount if k._series is not None else 0, stream)
+                k._account_run(1, 0)
+            else:
+                self._P.sweep(paths, acc.data_ptr(), bufs, self._sweepCount, M, stream)
             self._sweepCount += M
             self._accHost[i].copy_(acc.sum(), non_blocking=True)
             self._accEv[i].record(k._stream)
             if low:
                 self.fallbackBlocks += 1
                 k._launch(1, M)
-            else:
+            elif not selfMeasure:
                 k._launch(1, 0)
         k._stream.synchronize()
         if opRuns > 0:

@@ -316,6 +316,13 @@ int fs_lp_pack(fs_lp *p, const fs_lp_buffers *b, void *stream);
  * The whole-path move of a path of at most 1024 floats makes all n_sweeps moves in one launch. */
 int fs_lp_sweep(fs_lp *p, void *paths, void *accepts, const fs_lp_buffers *b, uint64_t sweep0, int32_t n_sweeps,
                 void *stream);
+/* fs_lp_sweep of the whole-path move (n_sweeps >= 1 moves), whose last move also records one measurement of the
+ * kernel's operators on the resulting paths into kb (op_means, op_totals, series at series_offset, the buffers
+ * of fs_run), as fs_run(operator_runs = 1, metro_steps = 0) records it, without that launch's read of every
+ * path; for kernels with operators and no correlators, bins or observable histograms.  No counterpart in the
+ * reference (its measurement follows its own sweeps, kernel.c:478-506). */
+int fs_lp_sweep_measure(fs_lp *p, void *paths, void *accepts, const fs_lp_buffers *b, const fs_buffers *kb,
+                        uint64_t sweep0, int32_t n_sweeps, int64_t series_offset, void *stream);
 /* One Adam step (step counts from 1) on `batch` rows drawn from paths, then a repack. */
 int fs_lp_train(fs_lp *p, const void *paths, const fs_lp_buffers *b, uint64_t step, void *stream);
 /* The network alone: X [rows][32] FP32 features -> O [rows][32] FP32 (logical outputs 0 .. 2 DOF - 1). */

@@ -320,3 +320,35 @@ def test_harmonic_reference_of_the_oscillator_leaves_no_fine_action(gpu):
     tr = lp.getTrace()[:, 0, :]
     print("max |dV| %.3e, max |dK| %.3e" % (np.abs(tr[:, 2]).max(), np.abs(tr[:, 1]).max()))
     assert np.abs(tr[:, 2]).max() < 1e-3 * max(1.0, np.abs(tr[:, 1]).max())
+
+
+# the two-kernel move with registers holding the new slices (aho2048) and with the bridges drawn twice
+# (he3_512), the fused move (ho256), each with the Levy bridge and a harmonic reference
+@pytest.mark.parametrize("case,omega2", [("aho2048", 0.0), ("aho2048", 1.5), ("he3_512", 0.0), ("he3_512", 0.2),
+                                         ("ho256", 0.0), ("ho256", 1.0)])
+def test_measurement_in_the_move_matches_the_measure_only_launch(gpu, case, omega2):
+    """LearnedProposal.run with the measurement made by each block's last move (fs_lp_sweep_measure) against
+    the measure-only launch of the kernel after each block: the same paths and accept counts bit for bit,
+    and the same series, operator means and totals up to the order of the FP32 slice sums."""
+    _case("whole-path move %s omega2 %s: measurement in the move" % (case, omega2))
+    rng = np.random.default_rng(31)
+    system, W, N, beta, D, paths0, V, scale = _setup(case, rng)
+    theta = L.init_params(D, 5, scale_last=scale, active=32)
+    out = []
+    for inMove in (False, True):
+        k = _kernel(system, W, N, beta)
+        k.setPaths(paths0.reshape(W, D * N))
+        k.setSeriesCapacity(8)
+        lp = LearnedProposal(k, batch=2048, seed=41, coarse=True, width=32, bridgeOmega2=omega2,
+                             fallbackAcceptance=0.0)
+        lp.setParameters(theta)
+        lp.measureInMove = inMove
+        lp.run(3, 2)
+        out.append((k.getPaths(), lp._runAcc.cpu().numpy(), k.getSeries(), k.getOperators(),
+                    k.getOperatorTotals()))
+        del lp, k
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2].shape == out[1][2].shape == (3, W, 1)
+    for i in (2, 3, 4):
+        np.testing.assert_allclose(out[1][i], out[0][i], rtol=2e-5, atol=1e-6 * np.abs(out[0][i]).max())
