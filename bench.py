@@ -385,12 +385,12 @@ def lp_bench(steps, warmup, spw=0):
                          "updates_per_launch": UPDATES_PER_SWEEP, "traffic": None}}
 
 
-def lpc_bench(steps, warmup, width=None, train_steps=500):
+def lpc_bench(steps, warmup, width=None, train_steps=None):
     """The learned whole-path move alone on the anharmonic workload (x^2/2 + 0.1 x^4, N = 4096,
     beta = 40, 65536 chains): `steps` moves timed with HIP events on the kernel's stream after
-    `warmup`, from paths 2048 plain sweeps (S = 6) into the run, with the network trained online for
-    `train_steps` Adam steps.  One move = the network kernel on the coarse nodes (fs_lp_sweep) and
-    the fine pass (fs_lpc_fine, Levy bridges, decision, write)."""
+    `warmup`, from paths 2048 plain sweeps (S = 6) into the run, with the network trained as the
+    fixed-sigma comparison trains it.  One move = the network kernel on the coarse nodes (fs_lp_sweep)
+    and the fine pass (fs_lpc_fine, harmonic reference bridges, decision, write)."""
     import numpy as np
     import torch
     from feynsimul_amd.kernel_args import KernelArgs
@@ -409,9 +409,12 @@ def lpc_bench(steps, warmup, width=None, train_steps=500):
     k.run()
     lp = LearnedProposal(k, batch=65536, lr=1e-3, seed=44, coarse=True, width=width,
                          bridgeOmega2=FIXED_SIGMA_LEARNED["aho"]["bridge"])
+    cfg = FIXED_SIGMA_LEARNED["aho"]
+    train_steps = train_steps or cfg["train_steps"]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    lp.trainOnline(train_steps, plainSweepsPerStep=1)
+    lp.trainOnline(train_steps, plainSweepsPerStep=cfg["plain_sweeps_per_step"])
+    torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     # Adam steps alone (no plain sweeps between them), for the cost of one training step
     t0 = time.perf_counter()
@@ -437,7 +440,8 @@ def lpc_bench(steps, warmup, width=None, train_steps=500):
                                    "one whole-path move per step, %d hidden units, %d Adam steps, harmonic "
                                    "reference bridges (omega^2 %.4g)" % (width, train_steps, lp.bridgeOmega2)},
             "acceptance": ar,
-            "training": {"adam_steps": train_steps, "plain_sweeps_per_step": 1, "seconds": t_train,
+            "training": {"adam_steps": train_steps, "plain_sweeps_per_step": cfg["plain_sweeps_per_step"],
+                         "seconds": t_train,
                          "ms_per_adam_step": ms_adam, "batch": 65536},
             "roofline": {"bound": "hbm", "achieved": algo / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": "fs_lp_sweep + fs_lpc_fine",
@@ -498,13 +502,13 @@ def run_to_sigma(block, get_series, sigma, exact, max_seconds, measurements_per_
     return st
 
 
-# the learned leg of the fixed-sigma comparison: the whole-path move, its network on 32 hidden units,
-# trained online (Adam steps, each after plain sweeps)
-# (one plain sweep per Adam step: on the anharmonic workload 500 steps reach the acceptance of 500 steps
-# after two sweeps each, 0.41 against 0.42, in 0.47 s of training instead of 0.76 s)
+# the learned leg of the fixed-sigma comparison: the whole-path move, its network on 32 hidden units, trained
+# with Adam on rows drawn from the equilibrated paths of the plain sampler.  Plain sweeps between the steps buy
+# nothing here: on the anharmonic workload 500 steps reach acceptance 0.630 after one sweep each (0.47 s) and
+# 0.629 without (0.066 s); 1200 steps 0.77, 2000 steps 0.81 (0.26 s), 3000 steps 0.82.
 # bridge: the whole-path move's harmonic reference, omega^2 = <x . grad V> / <|x|^2> of the start paths
-FIXED_SIGMA_LEARNED = {"ho": {"train_steps": 300, "batch": 16384, "plain_sweeps_per_step": 1, "bridge": "auto"},
-                       "aho": {"train_steps": 500, "batch": 65536, "plain_sweeps_per_step": 1, "bridge": "auto"}}
+FIXED_SIGMA_LEARNED = {"ho": {"train_steps": 300, "batch": 16384, "plain_sweeps_per_step": 0, "bridge": "auto"},
+                       "aho": {"train_steps": 2000, "batch": 65536, "plain_sweeps_per_step": 0, "bridge": "auto"}}
 LEARNED_WIDTH = 32
 # Measurement cadences each leg is run at (sweeps or moves per E0 measurement): a measurement costs about
 # as much as reading every path once, so measuring after every sweep wastes time, while measuring far

@@ -305,8 +305,11 @@ class LearnedProposal:
     @_on_engine_stream
     def trainOnline(self, steps, plainSweepsPerStep=2):
         """Online training: each Adam step follows plainSweepsPerStep plain bisection sweeps of
-        the kernel, whose paths supply its rows.  Returns the mean NLL of the last step."""
+        the kernel, whose paths supply its rows (0: every step draws its rows from the current
+        paths, as train does).  Returns the mean NLL of the last step."""
         k = self._k
+        if int(plainSweepsPerStep) <= 0:
+            return self.train(steps)
         for _ in range(int(steps)):
             k._launch(1, int(plainSweepsPerStep))
             self._step += 1
