@@ -227,12 +227,31 @@ def test_learned_module_compiles_for_gfx950(name, N, S, spw):
     ("batch", 1000, "multiple of 2048"),
     ("atoms3d", 1, "atoms3d needs DOF"),
     ("abi_version", 7, "ABI version"),
+    ("bridge_omega2", -1.0, "bridge_omega2 must be finite and >= 0"),
+    ("bridge_omega2", float("nan"), "bridge_omega2 must be finite and >= 0"),
+    ("bridge_omega2", 1.0, "bridge_omega2 applies to the whole-path move"),
 ])
 def test_learned_config_rejected(field, value, msg):
     from feynsimul_amd import _native
     with pytest.raises(_native.FsError, match=re.escape(msg)):
         _native.Proposal(_cfg(HarmOsc(), 64, 3, 1), system_section(HarmOsc(), ("x1",)), _lp(**{field: value}),
                          compile_only=True)
+
+
+def test_move_measurement_entry_checks_its_arguments():
+    """fs_lp_sweep_measure needs a proposal that was built for the device and the kernel's buffers (what it
+    refuses on the device: the GPU tests of the whole-path move)."""
+    import ctypes
+    from feynsimul_amd import _native
+    L = _native.lib()
+    ho = HarmOsc()
+    p = _native.Proposal(_cfg(ho, 256, 6, 1), system_section(ho, (ho.energyOp,)), _lp(coarse=1), compile_only=True)
+    b, kb = _native.FsLpBuffers(), _native.FsBuffers()
+    assert L.fs_lp_sweep_measure(p._h, 8, 8, ctypes.byref(b), None, 0, 1, 0, None) != 0
+    assert "kernel's buffers are required" in L.fs_last_error().decode()
+    assert L.fs_lp_sweep_measure(p._h, 8, 8, ctypes.byref(b), ctypes.byref(kb), 0, 1, 0, None) != 0
+    assert "compile-only" in L.fs_last_error().decode()
+    p.close()
 
 
 def test_learned_rejects_double_and_deep_levels():

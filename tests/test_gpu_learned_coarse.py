@@ -352,3 +352,33 @@ def test_measurement_in_the_move_matches_the_measure_only_launch(gpu, case, omeg
     assert out[0][2].shape == out[1][2].shape == (3, W, 1)
     for i in (2, 3, 4):
         np.testing.assert_allclose(out[1][i], out[0][i], rtol=2e-5, atol=1e-6 * np.abs(out[0][i]).max())
+
+
+def test_measurement_in_the_move_is_refused_where_it_cannot_record(gpu):
+    """fs_lp_sweep_measure refuses the segment proposal, kernels that also measure a correlator, and a call
+    without a move; LearnedProposal.run then keeps the measure-only launch."""
+    from feynsimul_amd import _native
+    _case("whole-path move: measurement refusals")
+    system, W, N, beta = HarmOsc(), 64, 256, 4.0
+    k = _kernel(system, W, N, beta)
+    k.setSeriesCapacity(4)
+    seg = LearnedProposal(k, batch=2048, seed=1, coarse=False)
+    with pytest.raises(_native.FsError, match="coarse = 1"):
+        seg._P.sweep_measure(k._paths.data_ptr(), seg._acc.data_ptr(), seg._bufs(), k._buffers(), 0, 1, 0,
+                             k._stream.cuda_stream)
+    lp = LearnedProposal(k, batch=2048, seed=1, coarse=True, width=32)
+    with pytest.raises(_native.FsError, match="at least one move"):
+        lp._P.sweep_measure(k._paths.data_ptr(), lp._acc.data_ptr(), lp._bufs(), k._buffers(), 0, 0, 0,
+                            k._stream.cuda_stream)
+    kc = PIMCKernel(KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=beta, S=6, enableOperator=True,
+                               enableCorrelator=True, enableBisection=True, enablePathShift=False,
+                               enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                               enableGlobalOldPath=False, enableBins=False, metroStepsPerOperatorRun=1,
+                               operatorRuns=1, nbrOfWalkersPerWorkGroup=1, operators=(system.energyOp,),
+                               correlators=("x1",), seed=3, sampler="wave"))
+    lc = LearnedProposal(kc, batch=2048, seed=1, coarse=True, width=32)
+    with pytest.raises(_native.FsError, match="no correlators"):
+        lc._P.sweep_measure(kc._paths.data_ptr(), lc._acc.data_ptr(), lc._bufs(), kc._buffers(), 0, 1, 0,
+                            kc._stream.cuda_stream)
+    lc.run(2, 1)  # the measure-only launch: the correlator is measured
+    assert kc._corRuns == 2
