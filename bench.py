@@ -672,6 +672,9 @@ def fixed_sigma(workload, sigma, max_seconds=60.0, S=None, copy_GBps=None):
             "ratio_learned_over_plain": learned["ess_per_s"] / plain["ess_per_s"],
             "ratio_without_training": learned["ess_per_s_without_training"] / plain["ess_per_s"],
             "ess_per_move_over_ess_per_plain_sweep": per[1] / per[0],
+            # against a plain sampler that measured after every sweep at no cost: W chains / (tau_int in sweeps
+            # x ms per sweep), the most any measurement cadence could give it
+            "ratio_vs_plain_with_free_measurement": learned["ess_per_s"] / (W / (tau_sweeps * ms_sweep * 1e-3)),
             "ceiling": ceiling,
             "bias_within_2_sigma": {"plain": plain["z"] <= 2.0, "learned": learned["z"] <= 2.0}}
     if full is not None:
