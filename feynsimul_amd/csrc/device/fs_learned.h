@@ -569,11 +569,11 @@ template <bool HARM> FS_DEV fs_lpc_brg fs_lpc_bridge_coefs(const fs_lp_args &a, 
     c.s = HARM ? a.bs : sqrtf(eps);
     c.wb1 = c.wb2 = c.pk = c.g = 1.0f;
     if constexpr (HARM) {
-        const float l = a.l2q, k = (float)(lane + 1);
+        const float l = a.l2q;
         c.wb1 = exp2f((float)((lane & 15) + 1) * l);
         c.wb2 = exp2f((float)(lane > 31 ? lane - 31 : 0) * l);
-        c.pk = exp2f(k * l);
-        c.g = exp2f(((float)FS_LPC_L - k) * l) * (1.0f - exp2f(2.0f * k * l)) / (1.0f - exp2f(2.0f * (float)FS_LPC_L * l));
+        c.pk = a.brg[lane];
+        c.g = a.brg[FS_LPC_L + lane];
     }
     return c;
 }

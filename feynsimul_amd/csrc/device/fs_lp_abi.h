@@ -37,6 +37,7 @@ struct fs_lp_args {
     // factor rho, its powers q1 = rho, q2, q4, q8 and rhoL = rho^64, log2 rho, bs = sqrt(rho eps) and
     // kc = 1 / (2 V_64), V_64 the variance of a node given the one before under the reference
     float hw, bs, q1, q2, q4, q8, rhoL, kc, l2q;
+    const float *brg;       // [2][64]: rho^k and the end weights g_k of the bridge's slices k = 1 .. 64
     // whole-path move, measure != 0: after its move the fine pass (the fused move: after its last move) records one
     // measurement of the kernel's operators on the resulting path, as a measure-only fs_run does: opMeans
     // [walkers][NOPS] = slice mean, opTotals [walkers][NOPS][2] += slice sum (Kahan), series [seriesOffset]

@@ -399,7 +399,8 @@ def bridge(cn, z, eps, omega2=0.0):
         tot = v[:, :, FINE_L - 1:FINE_L, :]
         k = np.arange(1, FINE_L + 1, dtype=np.float64)
         pk = (rho ** k).astype(np.float32)[None, None, :, None]
-        g = (rho ** (FINE_L - k) * (1.0 - rho ** (2 * k)) / (1.0 - rho ** (2 * FINE_L))).astype(np.float32)
+        lam = -np.log(rho)
+        g = (np.sinh(lam * k) / np.sinh(lam * FINE_L)).astype(np.float32)    # rho^(64-k) (1 - rho^2k) / (1 - rho^128)
         g = g[None, None, :, None]
         x = (ca * pk + v + g * (cb - ca * np.float32(rhoL) - tot)).astype(np.float32)
     x[:, :, FINE_L - 1, :] = cb[:, :, 0, :]
