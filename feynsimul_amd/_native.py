@@ -10,6 +10,7 @@ import os
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfeynsimul.so")
 
 FS_ABI_VERSION = 6
+FS_RUN_POT_CACHE_VALID = 1  # fs_run_flags: the potential cache is current
 FS_OK, FS_ERR_INVALID, FS_ERR_COMPILE, FS_ERR_HIP, FS_ERR_NO_DEVICE = 0, 1, 2, 3, 4
 FS_SAMPLER_EXACT, FS_SAMPLER_WAVE = 0, 1
 FS_RNG_XORSHIFT, FS_RNG_RANLUX = 0, 1
@@ -21,7 +22,7 @@ EXPORTS = ("fs_abi_version", "fs_last_error", "fs_device_count", "fs_create", "f
            "fs_storage_of", "fs_launch_info", "fs_hbm_probe",
            "fs_batch_run_noalloc",
            "fs_batch_capture", "fs_graph_launch", "fs_graph_destroy", "fs_build_log",
-           "fs_source", "fs_run", "fs_set_waves_per_simd", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
+           "fs_source", "fs_run", "fs_run_flags", "fs_set_waves_per_simd", "fs_eval_system", "fs_seed_xorshift", "fs_ranlux_init", "fs_xorshift_draw",
            "fs_ranlux_draw", "fs_rng_bench", "fs_lp_create", "fs_lp_destroy", "fs_lp_get_info", "fs_lp_build_log", "fs_lp_pack",
            "fs_lp_sweep", "fs_lp_sweep_measure", "fs_lp_train", "fs_lp_eval", "fs_lp_grad")
 
@@ -111,6 +112,7 @@ def lib():
     L.fs_build_log.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_source.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
     L.fs_run.argtypes = [vp, ctypes.POINTER(FsBuffers), i32, i32, i32, vp]
+    L.fs_run_flags.argtypes = [vp, ctypes.POINTER(FsBuffers), i32, i32, i32, i32, vp]
     L.fs_set_waves_per_simd.argtypes = [vp, i32]
     L.fs_eval_system.argtypes = [vp, vp, vp, vp, i64, vp]
     L.fs_seed_xorshift.argtypes = [vp, i64, u64, i64, vp]
@@ -193,8 +195,9 @@ class Engine:
     def set_waves_per_simd(self, cap):
         check(lib().fs_set_waves_per_simd(self._h, int(cap)))
 
-    def run(self, bufs: FsBuffers, operator_runs, metro_steps, series_offset, stream):
-        check(lib().fs_run(self._h, ctypes.byref(bufs), operator_runs, metro_steps, series_offset, stream))
+    def run(self, bufs: FsBuffers, operator_runs, metro_steps, series_offset, stream, flags=0):
+        check(lib().fs_run_flags(self._h, ctypes.byref(bufs), operator_runs, metro_steps, series_offset, flags,
+                                 stream))
 
     def eval_system(self, x_ptr, pot_ptr, ops_ptr, n, stream):
         check(lib().fs_eval_system(self._h, x_ptr, pot_ptr, ops_ptr, n, stream))

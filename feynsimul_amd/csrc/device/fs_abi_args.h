@@ -24,8 +24,9 @@ struct fs_kargs {
     int operatorRuns;
     int metroSteps;
     int seriesOffset;     // index of the first measurement of this launch in `series`
-    int flags;            // reserved (0)
+    int flags;            // FS_KF_POT_VALID: potCache already holds V at every slice of the current paths
 };
+#define FS_KF_POT_VALID 1
 
 // Correlator staging of the wave sampler (shared by the host, which sizes the
 // dynamic LDS, and the device code).  A lane owns fs_cor_lanelags(lags)

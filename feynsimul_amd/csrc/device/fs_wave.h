@@ -179,7 +179,8 @@ FS_DEV void fs_wave_walker(const fs_kargs &a, const long wIn, const int lane, ch
             for (int d = 0; d < DOF; d++) xs[d][k] = x[d];
             vc[k] = Sys::V(x);
         }
-    } else if (Sys::POTCACHE && active) {
+    } else if (Sys::POTCACHE && active && !(a.flags & FS_KF_POT_VALID)) {
+        // (skipped when the caller vouches that the cache still holds the potentials the previous launch left)
         // lane j only ever touches the cache entries of slices congruent to j mod B
 #pragma unroll 1
         for (int k = 0; k < NB; k++) {

@@ -489,6 +489,13 @@ int fs_source(const fs_engine *e, char *buf, size_t len, size_t *needed)
 int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t metro_steps, int32_t series_offset,
            void *stream)
 {
+    return fs_run_flags(e, b, operator_runs, metro_steps, series_offset, 0, stream);
+}
+
+int fs_run_flags(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t metro_steps,
+                 int32_t series_offset, int32_t flags, void *stream)
+{
+    if (flags & ~FS_RUN_POT_CACHE_VALID) return fail(FS_ERR_INVALID, "unknown run flags 0x%x", flags);
     if (!e || !b) return fail(FS_ERR_INVALID, "engine or buffers is NULL");
     if (!e->module) return fail(FS_ERR_INVALID, "engine was created compile-only");
     if (operator_runs < 0 || metro_steps < 0) return fail(FS_ERR_INVALID, "negative run length");
@@ -523,6 +530,7 @@ int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t met
     a.operatorRuns = operator_runs;
     a.metroSteps = metro_steps;
     a.seriesOffset = series_offset;
+    a.flags = (flags & FS_RUN_POT_CACHE_VALID) ? FS_KF_POT_VALID : 0;
     void *args[] = {&a};
 
     if (c.sampler == FS_SAMPLER_EXACT) {

@@ -373,6 +373,9 @@ class PIMCKernel:
                 self._seedsSnap = self._zeros((rows, 4), t.int32)
         self._oldPath = self._zeros((threads, (2 ** self._S - 1) * DOF), ft) if exact else None
         self._potCache = None
+        # the potential cache holds V at every slice of the current paths (the last launch left it so and
+        # nothing has written the paths since): the next launch need not recompute it
+        self._potValid = False
         if not exact and self._storage == _native.FS_STORAGE_MEMORY:
             self._potCache = self._zeros((W, N), ft)
         self._opTotals = None
@@ -457,9 +460,11 @@ class PIMCKernel:
                                 % (self._seriesCount, self._series.shape[0]))
             offset = self._seriesCount
         bufs = self._buffers()
+        flags = _native.FS_RUN_POT_CACHE_VALID if self._potValid else 0
         self._ev0.record(self._stream)
-        self._engine.run(bufs, opRuns, M, offset, self._stream.cuda_stream)
+        self._engine.run(bufs, opRuns, M, offset, self._stream.cuda_stream, flags)
         self._ev1.record(self._stream)
+        self._potValid = self._potCache is not None
         self._account_run(opRuns, M)
 
     @_on_engine_stream
@@ -504,6 +509,7 @@ class PIMCKernel:
         a = np.asarray(paths).astype(np.float64 if self._enableDouble else np.float32)
         a = a.reshape(self._paths.shape)
         self._paths.copy_(self._torch.from_numpy(np.ascontiguousarray(a)))
+        self._potValid = False
         self._stream.synchronize()
 
     @_on_engine_stream

@@ -219,6 +219,7 @@ class LearnedProposal:
         self._acc.zero_()
         self._P.sweep(k._paths.data_ptr(), self._acc.data_ptr(), self._bufs(), self._sweepCount, int(nSweeps),
                       k._stream.cuda_stream)
+        k._potValid = False  # the moves change the paths without the kernel's potential cache
         self._sweepCount += int(nSweeps)
         if not wait:
             return None
@@ -280,6 +281,7 @@ class LearnedProposal:
             else:
                 self._P.sweep(paths, acc.data_ptr(), bufs, self._sweepCount, M, stream)
             self._sweepCount += M
+            k._potValid = False
             self._accHost[i].copy_(acc.sum(), non_blocking=True)
             self._accEv[i].record(k._stream)
             if low:

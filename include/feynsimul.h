@@ -174,6 +174,14 @@ int fs_launch_info(const fs_engine *e, fs_launch_info_t *info);
 /* One launch of operator_runs x metro_steps sweeps (the reference `run`). */
 int fs_run(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t metro_steps,
            int32_t series_offset, void *stream);
+/* fs_run with flags.  FS_RUN_POT_CACHE_VALID: the caller vouches that b->pot_cache still holds the potential
+ * at every slice of b->paths, as the previous launch of this engine on these buffers left it (the paths have
+ * not been written since); the wave sampler with memory storage then skips recomputing it at the start of
+ * the launch (a pass over every path; the results are the same bit for bit).  No counterpart in the
+ * reference, which evaluates the potential afresh for every move (kernel.c:376-377). */
+#define FS_RUN_POT_CACHE_VALID 1
+int fs_run_flags(fs_engine *e, const fs_buffers *b, int32_t operator_runs, int32_t metro_steps,
+                 int32_t series_offset, int32_t flags, void *stream);
 
 /* Evaluate the compiled potential and operators at n points x[n][DOF]
  * (FLOAT); writes f64 pot[n] and ops[n][n_ops]. */

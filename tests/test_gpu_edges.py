@@ -155,3 +155,38 @@ def test_calls_under_another_current_stream(gpu):
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert out[0][2:] == out[1][2:]
+
+
+def test_potential_cache_kept_across_launches(gpu):
+    """Memory storage keeps the potential of every slice in HBM; consecutive launches on unchanged paths reuse
+    it (FS_RUN_POT_CACHE_VALID) instead of recomputing it at the start of each launch.  The results equal
+    those of launches that recompute it, bit for bit, and setPaths invalidates it (new paths are not scored
+    with the old paths' potentials)."""
+    _case("potential cache across launches")
+    system = Lm2m2Cluster(3)
+    W, N = 64, 128
+    rng = np.random.default_rng(8)
+    p0, p1 = _paths(system, W, N, rng), _paths(system, W, N, rng)
+    out = []
+    for keep in (True, False):
+        k = PIMCKernel(KernelArgs(system=system, nbrOfWalkers=W, N=N, beta=5.0, S=6, enableOperator=True,
+                                  enableCorrelator=False, enableBisection=True, enablePathShift=False,
+                                  enableSingleNodeMove=False, enableParallelizePath=True, enableGlobalPath=False,
+                                  enableGlobalOldPath=False, enableBins=False, metroStepsPerOperatorRun=4,
+                                  operatorRuns=2, nbrOfWalkersPerWorkGroup=1, operators=(system.energyOp,), seed=21,
+                                  sampler="wave", pathStorage="memory", potentialCache=True))
+        assert k._potCache is not None
+        k.setPaths(p0)
+        got = []
+        for paths in (None, None, None, p1, None):
+            if paths is not None:
+                k.setPaths(paths)
+            if not keep:
+                k._potValid = False
+            k.run()
+            got.append((k.getPaths(), k.getOperators()))
+        out.append(got)
+        del k
+    for (pa, oa), (pb, ob) in zip(*out):
+        assert np.array_equal(pa, pb)
+        assert np.array_equal(oa, ob)
